@@ -1,0 +1,336 @@
+/*
+ * Oracle solver and narrow phase, written once for a real type R with suffix
+ * S; included by oracle.c for float (S=f) and double (S=d).  TEST
+ * INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Floating-point op order is spelled out with explicit parentheses and the
+ * file is compiled with -ffp-contract=off, so the float instantiation is an
+ * exact IEEE restatement of each formula.
+ */
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(CAT(orc_, name), CAT(_, S))
+#define SQRT CAT(sqrt_, S)
+
+static inline R CAT(dot3_, S)(const R* a, const R* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static inline void CAT(sub3_, S)(const R* a, const R* b, R* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+static inline void CAT(cross3_, S)(const R* a, const R* b, R* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+#define DOT CAT(dot3_, S)
+#define SUB CAT(sub3_, S)
+#define CROSS CAT(cross3_, S)
+
+/* XPBD.py:115-118: y = x + f*v*dt + g*dt*dt, evaluated left to right. */
+void FN(compute_y)(int nv, const R* x, const R* v, const R* fixed, R* y, R dt, const R* g) {
+  for (int i = 0; i < nv; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      R t = (fixed[i] * v[3 * i + c]) * dt;
+      R gg = (g[c] * dt) * dt;
+      y[3 * i + c] = (x[3 * i + c] + t) + gg;
+    }
+  }
+}
+
+/* One distance constraint of solve_spring_constraints_x (XPBD.py:146-184),
+   scattered into dx/nc in the serial order of the reference loop. */
+static inline void CAT(spring_one_, S)(int v0, int v1, R l0, const R* p, R k, const R* fixed,
+                                      const R* m_inv, R* dx, R* nc) {
+  R d[3];
+  SUB(p + 3 * v0, p + 3 * v1, d);
+  R len = SQRT(DOT(d, d));
+  R C = len - l0;
+  R inv = (R)1 / len;
+  R n[3] = {inv * d[0], inv * d[1], inv * d[2]};
+  R fw0 = fixed[v0] * m_inv[v0], fw1 = fixed[v1] * m_inv[v1];
+  R schur = (fw0 + fw1) * DOT(n, n);
+  R ld = (k * C) / ((k * schur) + (R)1);
+  for (int c = 0; c < 3; ++c) {
+    dx[3 * v0 + c] = dx[3 * v0 + c] - (fw0 * ld) * n[c];
+    dx[3 * v1 + c] = dx[3 * v1 + c] + (fw1 * ld) * n[c];
+  }
+  nc[v0] = nc[v0] + (R)1;
+  nc[v1] = nc[v1] + (R)1;
+}
+
+/* XPBD.py:383-387 update_dx: y += f * (dx / nc) where nc > 0. */
+static void CAT(update_dx_, S)(int nv, R* y, const R* fixed, const R* dx, const R* nc) {
+  for (int i = 0; i < nv; ++i) {
+    if (nc[i] > (R)0) {
+      for (int c = 0; c < 3; ++c) y[3 * i + c] = y[3 * i + c] + fixed[i] * (dx[3 * i + c] / nc[i]);
+    }
+  }
+}
+
+/* XPBD.py:350-359: init_variables, solve_spring_constraints_x(k_s, k_b), update_dx.
+   Stretch constraints (i < ne) act on y, bending constraints on x (XPBD.py:169). */
+void FN(spring_pass)(int nv, int ne, int nb, const int* edges, const R* l0, const int* bend,
+                     const R* bl0, const R* x, R* y, const R* fixed, const R* m_inv, R k_s, R k_b,
+                     R* dx, R* nc) {
+  for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  for (int i = 0; i < nv; ++i) nc[i] = 0;
+  for (int i = 0; i < ne; ++i)
+    CAT(spring_one_, S)(edges[2 * i], edges[2 * i + 1], l0[i], y, k_s, fixed, m_inv, dx, nc);
+  for (int i = 0; i < nb; ++i)
+    CAT(spring_one_, S)(bend[2 * i], bend[2 * i + 1], bl0[i], x, k_b, fixed, m_inv, dx, nc);
+  CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+}
+
+/* XPBD.py:340-343 then 332-337:
+   v = (1 - damping) * f * (y - x) / dt;  x += dt * v. */
+void FN(update_vx)(int nv, R* x, R* v, const R* y, const R* fixed, R damping, R dt) {
+  R a = (R)1 - damping;
+  for (int i = 0; i < nv; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      R vv = ((a * fixed[i]) * (y[3 * i + c] - x[3 * i + c])) / dt;
+      v[3 * i + c] = vv;
+      x[3 * i + c] = x[3 * i + c] + dt * vv;
+    }
+  }
+}
+
+/* ---------------- narrow phase (distance.py, collision_constraints_x.py) -------- */
+
+/* Edge-region parameters of p against the edge a->b of a triangle with normal n:
+   solve [e.e, e.en; en.e, en.en] [s t]^T = [e.r, en.r] with e = b - a,
+   en = e x n, r = p - a, via the explicit 2x2 inverse (distance.py:7-17). */
+static inline void CAT(edge_param_, S)(const R* p, const R* a, const R* b, const R* n, R* s, R* t) {
+  R e[3], en[3], r[3];
+  SUB(b, a, e);
+  CROSS(e, n, en);
+  SUB(p, a, r);
+  R m00 = DOT(e, e), m01 = DOT(e, en), m10 = DOT(en, e), m11 = DOT(en, en);
+  R det = m00 * m11 - m01 * m10;
+  R id = (R)1 / det;
+  R i00 = id * m11, i01 = id * (-m01), i10 = id * (-m10), i11 = id * m00;
+  R b0 = DOT(e, r), b1 = DOT(en, r);
+  *s = i00 * b0 + i01 * b1;
+  *t = i10 * b0 + i11 * b1;
+}
+
+/* distance.py:4-47 d_type_PT: 0/1/2 vertex t0/t1/t2, 3/4/5 edge t0t1/t1t2/t2t0,
+   6 face interior. */
+int FN(d_type_pt)(const R* p, const R* t0, const R* t1, const R* t2) {
+  R e1[3], e2[3], n[3];
+  SUB(t1, t0, e1);
+  SUB(t2, t0, e2);
+  CROSS(e1, e2, n);
+  R s0, u0, s1, u1, s2, u2;
+  CAT(edge_param_, S)(p, t0, t1, n, &s0, &u0);
+  if (s0 > (R)0 && s0 < (R)1 && u0 >= (R)0) return 3;
+  CAT(edge_param_, S)(p, t1, t2, n, &s1, &u1);
+  if (s1 > (R)0 && s1 < (R)1 && u1 >= (R)0) return 4;
+  CAT(edge_param_, S)(p, t2, t0, n, &s2, &u2);
+  if (s2 > (R)0 && s2 < (R)1 && u2 >= (R)0) return 5;
+  if (s0 <= (R)0 && s2 >= (R)1) return 0;
+  if (s1 <= (R)0 && s0 >= (R)1) return 1;
+  if (s2 <= (R)0 && s1 >= (R)1) return 2;
+  return 6;
+}
+
+/* Squared point-point distance and gradient: d = |p-a|^2, dd/dp = 2(p-a). */
+static inline R CAT(pp_, S)(const R* p, const R* a, R* gp, R* ga) {
+  R d[3];
+  SUB(p, a, d);
+  for (int c = 0; c < 3; ++c) { gp[c] = (R)2 * d[c]; ga[c] = -gp[c]; }
+  return DOT(d, d);
+}
+
+/* Squared point-line distance d = |u x w|^2 / |w - u|^2 with u = a - p, w = b - p
+   (distance.py:105-109).  Gradient: with c = u x w and L = |w-u|^2,
+   dd/du = 2 (w x c)/L + 2 d (w-u)/L,  dd/dw = 2 (c x u)/L - 2 d (w-u)/L,
+   dd/dp = -(dd/du + dd/dw). */
+static inline R CAT(pe_, S)(const R* p, const R* a, const R* b, R* gp, R* ga, R* gb) {
+  R u[3], w[3], c[3], e[3], wc[3], cu[3];
+  SUB(a, p, u);
+  SUB(b, p, w);
+  CROSS(u, w, c);
+  SUB(b, a, e);
+  R L = DOT(e, e);
+  R d = DOT(c, c) / L;
+  CROSS(w, c, wc);
+  CROSS(c, u, cu);
+  R twoL = (R)2 / L;
+  R q = twoL * d;
+  for (int k = 0; k < 3; ++k) {
+    ga[k] = twoL * wc[k] + q * e[k];
+    gb[k] = twoL * cu[k] - q * e[k];
+    gp[k] = -(ga[k] + gb[k]);
+  }
+  return d;
+}
+
+/* Squared point-plane distance d = ((p-a).n)^2 / |n|^2, n = (b-a) x (c-a)
+   (distance.py:111-116).  With r = p - a, s = r.n, N = n.n, q = s/N:
+   dd/dp = 2 q n,  dd/db = 2 q (e2 x r) - 2 q^2 (e2 x n),
+   dd/dc = 2 q (r x e1) - 2 q^2 (n x e1),  dd/da = -(sum of the others). */
+static inline R CAT(pt_, S)(const R* p, const R* a, const R* b, const R* c3, R* gp, R* ga, R* gb, R* gc) {
+  R e1[3], e2[3], n[3], r[3], e2r[3], e2n[3], re1[3], ne1[3];
+  SUB(b, a, e1);
+  SUB(c3, a, e2);
+  CROSS(e1, e2, n);
+  SUB(p, a, r);
+  R s = DOT(r, n);
+  R N = DOT(n, n);
+  R d = (s * s) / N;
+  R q = s / N;
+  R q2 = (R)2 * q, qq2 = ((R)2 * q) * q;
+  CROSS(e2, r, e2r);
+  CROSS(e2, n, e2n);
+  CROSS(r, e1, re1);
+  CROSS(n, e1, ne1);
+  for (int k = 0; k < 3; ++k) {
+    gp[k] = q2 * n[k];
+    gb[k] = q2 * e2r[k] - qq2 * e2n[k];
+    gc[k] = q2 * re1[k] - qq2 * ne1[k];
+    ga[k] = -((gp[k] + gb[k]) + gc[k]);
+  }
+  return d;
+}
+
+/* Squared distance of type dtype and its gradient; g12 = [g_p, g_t0, g_t1, g_t2]
+   with zeros for points not in the pair (collision_constraints_x.py:24-52). */
+R FN(dist)(int dtype, const R* p, const R* t0, const R* t1, const R* t2, R* g) {
+  for (int k = 0; k < 12; ++k) g[k] = 0;
+  switch (dtype) {
+    case 0: return CAT(pp_, S)(p, t0, g, g + 3);
+    case 1: return CAT(pp_, S)(p, t1, g, g + 6);
+    case 2: return CAT(pp_, S)(p, t2, g, g + 9);
+    case 3: return CAT(pe_, S)(p, t0, t1, g, g + 3, g + 6);
+    case 4: return CAT(pe_, S)(p, t1, t2, g, g + 6, g + 9);
+    case 5: return CAT(pe_, S)(p, t0, t2, g, g + 3, g + 9);
+    default: return CAT(pt_, S)(p, t0, t1, t2, g, g + 3, g + 6, g + 9);
+  }
+}
+
+/* Slots of the pair's points among (p, t0, t1, t2) for each type. */
+static const unsigned char CAT(slots_, S)[7] = {0x3, 0x5, 0x9, 0x7, 0xD, 0xB, 0xF};
+
+/* collision_constraints_x.py:5-69 __vt_st: dynamic vertex vs static triangle. */
+static void CAT(vt_st_, S)(int vd, int fs, const int* faces_st, const R* y, const R* xs,
+                           const R* m_inv, R dhat, R k, R* dx, R* nc, int cap, int* pnum) {
+  const R* p = y + 3 * vd;
+  const R *t0 = xs + 3 * faces_st[3 * fs], *t1 = xs + 3 * faces_st[3 * fs + 1],
+          *t2 = xs + 3 * faces_st[3 * fs + 2];
+  int dtype = FN(d_type_pt)(p, t0, t1, t2);
+  R g[12];
+  R d = FN(dist)(dtype, p, t0, t1, t2, g);
+  if (d < dhat) {
+    R w = m_inv[vd];
+    R schur = w * DOT(g, g);
+    R ld = (k * (dhat - d)) / ((k * schur) + (R)1);
+    for (int c = 0; c < 3; ++c) dx[3 * vd + c] = dx[3 * vd + c] + (w * ld) * g[c];
+    nc[vd] = nc[vd] + (R)1;
+    if (pnum[vd] < cap) pnum[vd] += 1;
+  }
+}
+
+/* collision_constraints_x.py:71-173 __tv_st: static vertex vs dynamic triangle.
+   Only the triangle's vertices move; the pair is recorded whatever d is. */
+static void CAT(tv_st_, S)(int fd, int vs, const int* faces_dy, R* y, const R* xs,
+                           const R* fixed, const R* m_inv, R dhat, R k, R* dx, R* nc, int cap,
+                           int* pnum) {
+  const R* p = xs + 3 * vs;
+  int vid[4] = {-1, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
+  const R *t0 = y + 3 * vid[1], *t1 = y + 3 * vid[2], *t2 = y + 3 * vid[3];
+  int dtype = FN(d_type_pt)(p, t0, t1, t2);
+  R g[12];
+  R d = FN(dist)(dtype, p, t0, t1, t2, g);
+  if (d < dhat) {
+    unsigned m = CAT(slots_, S)[dtype];
+    R schur = 0;
+    int first = 1;
+    for (int s = 1; s < 4; ++s) {
+      if (!(m & (1u << s))) continue;
+      int v = vid[s];
+      R term = (fixed[v] * m_inv[v]) * DOT(g + 3 * s, g + 3 * s);
+      schur = first ? term : schur + term;
+      first = 0;
+    }
+    R ld = (k * (dhat - d)) / ((k * schur) + (R)1);
+    for (int s = 1; s < 4; ++s) {
+      if (!(m & (1u << s))) continue;
+      int v = vid[s];
+      R a = (fixed[v] * m_inv[v]) * ld;
+      for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s + c];
+    }
+    for (int s = 1; s < 4; ++s)
+      if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+  }
+  if (pnum[vs] < cap) pnum[vs] += 1;
+}
+
+/* collision_constraints_x.py:175-310 __vt_dy: dynamic vertex vs dynamic triangle
+   (self contact), hard constraint with ld = (dhat - d) / (sum f w |g|^2 + 1e-4).
+   Type 0 omits f on both points' deltas (collision_constraints_x.py:199-200). */
+static void CAT(vt_dy_, S)(int vd, int fd, const int* faces_dy, R* y, const R* fixed,
+                           const R* m_inv, R dhat, R* dx, R* nc, int cap, int* pnum) {
+  int vid[4] = {vd, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
+  const R *p = y + 3 * vid[0], *t0 = y + 3 * vid[1], *t1 = y + 3 * vid[2], *t2 = y + 3 * vid[3];
+  int dtype = FN(d_type_pt)(p, t0, t1, t2);
+  R g[12];
+  R d = FN(dist)(dtype, p, t0, t1, t2, g);
+  if (d < dhat) {
+    unsigned m = CAT(slots_, S)[dtype];
+    R schur = 0;
+    int first = 1;
+    for (int s = 0; s < 4; ++s) {
+      if (!(m & (1u << s))) continue;
+      int v = vid[s];
+      R term = (fixed[v] * m_inv[v]) * DOT(g + 3 * s, g + 3 * s);
+      schur = first ? term : schur + term;
+      first = 0;
+    }
+    schur = schur + (R)1e-4f;
+    R ld = (dhat - d) / schur;
+    for (int s = 0; s < 4; ++s) {
+      if (!(m & (1u << s))) continue;
+      int v = vid[s];
+      R a = (dtype == 0) ? m_inv[v] * ld : (fixed[v] * m_inv[v]) * ld;
+      for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s + c];
+    }
+    for (int s = 0; s < 4; ++s)
+      if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+  }
+  if (pnum[vd] < cap) pnum[vd] += 1;
+}
+
+void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
+                        const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
+                        const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
+                        const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
+                        int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st) {
+  for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  for (int i = 0; i < nv; ++i) { nc[i] = 0; pn_vt_st[i] = 0; pn_vt_dy[i] = 0; }
+  for (int i = 0; i < nvs; ++i) pn_tv_st[i] = 0;
+  if (vt_st_c)
+    for (int v = 0; v < nv; ++v) {
+      int n = vt_st_n[v] < cap ? vt_st_n[v] : cap;
+      for (int j = 0; j < n; ++j)
+        CAT(vt_st_, S)(v, vt_st_c[v * cap + j], faces_st, y, xs, m_inv, dhat, k_col, dx, nc, cap, pn_vt_st);
+    }
+  if (vt_dy_c)
+    for (int v = 0; v < nv; ++v) {
+      int n = vt_dy_n[v] < cap ? vt_dy_n[v] : cap;
+      for (int j = 0; j < n; ++j) {
+        int f = vt_dy_c[v * cap + j];
+        if (faces_dy[3 * f] == v || faces_dy[3 * f + 1] == v || faces_dy[3 * f + 2] == v) continue;
+        CAT(vt_dy_, S)(v, f, faces_dy, y, fixed, m_inv, dhat, dx, nc, cap, pn_vt_dy);
+      }
+    }
+  if (tv_st_c)
+    for (int v = 0; v < nvs; ++v) {
+      int n = tv_st_n[v] < cap ? tv_st_n[v] : cap;
+      for (int j = 0; j < n; ++j)
+        CAT(tv_st_, S)(tv_st_c[v * cap + j], v, faces_dy, y, xs, fixed, m_inv, dhat, k_col, dx, nc, cap, pn_tv_st);
+    }
+  CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+}
+
+#undef FN
+#undef SQRT
+#undef DOT
+#undef SUB
+#undef CROSS

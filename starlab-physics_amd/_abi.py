@@ -1,0 +1,104 @@
+"""ctypes binding of libslx.so (the C ABI declared in include/slx.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+libslx.so is missing, or a GPU entry point is called without a HIP device, the
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libslx.so")
+_LIB = None
+
+I32, F32, PTR, STREAM = C.c_int32, C.c_float, C.c_void_p, C.c_void_p
+I64 = C.c_int64
+
+# name -> argument types (return type is always int32 status unless listed in _RET)
+SIGNATURES = {
+    "slx_abi_version": [],
+    "slx_last_error": [],
+    "slx_timer_enable": [I32],
+    "slx_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)],
+    "slx_timer_names": [C.c_char_p, I32],
+    "slx_compute_y": [I32, PTR, PTR, PTR, F32, F32, F32, F32, STREAM],
+    "slx_spring_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, STREAM],
+    "slx_update_vx": [I32, PTR, PTR, PTR, PTR, F32, F32, STREAM],
+    "slx_substeps_plain": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, I32,
+                           F32, F32, F32, F32, F32, F32, F32, STREAM],
+    "slx_set_fixed": [I32, PTR, PTR, PTR, PTR, PTR, STREAM],
+}
+_RET = {"slx_last_error": C.c_char_p}
+
+
+class SlxError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise SlxError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                           "python starlab-physics_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RET.get(name, C.c_int)
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def last_error() -> str:
+    msg = lib().slx_last_error()
+    return msg.decode() if msg else ""
+
+
+def call(name: str, *args) -> None:
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise SlxError(f"{name} failed ({rc}): {last_error()}")
+
+
+def ptr(t) -> int:
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def timer_enable(max_launches: int) -> None:
+    call("slx_timer_enable", max_launches)
+
+
+def timer_read(name: str):
+    """(total_ms, count) of the timed launches of kernel `name`."""
+    tot, cnt = C.c_double(0.0), C.c_int(0)
+    call("slx_timer_read", name.encode(), C.byref(tot), C.byref(cnt))
+    return tot.value, cnt.value
+
+
+def timer_names():
+    buf = C.create_string_buffer(4096)
+    call("slx_timer_names", buf, len(buf))
+    v = buf.value.decode()
+    return v.split(",") if v else []
+
+
+def greedy_coloring(nv: int, edges: np.ndarray, max_colors: int = 64):
+    raise NotImplementedError

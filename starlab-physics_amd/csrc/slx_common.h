@@ -1,0 +1,73 @@
+// Shared device helpers for the gfx950 XPBD cloth kernels.
+//
+// Floating-point op order in every formula is written out explicitly and the
+// library is compiled with -ffp-contract=off, so each kernel performs the same
+// IEEE single-precision operations, in the same order, as the serial
+// restatement of the reference step; division and sqrt are the correctly
+// rounded HIP defaults.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/slx.h"
+
+namespace slx {
+
+constexpr int kWave = 64;
+constexpr int kGrid = 32;                       // cell LBVH resolution per axis
+constexpr int kCells = kGrid * kGrid * kGrid;   // 32768
+
+// status word bits (device int32 written with atomicOr)
+constexpr int kStatusCandOverflow = SLX_STATUS_CAND_OVERFLOW;
+constexpr int kStatusStackOverflow = SLX_STATUS_STACK_OVERFLOW;
+constexpr int kStatusCellRange = SLX_STATUS_CELL_RANGE;
+
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 mul3(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 xyz(float4 a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 ld3(const float* p, int i) {
+  return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
+// float -> int32 truncation: NaN -> INT32_MIN, saturating (host oracle does the same)
+__device__ __forceinline__ int32_t trunc_i32(float v) {
+  if (!(v > -2147483648.0f)) return INT32_MIN;
+  if (v >= 2147483648.0f) return INT32_MAX;
+  return (int32_t)v;
+}
+
+__device__ __forceinline__ bool overlap(float3 amin, float3 amax, float3 bmin, float3 bmax) {
+  return amin.x <= bmax.x && amax.x >= bmin.x && amin.y <= bmax.y && amax.y >= bmin.y &&
+         amin.z <= bmax.z && amax.z >= bmin.z;
+}
+
+inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// error reporting (thread-local message; see abi.hip)
+int fail(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+// optional per-kernel event timer (slx_timer_enable); no-ops when disabled
+void timer_begin(const char* name, hipStream_t s, int* slot);
+void timer_end(int slot, hipStream_t s);
+
+}  // namespace slx
+
+// Launch with optional event timing around the kernel, on the same stream.
+#define SLX_LAUNCH(name, kernel, grid, block, shmem, stream, ...)            \
+  do {                                                                       \
+    int _slx_ts;                                                             \
+    ::slx::timer_begin(name, stream, &_slx_ts);                              \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);     \
+    ::slx::timer_end(_slx_ts, stream);                                       \
+  } while (0)
+
+#define SLX_REQUIRE(cond, ...)                          \
+  do {                                                  \
+    if (!(cond)) return ::slx::fail(SLX_EINVAL, __VA_ARGS__); \
+  } while (0)

@@ -1,0 +1,151 @@
+"""Host topology (meshTaichiWrapper.py:127-256) against closed forms and a literal
+loop restatement of the reference's bending-pair extraction."""
+import numpy as np
+import pytest
+
+
+def reference_bending_loop(fid):
+    """Literal per-face loop of meshTaichiWrapper.findTriNeighbors / getBendingPair
+    (:152-225), kept slow and obvious; pins the vectorised version."""
+    nf = len(fid)
+    table = np.zeros((3 * nf, 3), dtype=np.int64)
+    for f in range(nf):
+        a, b, c = np.sort(fid[f])
+        table[3 * f:3 * f + 3] = [[a, b, 3 * f], [a, c, 3 * f + 1], [b, c, 3 * f + 2]]
+    table = table[np.lexsort((table[:, 1], table[:, 0]))]
+    nb = np.full(3 * nf, -1, dtype=np.int64)
+    ii, count = 0, 0
+    while ii < 3 * nf - 1:
+        e0, e1 = table[ii], table[ii + 1]
+        if e0[0] == e1[0] and e0[1] == e1[1]:
+            nb[e0[2]], nb[e1[2]] = e1[2], e0[2]
+            count += 1
+            ii += 2
+        else:
+            ii += 1
+    pairs = []
+    for f in range(nf):
+        for k in range(3):
+            eid = 3 * f + k
+            ne = nb[eid]
+            if ne >= 0:
+                nb[ne] = -1
+                v = np.sort(fid[f])
+                w = np.sort(fid[ne // 3])
+                pairs.append((v[~np.isin(v, w)][0], w[~np.isin(w, v)][0]))
+    assert len(pairs) == count
+    return np.array(pairs, dtype=np.int64).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 17])
+def test_grid_counts_closed_form(slx, n):
+    g = slx.scene.grid_cloth(n)
+    m = slx.Mesh(x=g.x, faces=g.faces, device=None)
+    q = n - 1
+    assert m.num_verts == n * n
+    assert m.num_faces == 2 * q * q
+    assert m.num_edges == 3 * q * q + 2 * q
+    # interior edges = all edges minus the 4(n-1) boundary edges
+    assert m.bending_constraint_count == m.num_edges - 4 * q
+
+
+@pytest.mark.parametrize("maker", ["grid8", "grid13_jitter", "ico2", "layers"])
+def test_bending_pairs_match_reference_loop(slx, maker):
+    if maker == "grid8":
+        t = slx.scene.grid_cloth(8)
+    elif maker == "grid13_jitter":
+        t = slx.scene.grid_cloth(13, jitter=1e-3, seed=3)
+        # shuffle face order and vertex order within faces to exercise the sort
+        r = np.random.default_rng(5)
+        f = t.faces[r.permutation(len(t.faces))]
+        t.faces = np.array([row[r.permutation(3)] for row in f], np.int32)
+    elif maker == "ico2":
+        t = slx.scene.icosphere(2)
+    else:
+        t = slx.scene.stacked_layers(6, 3, 0.02)
+    got = slx.topology.bending_pairs(t.faces).astype(np.int64)
+    ref = reference_bending_loop(t.faces)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_bending_pairs_are_opposite_vertices(slx):
+    """Every pair (a, b) are the two vertices opposite a shared interior edge."""
+    t = slx.scene.grid_cloth(9)
+    pairs = slx.topology.bending_pairs(t.faces)
+    faces = [set(f) for f in t.faces.tolist()]
+    for a, b in pairs.tolist():
+        fa = [f for f in faces if a in f]
+        fb = [f for f in faces if b in f]
+        assert any(len(x & y) == 2 and a not in y and b not in x for x in fa for y in fb)
+
+
+def test_lumped_mass_closed_form(slx):
+    """Regular grid, spacing h: an interior vertex has 4 axis edges (h) and 2
+    diagonals (sqrt2 h): m = 0.5(4h + 2 sqrt2 h); corners differ by type."""
+    n, size = 11, 1.0
+    g = slx.scene.grid_cloth(n, size=size)
+    m = slx.Mesh(x=g.x, faces=g.faces, device=None)
+    h = size / (n - 1)
+    interior = 5 * n + 5
+    expect = 1.0 / (0.5 * (4 * h + 2 * np.sqrt(2) * h))
+    assert abs(m.m_inv_np[interior] - expect) / expect < 1e-5
+    # corner (0,0): edges to (1,0),(0,1) and diagonal (1,1)
+    c0 = 1.0 / (0.5 * (2 * h + np.sqrt(2) * h))
+    assert abs(m.m_inv_np[0] - c0) / c0 < 1e-5
+    # corner (0,n-1) has only the two axis edges
+    c1 = 1.0 / (0.5 * (2 * h))
+    assert abs(m.m_inv_np[n - 1] - c1) / c1 < 1e-5
+
+
+def test_rest_lengths_closed_form(slx):
+    g = slx.scene.grid_cloth(5, size=2.0)
+    m = slx.Mesh(x=g.x, faces=g.faces, device=None)
+    h = 0.5
+    ok = np.isclose(m.l0_np, h, rtol=1e-6) | np.isclose(m.l0_np, h * np.sqrt(2), rtol=1e-6)
+    assert ok.all()
+    # bending pairs on this triangulation span 2h (straight) or sqrt(2)h (across a diagonal)
+    ok = np.isclose(m.bending_l0_np, 2 * h * np.sqrt(2), rtol=1e-6) | np.isclose(
+        m.bending_l0_np, np.sqrt(2) * h, rtol=1e-6) | np.isclose(m.bending_l0_np, h * np.sqrt(5), rtol=1e-6)
+    assert ok.all()
+
+
+def test_adjacency_is_serial_scatter_order(slx):
+    """Row v of the SELL adjacency lists exactly the constraints containing v, in
+    ascending global constraint index (stretch first), i.e. the order in which a
+    serial loop over constraints would have added into dx[v]."""
+    t = slx.scene.grid_cloth(7, jitter=1e-3, seed=1)
+    m = slx.Mesh(x=t.x, faces=t.faces, device=None)
+    A = m.adj
+    cons = np.concatenate([m.eid_np, m.bend_np])
+    rl = np.concatenate([m.l0_np, m.bending_l0_np])
+    for v in range(m.num_verts):
+        s, lane = v // 64, v % 64
+        rows = []
+        for k in range(A.slice_width[s]):
+            w0, w1 = A.entries[A.slice_off[s] + 64 * k + lane]
+            if w0 == -1:
+                break
+            rows.append((int(w0) & 0x7FFFFFFF, int(w0) < 0, np.int32(w1).view(np.float32)))
+        expect = []
+        for c, (a, b) in enumerate(cons):
+            if v in (a, b):
+                expect.append((int(b if a == v else a), c >= len(m.eid_np), rl[c]))
+        assert [(u, k) for u, k, _ in rows] == [(u, k) for u, k, _ in expect]
+        assert all(x == y for (_, _, x), (_, _, y) in zip(rows, expect))
+        assert A.degree[v] == len(expect)
+
+
+def test_obj_roundtrip(slx, tmp_path):
+    t = slx.scene.icosphere(1, 0.5)
+    p = str(tmp_path / "m.obj")
+    slx.scene.write_obj(p, t.x, t.faces)
+    r = slx.scene.read_obj(p)
+    np.testing.assert_array_equal(r.faces, t.faces)
+    np.testing.assert_allclose(r.x, t.x, rtol=1e-7, atol=1e-8)
+
+
+def test_concat_offsets(slx):
+    a, b = slx.scene.grid_cloth(3), slx.scene.grid_cloth(4)
+    c = slx.scene.concat([a, b])
+    assert slx.scene.concat_offsets([a, b]) == [0, 9, 25]
+    np.testing.assert_array_equal(c.faces[len(a.faces):], b.faces + 9)
