@@ -96,6 +96,156 @@ int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, float* v4_al
 int slx_set_fixed(int nv, const float* fixed, const float* m_inv, float* x4, float* v4,
                   float* y4, slx_stream_t stream);
 
+/* ---------------- device primitives ---------------- */
+
+/* Exclusive scan of int32 (in -> out), total written to *total_dev when non-null.
+ * n_dev (nullable) holds the element count on the device (<= n_max). */
+size_t slx_scan_ws_bytes(int n_max);
+int slx_exclusive_scan(int n_max, const int32_t* n_dev, const int32_t* in, int32_t* out,
+                       int32_t* total_dev, void* ws, size_t ws_bytes, slx_stream_t stream);
+
+/* Stable LSD radix sort of (keys, vals) in place by the low key_bits bits
+ * (8-bit digits; the reference sorts with 6-bit digits, lbvh_cell.py:524-562, a
+ * stable sort by the same key gives the same order).  n_dev as above. */
+size_t slx_radix_sort_ws_bytes(int n_max);
+int slx_radix_sort_pairs(int n_max, const int32_t* n_dev, uint32_t* keys, int32_t* vals,
+                         int key_bits, void* ws, size_t ws_bytes, slx_stream_t stream);
+
+/* ---------------- broad phase (framework/collision/lbvh_cell.py) ---------------- */
+
+/* computeAABB_faces, meshTaichiWrapper.py:350-361: face boxes of x4 -+ padding,
+ * float4 per face (w = face id). */
+int slx_face_aabbs(int nf, const int32_t* faces, const float* x4, float padding, float* fmin4,
+                   float* fmax4, slx_stream_t stream);
+
+/* computeAABB, meshTaichiWrapper.py:334-347: box6 = (min - padding, max + padding),
+ * left on the device (the reference returns it to the host). */
+size_t slx_mesh_aabb_ws_bytes(void);
+int slx_mesh_aabb(int nv, const float* x4, float padding, float* box6, void* ws,
+                  size_t ws_bytes, slx_stream_t stream);
+
+/* LBVH_CELL.__init__, lbvh_cell.py:17-80: Morton codes of the 32^3 cell centres
+ * of [0,1e3]^3 (assign_cell_morton 122-152), stable radix sort (553-562), Karras
+ * 2012 tree (407-430).  codes/sorted_ids [32768], child_a/child_b/range_l/range_r
+ * [32767], parent [65535]. */
+size_t slx_cell_topology_ws_bytes(void);
+int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int32_t* child_b,
+                      int32_t* parent, int32_t* range_l, int32_t* range_r, void* ws,
+                      size_t ws_bytes, slx_stream_t stream);
+
+/* LBVH_CELL.build, lbvh_cell.py:637-653: faces bucketed by cell (stable, ascending
+ * face id inside a cell), inclusive cell prefix, leaf slot i = row-major cell i
+ * (box = cell centre U its faces' boxes), bottom-up refit (490-522).  Node arrays
+ * hold 65535 nodes: num [65535], nlo4/nhi4 float4 [65535]; leaf_l/leaf_r/prefix
+ * [32768]; face_cell/sorted_faces [nf]. */
+size_t slx_cell_build_ws_bytes(int nf);
+int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
+                   const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
+                   int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces, int32_t* num,
+                   int32_t* leaf_l, int32_t* leaf_r, float* nlo4, float* nhi4, int32_t* status,
+                   void* ws, size_t ws_bytes, slx_stream_t stream);
+
+/* Per-triangle LBVH (the reference's unused per-face path, lbvh_cell.py:153-194,
+ * 237-247, 383-405, completed): 30-bit Morton codes of face-box centres in box6,
+ * stable radix sort of (code, face id), Karras tree with an index tie-break for
+ * equal codes, refit.  codes/sorted_faces [nf], child_a/child_b [nf-1], parent
+ * [2nf-1], nlo4/nhi4 float4 [2nf-1]. */
+size_t slx_tri_build_ws_bytes(int nf);
+int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
+                  int32_t* codes, int32_t* sorted_faces, int32_t* child_a, int32_t* child_b,
+                  int32_t* parent, float* nlo4, float* nhi4, void* ws, size_t ws_bytes,
+                  slx_stream_t stream);
+/* Refit of the triangle LBVH over new face boxes (topology kept). ws >= 4*nf bytes. */
+int slx_tri_refit(int nf, const float* fmin4, const float* fmax4, const int32_t* sorted_faces,
+                  const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
+                  float* nlo4, float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream);
+
+/* traverse_cell_bvh_single, lbvh_cell.py:655-714, for nq query points p -+ padding
+ * (p = xyz of q4).  mode 0 = cell tree (queries whose centre cell id exceeds 32767
+ * are skipped, as in the reference), 1 = triangle tree.  Candidates in traversal
+ * order to cand[j*nq + q], j < cap; cand_n[q] = min(count, cap); overflow sets a
+ * status bit (the reference writes past its 40-entry cache). */
+int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves,
+                 const int32_t* num, const int32_t* sorted_faces, const float* nlo4,
+                 const float* nhi4, const float* fmin4, const float* fmax4, const float* box6,
+                 int cap, int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
+
+/* ---------------- narrow phase (collision_constraints_x.py, distance.py) ---------------- */
+
+/* solve_collision_constraints_x + update_dx, XPBD.py:236-264, 383-387, with
+ * __vt_st / __vt_dy / __tv_st (collision_constraints_x.py:5-310): the three
+ * candidate sets (any may be null) in the reference's loop order; deltas are
+ * accumulated per vertex in that serial order without float atomics; y4 updated
+ * in place.  pair_num [2nv+nvs] (nullable) receives the recorded pair counts.
+ * rec_cap bounds the contact records of one pass (overflow -> status bit). */
+size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap);
+int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st,
+                       float* y4, const float* xs4, const float* fixed, const float* m_inv,
+                       float dhat, float k_col, int cap, const int32_t* c_vt_st,
+                       const int32_t* n_vt_st, const int32_t* c_vt_dy, const int32_t* n_vt_dy,
+                       const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
+                       int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
+                       slx_stream_t stream);
+
+/* ---------------- one frame (Solver.forward, XPBD.py:443-469) ---------------- */
+
+typedef struct slx_tree {
+  int32_t mode;              /* 0 = cell LBVH, 1 = triangle LBVH */
+  int32_t nf;                /* faces */
+  float* fmin4;              /* float4 [nf] face boxes */
+  float* fmax4;
+  float* box6;               /* mesh box (after padding) */
+  int32_t* codes;            /* triangle: Morton codes [nf] (sorted) */
+  int32_t* sorted_faces;     /* [nf] */
+  int32_t* child_a;          /* cell: static topology [32767]; triangle: [nf-1] */
+  int32_t* child_b;
+  int32_t* parent;           /* cell: [65535]; triangle: [2nf-1] */
+  int32_t* num;              /* cell: faces under each node [65535] */
+  int32_t* face_cell;        /* cell: [nf] */
+  int32_t* prefix;           /* cell: inclusive prefix [32768] */
+  int32_t* leaf_l;           /* cell: [32768] */
+  int32_t* leaf_r;
+  float* nlo4;               /* float4 [nodes] */
+  float* nhi4;
+} slx_tree;
+
+typedef struct slx_frame_desc {
+  /* dynamic mesh */
+  int32_t nv;
+  float* x4;
+  float* v4;
+  float* y4;
+  float* y4_alt;
+  const float* fixed;
+  const float* m_inv;
+  const int32_t* faces;                  /* [3 nf] */
+  const int32_t* adj_slice_off;
+  const int32_t* adj_slice_width;
+  const int32_t* adj_entries;
+  /* static mesh (nvs = 0 when absent) */
+  int32_t nvs;
+  const float* xs4;
+  const int32_t* faces_st;
+  /* per-substep scalars (float32, as the reference passes them) */
+  float dt, gx, gy, gz, k_s, k_b, damping;
+  float padding, dhat, k_col;
+  int32_t n_substeps, n_iters;
+  int32_t collision;                     /* enable_collision_handling */
+  int32_t refit_every_substep;           /* triangle tree: refit from x at each substep */
+  int32_t rebuild_static;                /* rebuild the static tree this frame */
+  int32_t cap, rec_cap;
+  slx_tree dyn, st;
+  int32_t* cand;                         /* [cap*nv] vt_st, [cap*nv] vt_dy, [cap*nvs] tv_st */
+  int32_t* cand_n;                       /* [2nv + nvs] */
+  int32_t* pair_num;                     /* [2nv + nvs], nullable */
+  int32_t* status;                       /* device status word */
+  void* ws;
+  size_t ws_bytes;
+} slx_frame_desc;
+
+size_t slx_frame_ws_bytes(const slx_frame_desc* d);
+int slx_frame(const slx_frame_desc* d, slx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,538 @@
+// Broad phase on gfx950: face / mesh AABBs, Morton codes, Karras radix tree,
+// bottom-up refit with atomic arrival flags, stack traversal with the stack
+// in LDS.
+//
+// Two trees share the kernels:
+//  * the reference's cell LBVH (framework/collision/lbvh_cell.py): a static
+//    Karras tree over the Morton-sorted 32^3 cell grid, refilled every build
+//    with the faces bucketed by cell (leaf slot i holds row-major cell i,
+//    lbvh_cell.py:601-634);
+//  * a per-triangle LBVH: one leaf per face in Morton order of the face-box
+//    centres, index tie-break for equal codes (64-bit keys code<<32 | index).
+//
+// Node storage (HBM): nlo[k] / nhi[k] are float4 = (box min, child_a) and
+// (box max, child_b) for internal nodes, (box, first face slot) / (box, last
+// face slot) for cell leaves and (box, face id) for triangle leaves, so one
+// 32-byte fetch gives the traversal everything it needs at a node.
+#include "slx_common.h"
+
+namespace slx {
+
+size_t radix_sort_ws_bytes(int n_max);
+int radix_sort_pairs(int n_max, const int* n_dev, uint32_t* keys, int32_t* vals, int key_bits,
+                     void* ws, size_t ws_bytes, hipStream_t st);
+size_t scan_ws_bytes(int n_max);
+int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* total_dev, void* ws,
+                   size_t ws_bytes, hipStream_t st);
+
+constexpr int kStack = 64;       // traversal stack depth (entries)
+constexpr int kTravBS = 128;     // traversal block size
+
+// ----------------------------------------------------------------- AABBs
+
+// computeAABB_faces, meshTaichiWrapper.py:350-361
+__global__ void k_face_aabbs(int nf, const int32_t* __restrict__ faces, const float4* __restrict__ x4,
+                             float pad, float4* __restrict__ fmin, float4* __restrict__ fmax) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  float3 a = xyz(x4[faces[3 * f]]), b = xyz(x4[faces[3 * f + 1]]), c = xyz(x4[faces[3 * f + 2]]);
+  const float p = pad * 1.0f;
+  fmin[f] = make_float4(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p,
+                        fminf(fminf(a.z, b.z), c.z) - p, __int_as_float(f));
+  fmax[f] = make_float4(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p,
+                        fmaxf(fmaxf(a.z, b.z), c.z) + p, __int_as_float(f));
+}
+
+// computeAABB, meshTaichiWrapper.py:334-347: min/max reduction (order-independent,
+// hence exact), stage 1 per block, stage 2 one block adds the padding.
+constexpr int kBoxBS = 256;
+__global__ void __launch_bounds__(kBoxBS)
+k_box_partial(int nv, const float4* __restrict__ x4, float* __restrict__ part) {
+  __shared__ float s[6][kBoxBS];
+  float lo[3] = {1e5f, 1e5f, 1e5f}, hi[3] = {-1e5f, -1e5f, -1e5f};
+  for (int i = blockIdx.x * kBoxBS + threadIdx.x; i < nv; i += gridDim.x * kBoxBS) {
+    float4 p = x4[i];
+    lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+    hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+  }
+  for (int c = 0; c < 3; ++c) { s[c][threadIdx.x] = lo[c]; s[3 + c][threadIdx.x] = hi[c]; }
+  __syncthreads();
+  for (int o = kBoxBS / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + o]);
+        s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + o]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_box_final(int nparts, const float* __restrict__ part, float pad, float* __restrict__ box) {
+  if (threadIdx.x != 0) return;
+  float lo[3] = {1e5f, 1e5f, 1e5f}, hi[3] = {-1e5f, -1e5f, -1e5f};
+  for (int b = 0; b < nparts; ++b)
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fminf(lo[c], part[6 * b + c]);
+      hi[c] = fmaxf(hi[c], part[6 * b + 3 + c]);
+    }
+  for (int c = 0; c < 3; ++c) { box[c] = lo[c] - pad * 1.0f; box[3 + c] = hi[c] + pad * 1.0f; }
+}
+
+constexpr int kBoxParts = 1024;
+
+// ----------------------------------------------------------------- Morton
+
+// expand_bits / morton_3d, lbvh_cell.py:85-102 (x in the lowest bit)
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ float clampf(float a, float lo, float hi) { return fminf(fmaxf(a, lo), hi); }
+__device__ __forceinline__ uint32_t morton3d(float x, float y, float z) {
+  x = clampf(x * 1024.0f, 0.0f, 1023.0f);
+  y = clampf(y * 1024.0f, 0.0f, 1023.0f);
+  z = clampf(z * 1024.0f, 0.0f, 1023.0f);
+  return expand_bits((uint32_t)x) | (expand_bits((uint32_t)y) << 1) | (expand_bits((uint32_t)z) << 2);
+}
+// normalised code of a point in [lo, hi] (lbvh_cell.py:139-148)
+__device__ __forceinline__ uint32_t morton_in_box(float3 p, float3 lo, float3 hi) {
+  return morton3d(clampf((p.x - lo.x) / (hi.x - lo.x), 0.f, 1.f), clampf((p.y - lo.y) / (hi.y - lo.y), 0.f, 1.f),
+                  clampf((p.z - lo.z) / (hi.z - lo.z), 0.f, 1.f));
+}
+
+__device__ __forceinline__ float3 cell_size_of(const float* box) {
+  return make_float3((box[3] - box[0]) / (float)kGrid, (box[4] - box[1]) / (float)kGrid,
+                     (box[5] - box[2]) / (float)kGrid);
+}
+
+// lbvh_cell.py:564-574 (truncation, int32 wrap-around flattening)
+__device__ __forceinline__ int32_t flat_cell_id(float3 p, float3 cs, float3 o) {
+  uint32_t i = (uint32_t)trunc_i32((p.x - o.x) / cs.x), j = (uint32_t)trunc_i32((p.y - o.y) / cs.y),
+           k = (uint32_t)trunc_i32((p.z - o.z) / cs.z);
+  return (int32_t)(i * (uint32_t)(kGrid * kGrid) + j * (uint32_t)kGrid + k);
+}
+
+// assign_cell_morton over [lo, hi], lbvh_cell.py:122-152
+__global__ void k_cell_morton(float3 lo, float3 hi, uint32_t* __restrict__ codes, int32_t* __restrict__ ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kCells) return;
+  float3 cs = make_float3((hi.x - lo.x) / (float)kGrid, (hi.y - lo.y) / (float)kGrid, (hi.z - lo.z) / (float)kGrid);
+  int x0 = i / (kGrid * kGrid), y0 = (i % (kGrid * kGrid)) / kGrid, z0 = i % kGrid;
+  float3 c = make_float3(((cs.x * (float)x0) + (0.5f * cs.x)) + lo.x, ((cs.y * (float)y0) + (0.5f * cs.y)) + lo.y,
+                         ((cs.z * (float)z0) + (0.5f * cs.z)) + lo.z);
+  codes[i] = morton_in_box(c, lo, hi);
+  ids[i] = i;
+}
+
+__global__ void k_face_morton(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                              const float* __restrict__ box, uint32_t* __restrict__ codes,
+                              int32_t* __restrict__ ids) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  float4 a = fmin[f], b = fmax[f];
+  float3 c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+  codes[f] = morton_in_box(c, make_float3(box[0], box[1], box[2]), make_float3(box[3], box[4], box[5]));
+  ids[f] = f;
+}
+
+// ----------------------------------------------------------------- Karras
+
+// common prefix length of sorted keys i and j; -1 outside [0, n) (lbvh_cell.py:255-264)
+__device__ __forceinline__ int prefix_len(const uint32_t* codes, int n, bool wide, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  if (wide) {
+    uint64_t a = ((uint64_t)codes[i] << 32) | (uint32_t)i, b = ((uint64_t)codes[j] << 32) | (uint32_t)j;
+    return __clzll(a ^ b);
+  }
+  uint32_t x = codes[i] ^ codes[j];
+  return x ? __clz(x) : 32;
+}
+
+// Karras 2012 internal node i (lbvh_cell.py:301-336, 272-298, 407-430)
+__global__ void k_karras(int n, const uint32_t* __restrict__ codes, int wide, int32_t* __restrict__ ca,
+                         int32_t* __restrict__ cb, int32_t* __restrict__ parent, int32_t* __restrict__ rl,
+                         int32_t* __restrict__ rr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) parent[0] = -1;
+  if (i >= n - 1) return;
+  const bool W = wide != 0;
+  int dl = prefix_len(codes, n, W, i, i - 1), dr = prefix_len(codes, n, W, i, i + 1);
+  int d = (dr < dl) ? -1 : 1;
+  int dmin = (dr < dl) ? dr : dl;
+  int hi = 2;
+  while (prefix_len(codes, n, W, i, i + hi * d) > dmin) hi *= 2;
+  int len = 0;
+  for (int t = hi / 2; t > 0; t /= 2)
+    if (prefix_len(codes, n, W, i, i + (len + t) * d) > dmin) len += t;
+  int j = i + len * d;
+  int first = d > 0 ? i : j, last = d > 0 ? j : i;
+  int node_prefix = prefix_len(codes, n, W, first, last);
+  int split;
+  if (!W && node_prefix == 32) {
+    split = (first + last) / 2;
+  } else {
+    split = first;
+    int step = last - first;
+    do {
+      step = (step + 1) / 2;
+      int c = split + step;
+      if (c < last && prefix_len(codes, n, W, first, c) > node_prefix) split = c;
+    } while (step > 1);
+  }
+  int a = (split == first) ? split + (n - 1) : split;
+  int b = (split + 1 == last) ? split + 1 + (n - 1) : split + 1;
+  ca[i] = a;
+  cb[i] = b;
+  parent[a] = i;
+  parent[b] = i;
+  if (rl) rl[i] = first;
+  if (rr) rr[i] = last;
+}
+
+// ----------------------------------------------------------------- refit
+
+__global__ void k_fill_i32(int n, int32_t* __restrict__ p, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// lbvh_cell.py:499-515: union of non-empty children; a node whose children are
+// both empty gets the degenerate box 0.25*(min0+max0+min1+max1)
+__device__ __forceinline__ void refit_internal(int i, int nl, const int32_t* ca, const int32_t* cb,
+                                               int32_t* num, float4* nlo, float4* nhi) {
+  int l = ca[i], r = cb[i];
+  float4 a0 = nlo[l], a1 = nhi[l], b0 = nlo[r], b1 = nhi[r];
+  int nlv = num ? num[l] : 1, nrv = num ? num[r] : 1;
+  float3 mn, mx;
+  if (nlv > 0 && nrv > 0) {
+    mn = make_float3(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z));
+    mx = make_float3(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z));
+  } else if (nlv > 0) {
+    mn = xyz(a0); mx = xyz(a1);
+  } else if (nrv > 0) {
+    mn = xyz(b0); mx = xyz(b1);
+  } else {
+    mn = make_float3(0.25f * (((a0.x + a1.x) + b0.x) + b1.x), 0.25f * (((a0.y + a1.y) + b0.y) + b1.y),
+                     0.25f * (((a0.z + a1.z) + b0.z) + b1.z));
+    mx = mn;
+  }
+  if (num) num[i] = nlv + nrv;
+  nlo[i] = make_float4(mn.x, mn.y, mn.z, __int_as_float(l));
+  nhi[i] = make_float4(mx.x, mx.y, mx.z, __int_as_float(r));
+}
+
+// Walk up from leaf `node`; the second child to arrive at a parent refits it.
+__device__ void refit_up(int node, int nl, const int32_t* ca, const int32_t* cb, const int32_t* parent,
+                         int32_t* num, float4* nlo, float4* nhi, int32_t* flags) {
+  __threadfence();
+  int p = parent[node];
+  while (p >= 0) {
+    int prev = atomicAdd(&flags[p], 1);
+    if (prev == 0) return;           // sibling subtree not finished: it will continue
+    __threadfence();                 // see the sibling's writes
+    refit_internal(p, nl, ca, cb, num, nlo, nhi);
+    __threadfence();
+    p = parent[p];
+  }
+}
+
+// Triangle leaves: leaf k holds face sorted[k]
+__global__ void k_tri_refit(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                            const int32_t* __restrict__ sorted, const int32_t* __restrict__ ca,
+                            const int32_t* __restrict__ cb, const int32_t* __restrict__ parent,
+                            float4* nlo, float4* nhi, int32_t* flags) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  int f = sorted[k], node = k + nf - 1;
+  float4 a = fmin[f], b = fmax[f];
+  nlo[node] = make_float4(a.x, a.y, a.z, __int_as_float(f));
+  nhi[node] = make_float4(b.x, b.y, b.z, __int_as_float(f));
+  refit_up(node, nf, ca, cb, parent, nullptr, nlo, nhi, flags);
+}
+
+// Cell leaves (assign_leaf_cell_nodes, lbvh_cell.py:601-634): leaf slot i holds
+// row-major cell i: range from the inclusive prefix, box = cell centre united
+// with its faces' boxes.
+__global__ void k_cell_leaves(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                              const float* __restrict__ box, const int32_t* __restrict__ prefix,
+                              const int32_t* __restrict__ sorted, const int32_t* __restrict__ ca,
+                              const int32_t* __restrict__ cb, const int32_t* __restrict__ parent,
+                              int32_t* leaf_l, int32_t* leaf_r, int32_t* num, float4* nlo, float4* nhi,
+                              int32_t* flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kCells) return;
+  const float3 lo = make_float3(box[0], box[1], box[2]);
+  const float3 cs = cell_size_of(box);
+  int x0 = i / (kGrid * kGrid), y0 = (i % (kGrid * kGrid)) / kGrid, z0 = i % kGrid;
+  float3 mn = make_float3(((cs.x * (float)x0) + (0.5f * cs.x)) + lo.x, ((cs.y * (float)y0) + (0.5f * cs.y)) + lo.y,
+                          ((cs.z * (float)z0) + (0.5f * cs.z)) + lo.z);
+  float3 mx = mn;
+  int l = i ? prefix[i - 1] : 0, r = prefix[i] - 1;
+  for (int j = l; j <= r; ++j) {
+    int f = sorted[j];
+    float4 a = fmin[f], b = fmax[f];
+    mn = make_float3(fminf(mn.x, a.x), fminf(mn.y, a.y), fminf(mn.z, a.z));
+    mx = make_float3(fmaxf(mx.x, b.x), fmaxf(mx.y, b.y), fmaxf(mx.z, b.z));
+  }
+  int node = i + kCells - 1;
+  leaf_l[i] = l;
+  leaf_r[i] = r;
+  num[node] = r - l + 1;
+  nlo[node] = make_float4(mn.x, mn.y, mn.z, __int_as_float(l));
+  nhi[node] = make_float4(mx.x, mx.y, mx.z, __int_as_float(r));
+  refit_up(node, kCells, ca, cb, parent, num, nlo, nhi, flags);
+}
+
+// face -> cell id (assign_face_cell_ids, lbvh_cell.py:576-586) + histogram
+__global__ void k_face_cells(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                             const float* __restrict__ box, uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ ids, int32_t* __restrict__ face_cell,
+                             int32_t* __restrict__ hist, int32_t* status) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  float4 a = fmin[f], b = fmax[f];
+  float3 p = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+  int32_t c = flat_cell_id(p, cell_size_of(box), make_float3(box[0], box[1], box[2]));
+  if (c < 0 || c >= kCells) {
+    if (status) atomicOr(status, kStatusCellRange);
+    c = c < 0 ? 0 : kCells - 1;
+  }
+  keys[f] = (uint32_t)c;
+  ids[f] = f;
+  face_cell[f] = c;
+  atomicAdd(&hist[c], 1);
+}
+
+__global__ void k_add_in_place(int n, const int32_t* __restrict__ a, int32_t* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] += a[i];
+}
+
+// ----------------------------------------------------------------- traversal
+
+// traverse_cell_bvh_single, lbvh_cell.py:661-714, for a batch of query points
+// p +- pad.  CELL: skip queries whose centre cell id exceeds 32767, skip empty
+// nodes, leaves list faces [l, r] of sorted; else triangle leaves (one face).
+// Children are pushed a then b, so b's subtree is visited first.  Candidates go
+// to cand[j * nq + q] (column-major) up to cap; overflow sets a status bit.
+template <bool CELL>
+__global__ void __launch_bounds__(kTravBS)
+k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32_t* __restrict__ num,
+           const int32_t* __restrict__ sorted, const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+           const float4* __restrict__ fmin, const float4* __restrict__ fmax, const float* __restrict__ box,
+           int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status) {
+  __shared__ int32_t s_stack[kStack * kTravBS];
+  const int q = blockIdx.x * kTravBS + threadIdx.x;
+  if (q >= nq) return;
+  int32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
+  const float3 p = xyz(q4[q]);
+  const float pp = pad * 1.0f;
+  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  int cnt = 0, flags = 0;
+  if (CELL) {
+    float3 c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+    if (flat_cell_id(c, cell_size_of(box), make_float3(box[0], box[1], box[2])) > kCells - 1) {
+      cand_n[q] = 0;
+      return;
+    }
+  }
+  if (nl <= 0) {
+    cand_n[q] = 0;
+    return;
+  }
+  int sp = 0;
+  stk[0] = 0;
+  sp = 1;
+  while (sp > 0) {
+    const int idx = stk[(--sp) * kTravBS];
+    if (CELL && num[idx] <= 0) continue;
+    const float4 lo = nlo[idx], hi = nhi[idx];
+    if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
+    if (idx >= nl - 1) {
+      const int l = __float_as_int(lo.w), r = __float_as_int(hi.w);
+      if (CELL) {
+        for (int j = l; j <= r; ++j) {
+          const int f = sorted[j];
+          if (overlap(a, b, xyz(fmin[f]), xyz(fmax[f]))) {
+            if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
+            else flags |= kStatusCandOverflow;
+            ++cnt;
+          }
+        }
+      } else {
+        // triangle leaf: its box is the face box, already tested
+        if (cnt < cap) cand[(size_t)cnt * nq + q] = l;
+        else flags |= kStatusCandOverflow;
+        ++cnt;
+      }
+    } else {
+      const int ca = __float_as_int(lo.w), cb = __float_as_int(hi.w);
+      if (!CELL || num[ca] > 0) {
+        if (sp < kStack) stk[(sp++) * kTravBS] = ca; else flags |= kStatusStackOverflow;
+      }
+      if (!CELL || num[cb] > 0) {
+        if (sp < kStack) stk[(sp++) * kTravBS] = cb; else flags |= kStatusStackOverflow;
+      }
+    }
+  }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (flags && status) atomicOr(status, flags);
+}
+
+}  // namespace slx
+
+using namespace slx;
+static inline hipStream_t S(slx_stream_t s) { return (hipStream_t)s; }
+
+extern "C" int slx_face_aabbs(int nf, const int32_t* faces, const float* x4, float padding, float* fmin4,
+                              float* fmax4, slx_stream_t stream) {
+  SLX_REQUIRE(nf >= 0, "slx_face_aabbs: nf < 0");
+  if (nf == 0) return SLX_OK;
+  SLX_REQUIRE(faces && x4 && fmin4 && fmax4, "slx_face_aabbs: null pointer");
+  SLX_LAUNCH("face_aabbs", k_face_aabbs, dim3(blocks_for(nf, 256)), dim3(256), 0, S(stream), nf, faces,
+             (const float4*)x4, padding, (float4*)fmin4, (float4*)fmax4);
+  return check_launch("k_face_aabbs");
+}
+
+extern "C" size_t slx_mesh_aabb_ws_bytes(void) { return sizeof(float) * 6 * kBoxParts; }
+
+extern "C" int slx_mesh_aabb(int nv, const float* x4, float padding, float* box6, void* ws, size_t ws_bytes,
+                             slx_stream_t stream) {
+  SLX_REQUIRE(nv >= 0, "slx_mesh_aabb: nv < 0");
+  SLX_REQUIRE(box6 && ws && (nv == 0 || x4), "slx_mesh_aabb: null pointer");
+  if (ws_bytes < sizeof(float) * 6 * kBoxParts) return fail(SLX_EWS, "slx_mesh_aabb: workspace too small");
+  int parts = blocks_for(nv > 0 ? nv : 1, kBoxBS);
+  if (parts > kBoxParts) parts = kBoxParts;
+  SLX_LAUNCH("box_partial", k_box_partial, dim3(parts), dim3(kBoxBS), 0, S(stream), nv, (const float4*)x4,
+             (float*)ws);
+  SLX_LAUNCH("box_final", k_box_final, dim3(1), dim3(64), 0, S(stream), parts, (const float*)ws, padding, box6);
+  return check_launch("slx_mesh_aabb");
+}
+
+extern "C" size_t slx_cell_topology_ws_bytes(void) { return radix_sort_ws_bytes(kCells); }
+
+extern "C" int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int32_t* child_b,
+                                 int32_t* parent, int32_t* range_l, int32_t* range_r, void* ws, size_t ws_bytes,
+                                 slx_stream_t stream) {
+  SLX_REQUIRE(codes && sorted_ids && child_a && child_b && parent && ws, "slx_cell_topology: null pointer");
+  hipStream_t st = S(stream);
+  SLX_LAUNCH("cell_morton", k_cell_morton, dim3(blocks_for(kCells, 256)), dim3(256), 0, st,
+             make_float3(0.f, 0.f, 0.f), make_float3(1e3f, 1e3f, 1e3f), (uint32_t*)codes, sorted_ids);
+  int rc = radix_sort_pairs(kCells, nullptr, (uint32_t*)codes, sorted_ids, 30, ws, ws_bytes, st);
+  if (rc) return rc;
+  SLX_LAUNCH("karras", k_karras, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells,
+             (const uint32_t*)codes, 0, child_a, child_b, parent, range_l, range_r);
+  return check_launch("slx_cell_topology");
+}
+
+extern "C" size_t slx_cell_build_ws_bytes(int nf) {
+  size_t n = (size_t)(nf > 0 ? nf : 1);
+  return 2 * n * 4 + radix_sort_ws_bytes(nf) + scan_ws_bytes(kCells) + 4 * (size_t)kCells * 2 + 1024;
+}
+
+extern "C" int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
+                              const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
+                              int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces, int32_t* num,
+                              int32_t* leaf_l, int32_t* leaf_r, float* nlo4, float* nhi4, int32_t* status,
+                              void* ws, size_t ws_bytes, slx_stream_t stream) {
+  SLX_REQUIRE(nf >= 0, "slx_cell_build: nf < 0");
+  SLX_REQUIRE(box6 && child_a && child_b && parent && face_cell && prefix && sorted_faces && num && leaf_l &&
+                  leaf_r && nlo4 && nhi4 && ws,
+              "slx_cell_build: null pointer");
+  SLX_REQUIRE(nf == 0 || (fmin4 && fmax4), "slx_cell_build: null face boxes");
+  if (ws_bytes < slx_cell_build_ws_bytes(nf)) return fail(SLX_EWS, "slx_cell_build: workspace too small");
+  hipStream_t st = S(stream);
+  char* p = (char*)ws;
+  size_t n = (size_t)(nf > 0 ? nf : 1);
+  uint32_t* keys = (uint32_t*)p; p += 4 * n;
+  int32_t* hist = (int32_t*)p; p += 4 * (size_t)kCells;
+  int32_t* flags = (int32_t*)p; p += 4 * (size_t)kCells;
+  void* sort_ws = p; size_t sort_bytes = radix_sort_ws_bytes(nf); p += sort_bytes;
+  void* scan_ws = p; size_t scan_bytes = scan_ws_bytes(kCells);
+  SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, hist, 0);
+  SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, flags, 0);
+  if (nf > 0) {
+    SLX_LAUNCH("face_cells", k_face_cells, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf,
+               (const float4*)fmin4, (const float4*)fmax4, box6, keys, sorted_faces, face_cell, hist, status);
+    int rc = radix_sort_pairs(nf, nullptr, keys, sorted_faces, 15, sort_ws, sort_bytes, st);
+    if (rc) return rc;
+  }
+  // inclusive prefix = exclusive scan + counts (lbvh_cell.py:644)
+  int rc = exclusive_scan(kCells, nullptr, hist, prefix, nullptr, scan_ws, scan_bytes, st);
+  if (rc) return rc;
+  SLX_LAUNCH("add", k_add_in_place, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, hist, prefix);
+  SLX_LAUNCH("cell_leaves", k_cell_leaves, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, nf,
+             (const float4*)fmin4, (const float4*)fmax4, box6, prefix, sorted_faces, child_a, child_b, parent,
+             leaf_l, leaf_r, num, (float4*)nlo4, (float4*)nhi4, flags);
+  return check_launch("slx_cell_build");
+}
+
+extern "C" size_t slx_tri_build_ws_bytes(int nf) {
+  size_t n = (size_t)(nf > 0 ? nf : 1);
+  return 2 * n * 4 + radix_sort_ws_bytes(nf) + 1024;
+}
+
+extern "C" int slx_tri_refit(int nf, const float* fmin4, const float* fmax4, const int32_t* sorted_faces,
+                             const int32_t* child_a, const int32_t* child_b, const int32_t* parent, float* nlo4,
+                             float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
+  SLX_REQUIRE(nf >= 0, "slx_tri_refit: nf < 0");
+  if (nf == 0) return SLX_OK;
+  SLX_REQUIRE(fmin4 && fmax4 && sorted_faces && child_a && child_b && parent && nlo4 && nhi4 && ws,
+              "slx_tri_refit: null pointer");
+  if (ws_bytes < 4 * (size_t)nf) return fail(SLX_EWS, "slx_tri_refit: workspace too small");
+  hipStream_t st = S(stream);
+  int32_t* flags = (int32_t*)ws;
+  SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, flags, 0);
+  SLX_LAUNCH("tri_refit", k_tri_refit, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const float4*)fmin4,
+             (const float4*)fmax4, sorted_faces, child_a, child_b, parent, (float4*)nlo4, (float4*)nhi4, flags);
+  return check_launch("slx_tri_refit");
+}
+
+extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6, int32_t* codes,
+                             int32_t* sorted_faces, int32_t* child_a, int32_t* child_b, int32_t* parent,
+                             float* nlo4, float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
+  SLX_REQUIRE(nf >= 0, "slx_tri_build: nf < 0");
+  if (nf == 0) return SLX_OK;
+  SLX_REQUIRE(fmin4 && fmax4 && box6 && codes && sorted_faces && child_a && child_b && parent && nlo4 && nhi4 &&
+                  ws,
+              "slx_tri_build: null pointer");
+  if (ws_bytes < slx_tri_build_ws_bytes(nf)) return fail(SLX_EWS, "slx_tri_build: workspace too small");
+  hipStream_t st = S(stream);
+  SLX_LAUNCH("face_morton", k_face_morton, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf,
+             (const float4*)fmin4, (const float4*)fmax4, box6, (uint32_t*)codes, sorted_faces);
+  int rc = radix_sort_pairs(nf, nullptr, (uint32_t*)codes, sorted_faces, 30, (char*)ws + 8 * (size_t)nf + 512,
+                            ws_bytes - 8 * (size_t)nf - 512, st);
+  if (rc) return rc;
+  SLX_LAUNCH("karras", k_karras, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const uint32_t*)codes, 1,
+             child_a, child_b, parent, (int32_t*)nullptr, (int32_t*)nullptr);
+  rc = check_launch("k_karras");
+  if (rc) return rc;
+  return slx_tri_refit(nf, fmin4, fmax4, sorted_faces, child_a, child_b, parent, nlo4, nhi4, ws, ws_bytes, stream);
+}
+
+extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
+                            const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* fmin4,
+                            const float* fmax4, const float* box6, int cap, int32_t* cand, int32_t* cand_n,
+                            int32_t* status, slx_stream_t stream) {
+  SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
+  SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
+  if (nq == 0) return SLX_OK;
+  SLX_REQUIRE(q4 && cand_n && (cap == 0 || cand), "slx_traverse: null pointer");
+  SLX_REQUIRE(n_leaves <= 0 || (nlo4 && nhi4), "slx_traverse: null tree");
+  SLX_REQUIRE(mode == 1 || (num && sorted_faces && fmin4 && fmax4 && box6), "slx_traverse: null cell tree");
+  hipStream_t st = S(stream);
+  if (mode == 0)
+    SLX_LAUNCH("traverse_cell", k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+               (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
+               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
+  else
+    SLX_LAUNCH("traverse_tri", k_traverse<false>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+               (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
+               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
+  return check_launch("slx_traverse");
+}
