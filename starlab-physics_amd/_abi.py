@@ -31,8 +31,54 @@ SIGNATURES = {
     "slx_substeps_plain": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, I32,
                            F32, F32, F32, F32, F32, F32, F32, STREAM],
     "slx_set_fixed": [I32, PTR, PTR, PTR, PTR, PTR, STREAM],
+    "slx_scan_ws_bytes": [I32],
+    "slx_exclusive_scan": [I32, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_radix_sort_ws_bytes": [I32],
+    "slx_radix_sort_pairs": [I32, PTR, PTR, PTR, I32, PTR, C.c_size_t, STREAM],
+    "slx_face_aabbs": [I32, PTR, PTR, F32, PTR, PTR, STREAM],
+    "slx_mesh_aabb_ws_bytes": [],
+    "slx_mesh_aabb": [I32, PTR, F32, PTR, PTR, C.c_size_t, STREAM],
+    "slx_cell_topology_ws_bytes": [],
+    "slx_cell_topology": [PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_cell_build_ws_bytes": [I32],
+    "slx_cell_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
+                       PTR, C.c_size_t, STREAM],
+    "slx_tri_build_ws_bytes": [I32],
+    "slx_tri_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_tri_refit": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_collision_ws_bytes": [I32, I32, I32],
+    "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
+                           PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
+    "slx_frame_ws_bytes": [PTR],
+    "slx_frame": [PTR, STREAM],
 }
-_RET = {"slx_last_error": C.c_char_p}
+_RET = {"slx_last_error": C.c_char_p, "slx_scan_ws_bytes": C.c_size_t,
+        "slx_radix_sort_ws_bytes": C.c_size_t, "slx_mesh_aabb_ws_bytes": C.c_size_t,
+        "slx_cell_topology_ws_bytes": C.c_size_t, "slx_cell_build_ws_bytes": C.c_size_t,
+        "slx_tri_build_ws_bytes": C.c_size_t, "slx_collision_ws_bytes": C.c_size_t,
+        "slx_frame_ws_bytes": C.c_size_t}
+
+
+class Tree(C.Structure):
+    """slx_tree (include/slx.h)."""
+    _fields_ = [("mode", I32), ("nf", I32), ("fmin4", PTR), ("fmax4", PTR), ("box6", PTR),
+                ("codes", PTR), ("sorted_faces", PTR), ("child_a", PTR), ("child_b", PTR),
+                ("parent", PTR), ("num", PTR), ("face_cell", PTR), ("prefix", PTR), ("leaf_l", PTR),
+                ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR)]
+
+
+class FrameDesc(C.Structure):
+    """slx_frame_desc (include/slx.h)."""
+    _fields_ = [("nv", I32), ("x4", PTR), ("v4", PTR), ("y4", PTR), ("y4_alt", PTR), ("fixed", PTR),
+                ("m_inv", PTR), ("faces", PTR), ("adj_slice_off", PTR), ("adj_slice_width", PTR),
+                ("adj_entries", PTR), ("nvs", I32), ("xs4", PTR), ("faces_st", PTR),
+                ("dt", F32), ("gx", F32), ("gy", F32), ("gz", F32), ("k_s", F32), ("k_b", F32),
+                ("damping", F32), ("padding", F32), ("dhat", F32), ("k_col", F32),
+                ("n_substeps", I32), ("n_iters", I32), ("collision", I32), ("refit_every_substep", I32),
+                ("rebuild_static", I32), ("cap", I32), ("rec_cap", I32), ("dyn", Tree), ("st", Tree),
+                ("cand", PTR), ("cand_n", PTR), ("pair_num", PTR), ("status", PTR), ("ws", PTR),
+                ("ws_bytes", C.c_size_t)]
 
 
 class SlxError(RuntimeError):

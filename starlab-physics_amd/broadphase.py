@@ -1,0 +1,136 @@
+"""Device LBVH over a mesh's triangles: the reference's LBVH_CELL
+(framework/collision/lbvh_cell.py:17-80, build 637-653, traversal 661-714) and a
+per-triangle LBVH, both backed by libslx.so.
+
+    tree = DeviceTree(mesh, mode="cell")   # or "tri"
+    tree.build(padding)                     # from mesh.x4
+    cand, n = tree.query(points4, padding, cap)
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _abi
+
+NCELLS = 32768
+_TOPO = {}
+
+
+def cell_topology(device):
+    """Static 32^3 cell tree (lbvh_cell.py:78-80), built once per device."""
+    key = str(device)
+    if key not in _TOPO:
+        i32 = dict(dtype=torch.int32, device=device)
+        t = dict(codes=torch.empty(NCELLS, **i32), sorted_ids=torch.empty(NCELLS, **i32),
+                 child_a=torch.empty(NCELLS - 1, **i32), child_b=torch.empty(NCELLS - 1, **i32),
+                 parent=torch.empty(2 * NCELLS - 1, **i32), range_l=torch.empty(NCELLS - 1, **i32),
+                 range_r=torch.empty(NCELLS - 1, **i32))
+        ws = torch.empty(_abi.lib().slx_cell_topology_ws_bytes(), dtype=torch.uint8, device=device)
+        _abi.call("slx_cell_topology", *[_abi.ptr(t[k]) for k in
+                                         ("codes", "sorted_ids", "child_a", "child_b", "parent", "range_l",
+                                          "range_r")], _abi.ptr(ws), ws.numel(), _abi.stream_handle())
+        _TOPO[key] = t
+    return _TOPO[key]
+
+
+class DeviceTree:
+    def __init__(self, mesh, mode="cell"):
+        if mode not in ("cell", "tri"):
+            raise ValueError("mode must be 'cell' or 'tri'")
+        self.mesh = mesh
+        self.mode = mode
+        dev = mesh.x4.device
+        nf = mesh.num_faces
+        self.nf = nf
+        f32 = dict(dtype=torch.float32, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.fmin4 = torch.zeros((max(nf, 1), 4), **f32)
+        self.fmax4 = torch.zeros((max(nf, 1), 4), **f32)
+        self.box6 = torch.zeros(8, **f32)
+        self.sorted_faces = torch.zeros(max(nf, 1), **i32)
+        self.faces = mesh.face_indices
+        if mode == "cell":
+            topo = cell_topology(dev)
+            self.child_a, self.child_b, self.parent = topo["child_a"], topo["child_b"], topo["parent"]
+            nn = 2 * NCELLS - 1
+            self.num = torch.zeros(nn, **i32)
+            self.face_cell = torch.zeros(max(nf, 1), **i32)
+            self.prefix = torch.zeros(NCELLS, **i32)
+            self.leaf_l = torch.zeros(NCELLS, **i32)
+            self.leaf_r = torch.zeros(NCELLS, **i32)
+            self.codes = None
+            self.n_leaves = NCELLS
+            ws = _abi.lib().slx_cell_build_ws_bytes(nf)
+        else:
+            nn = max(2 * nf - 1, 1)
+            self.child_a = torch.zeros(max(nf - 1, 1), **i32)
+            self.child_b = torch.zeros(max(nf - 1, 1), **i32)
+            self.parent = torch.zeros(nn, **i32)
+            self.codes = torch.zeros(max(nf, 1), **i32)
+            self.num = self.face_cell = self.prefix = self.leaf_l = self.leaf_r = None
+            self.n_leaves = nf
+            ws = _abi.lib().slx_tri_build_ws_bytes(nf)
+        self.nlo4 = torch.zeros((nn, 4), **f32)
+        self.nhi4 = torch.zeros((nn, 4), **f32)
+        ws = max(ws, _abi.lib().slx_mesh_aabb_ws_bytes())
+        self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
+        self.status = torch.zeros(1, **i32)
+
+    def struct(self) -> _abi.Tree:
+        P = _abi.ptr
+        return _abi.Tree(0 if self.mode == "cell" else 1, self.nf, P(self.fmin4), P(self.fmax4), P(self.box6),
+                         P(self.codes), P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent),
+                         P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
+                         P(self.nlo4), P(self.nhi4))
+
+    # --- stages (the frame driver calls the same entry points itself) ---
+    def face_aabbs(self, padding, x4=None):
+        x4 = self.mesh.x4 if x4 is None else x4
+        _abi.call("slx_face_aabbs", self.nf, _abi.ptr(self.faces), _abi.ptr(x4), float(padding),
+                  _abi.ptr(self.fmin4), _abi.ptr(self.fmax4), _abi.stream_handle())
+
+    def mesh_aabb(self, padding, x4=None):
+        x4 = self.mesh.x4 if x4 is None else x4
+        _abi.call("slx_mesh_aabb", self.mesh.num_verts, _abi.ptr(x4), float(padding), _abi.ptr(self.box6),
+                  _abi.ptr(self.ws), self.ws.numel(), _abi.stream_handle())
+
+    def build(self, padding, x4=None):
+        """computeAABB_faces + computeAABB + LBVH build from x (XPBD.py:450-452)."""
+        P = _abi.ptr
+        self.face_aabbs(padding, x4)
+        self.mesh_aabb(padding, x4)
+        s = _abi.stream_handle()
+        if self.mode == "cell":
+            _abi.call("slx_cell_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.child_a),
+                      P(self.child_b), P(self.parent), P(self.face_cell), P(self.prefix), P(self.sorted_faces),
+                      P(self.num), P(self.leaf_l), P(self.leaf_r), P(self.nlo4), P(self.nhi4), P(self.status),
+                      P(self.ws), self.ws.numel(), s)
+        else:
+            _abi.call("slx_tri_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.codes),
+                      P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent), P(self.nlo4),
+                      P(self.nhi4), P(self.ws), self.ws.numel(), s)
+
+    def refit(self, padding, x4=None):
+        """Triangle tree: new face boxes, same topology."""
+        assert self.mode == "tri"
+        P = _abi.ptr
+        self.face_aabbs(padding, x4)
+        _abi.call("slx_tri_refit", self.nf, P(self.fmin4), P(self.fmax4), P(self.sorted_faces), P(self.child_a),
+                  P(self.child_b), P(self.parent), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(),
+                  _abi.stream_handle())
+
+    def query(self, q4, padding, cap, cand=None, cand_n=None, status=None):
+        """Candidate faces of boxes q4.xyz -+ padding, column-major (cap, nq)."""
+        nq = q4.shape[0]
+        dev = q4.device
+        if cand is None:
+            cand = torch.full((max(cap, 1), max(nq, 1)), -1, dtype=torch.int32, device=dev)
+        if cand_n is None:
+            cand_n = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)
+        st = self.status if status is None else status
+        P = _abi.ptr
+        _abi.call("slx_traverse", 0 if self.mode == "cell" else 1, nq, P(q4), float(padding),
+                  self.n_leaves if self.nf > 0 else 0, P(self.num), P(self.sorted_faces), P(self.nlo4),
+                  P(self.nhi4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n), P(st),
+                  _abi.stream_handle())
+        return cand, cand_n

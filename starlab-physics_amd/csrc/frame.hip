@@ -1,0 +1,153 @@
+// One frame of the cloth step, orchestrated on the host side of the library:
+// the sequence of Solver.forward (framework/physics/XPBD.py:443-469).
+//
+//  if collision: rebuild the dynamic tree from x (and the static tree when asked)
+//  per substep:  [triangle tree: refit from x]
+//                spring pass with compute_y fused in
+//                per iteration: [spring pass] broad phase (3 query sets),
+//                               collision pass
+//                compute_velocity + update_x
+//  without collision the substeps run as two fused kernels per substep
+//  (slx_substeps_plain).
+//
+// Every size the kernels need is either static or device-resident, so the
+// frame enqueues work without any host synchronisation and can be captured in
+// a HIP graph.
+#include "slx_common.h"
+
+extern "C" {
+int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, float* v4_alt, float* y4a, float* y4b,
+                       const float* fixed, const int32_t* slice_off, const int32_t* slice_width,
+                       const int32_t* entries, int n_substeps, int n_iters, float dt, float gx, float gy, float gz,
+                       float k_s, float k_b, float damping, slx_stream_t stream);
+}
+
+namespace {
+
+using namespace slx;
+
+size_t tree_ws(const slx_tree& t) {
+  if (t.nf <= 0) return 0;
+  return t.mode == 0 ? slx_cell_build_ws_bytes(t.nf) : slx_tri_build_ws_bytes(t.nf);
+}
+
+int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces, float pad, int32_t* status,
+               void* ws, size_t ws_bytes, slx_stream_t s) {
+  if (t.nf <= 0) return SLX_OK;
+  int rc = slx_face_aabbs(t.nf, faces, x4, pad, t.fmin4, t.fmax4, s);
+  if (rc) return rc;
+  rc = slx_mesh_aabb(nv, x4, pad, t.box6, ws, ws_bytes, s);
+  if (rc) return rc;
+  if (t.mode == 0)
+    return slx_cell_build(t.nf, t.fmin4, t.fmax4, t.box6, t.child_a, t.child_b, t.parent, t.face_cell, t.prefix,
+                          t.sorted_faces, t.num, t.leaf_l, t.leaf_r, t.nlo4, t.nhi4, status, ws, ws_bytes, s);
+  return slx_tri_build(t.nf, t.fmin4, t.fmax4, t.box6, t.codes, t.sorted_faces, t.child_a, t.child_b, t.parent,
+                       t.nlo4, t.nhi4, ws, ws_bytes, s);
+}
+
+int query(const char* name, const slx_tree& t, int nq, const float* q4, float pad, int cap, int32_t* cand,
+          int32_t* cand_n, int32_t* status, slx_stream_t s) {
+  int nl = t.mode == 0 ? kCells : t.nf;
+  if (t.nf <= 0) nl = 0;
+  return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4, t.fmin4, t.fmax4,
+                         t.box6, cap, cand, cand_n, status, (hipStream_t)s);
+}
+
+}  // namespace
+
+extern "C" size_t slx_frame_ws_bytes(const slx_frame_desc* d) {
+  if (!d) return 0;
+  if (!d->collision) return 2 * sizeof(float4) * (size_t)(d->nv > 0 ? d->nv : 0);
+  size_t w = slx_mesh_aabb_ws_bytes();
+  if (d->collision) {
+    size_t a = tree_ws(d->dyn), b = tree_ws(d->st);
+    size_t c = slx_collision_ws_bytes(d->nv, d->nvs, d->rec_cap);
+    if (a > w) w = a;
+    if (b > w) w = b;
+    if (c > w) w = c;
+  }
+  return w;
+}
+
+extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
+  SLX_REQUIRE(d, "slx_frame: null descriptor");
+  SLX_REQUIRE(d->nv >= 0 && d->nvs >= 0 && d->n_substeps >= 0 && d->n_iters >= 1, "slx_frame: bad sizes");
+  if (d->nv == 0 || d->n_substeps == 0) return SLX_OK;
+  SLX_REQUIRE(d->x4 && d->v4 && d->y4 && d->y4_alt && d->fixed && d->m_inv && d->adj_slice_off &&
+                  d->adj_slice_width && d->adj_entries,
+              "slx_frame: null state pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (!d->collision) {
+    // x4_alt / v4_alt: the caller's y4_alt doubles as one ping-pong buffer is not
+    // possible (four buffers are needed); use the workspace for the alternates.
+    size_t need = 2 * sizeof(float4) * (size_t)d->nv;
+    SLX_REQUIRE(d->ws && d->ws_bytes >= need, "slx_frame: workspace too small for the collision-free path");
+    float* xa = (float*)d->ws;
+    float* va = xa + 4 * (size_t)d->nv;
+    return slx_substeps_plain(d->nv, d->x4, d->v4, xa, va, d->y4, d->y4_alt, d->fixed, d->adj_slice_off,
+                              d->adj_slice_width, d->adj_entries, d->n_substeps, d->n_iters, d->dt, d->gx, d->gy,
+                              d->gz, d->k_s, d->k_b, d->damping, stream);
+  }
+  SLX_REQUIRE(d->faces && d->cand && d->cand_n && d->ws, "slx_frame: null collision pointer");
+  SLX_REQUIRE(d->dyn.mode == 0 || d->dyn.mode == 1, "slx_frame: bad tree mode");
+  SLX_REQUIRE(d->nvs == 0 || (d->xs4 && d->faces_st), "slx_frame: static mesh missing");
+  SLX_REQUIRE(d->ws_bytes >= slx_frame_ws_bytes(d), "slx_frame: workspace too small");
+  const int nv = d->nv, nvs = d->nvs, cap = d->cap;
+  const float pad = d->padding;
+  int rc;
+  // per-frame rebuild from x (XPBD.py:448-458)
+  rc = build_tree(d->dyn, nv, d->x4, d->faces, pad, d->status, d->ws, d->ws_bytes, stream);
+  if (rc) return rc;
+  if (nvs > 0 && d->rebuild_static) {
+    rc = build_tree(d->st, nvs, d->xs4, d->faces_st, pad, d->status, d->ws, d->ws_bytes, stream);
+    if (rc) return rc;
+  }
+  int32_t *c0 = d->cand, *c1 = d->cand + (size_t)cap * nv, *c2 = d->cand + 2 * (size_t)cap * nv;
+  int32_t *n0 = d->cand_n, *n1 = d->cand_n + nv, *n2 = d->cand_n + 2 * (size_t)nv;
+  int32_t* pn = d->pair_num;
+  float4* ycur = (float4*)d->y4;
+  float4* yalt = (float4*)d->y4_alt;
+  bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
+  for (int s = 0; s < d->n_substeps; ++s) {
+    if (s > 0 && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0) {
+      rc = slx_face_aabbs(d->dyn.nf, d->faces, d->x4, pad, d->dyn.fmin4, d->dyn.fmax4, stream);
+      if (rc) return rc;
+      rc = slx_tri_refit(d->dyn.nf, d->dyn.fmin4, d->dyn.fmax4, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
+                         d->dyn.parent, d->dyn.nlo4, d->dyn.nhi4, d->ws, d->ws_bytes, stream);
+      if (rc) return rc;
+      tv_fresh = false;
+    }
+    for (int it = 0; it < d->n_iters; ++it) {
+      // spring pass (compute_y fused into the first)
+      rc = spring_launch(it == 0, false, nv, (const float4*)d->x4, (const float4*)d->v4, d->fixed, ycur, yalt,
+                         nullptr, nullptr, d->adj_slice_off, d->adj_slice_width, d->adj_entries, d->k_s, d->k_b, d->dt,
+                         d->gx, d->gy, d->gz, d->damping, st);
+      if (rc) return rc;
+      float4* t = ycur; ycur = yalt; yalt = t;
+      // broad phase (XPBD.py:186-215)
+      if (nvs > 0) {
+        rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, pad, cap, c0, n0, d->status, stream);
+        if (rc) return rc;
+      }
+      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, pad, cap, c1, n1, d->status, stream);
+      if (rc) return rc;
+      if (nvs > 0 && !tv_fresh) {
+        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, pad, cap, c2, n2, d->status, stream);
+        if (rc) return rc;
+        tv_fresh = true;
+      }
+      // narrow phase + projection + update_dx (XPBD.py:236-264, 365-369)
+      rc = slx_collision_pass(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
+                              d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
+                              d->rec_cap, d->status, d->ws, d->ws_bytes, stream);
+      if (rc) return rc;
+    }
+    rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
+    if (rc) return rc;
+  }
+  if (ycur != (float4*)d->y4) {
+    if (hipMemcpyAsync(d->y4, ycur, sizeof(float4) * (size_t)nv, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return fail(SLX_EHIP, "slx_frame: y copy failed");
+  }
+  return SLX_OK;
+}

@@ -515,24 +515,33 @@ extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, con
   return slx_tri_refit(nf, fmin4, fmax4, sorted_faces, child_a, child_b, parent, nlo4, nhi4, ws, ws_bytes, stream);
 }
 
-extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
-                            const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* fmin4,
-                            const float* fmax4, const float* box6, int cap, int32_t* cand, int32_t* cand_n,
-                            int32_t* status, slx_stream_t stream) {
+namespace slx {
+int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
+                    const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
+                    const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
+                    int32_t* cand_n, int32_t* status, hipStream_t st) {
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
   if (nq == 0) return SLX_OK;
   SLX_REQUIRE(q4 && cand_n && (cap == 0 || cand), "slx_traverse: null pointer");
   SLX_REQUIRE(n_leaves <= 0 || (nlo4 && nhi4), "slx_traverse: null tree");
   SLX_REQUIRE(mode == 1 || (num && sorted_faces && fmin4 && fmax4 && box6), "slx_traverse: null cell tree");
-  hipStream_t st = S(stream);
   if (mode == 0)
-    SLX_LAUNCH("traverse_cell", k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+    SLX_LAUNCH(name, k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
                (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
   else
-    SLX_LAUNCH("traverse_tri", k_traverse<false>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+    SLX_LAUNCH(name, k_traverse<false>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
                (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
   return check_launch("slx_traverse");
+}
+}  // namespace slx
+
+extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
+                            const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* fmin4,
+                            const float* fmax4, const float* box6, int cap, int32_t* cand, int32_t* cand_n,
+                            int32_t* status, slx_stream_t stream) {
+  return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
+                         sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream));
 }

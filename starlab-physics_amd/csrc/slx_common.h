@@ -52,6 +52,19 @@ inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 int fail(int code, const char* fmt, ...);
 int check_launch(const char* what);
 
+// Jacobi spring pass launcher (xpbd.hip): pred = compute_y fused in (reads x4, v4),
+// final = velocity/position update fused in (writes x_out, v_out)
+int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4* v4, const float* fixed,
+                  const float4* y_in, float4* y_out, float4* x_out, float4* v_out, const int32_t* slice_off,
+                  const int32_t* slice_width, const int32_t* entries, float k_s, float k_b, float dt, float gx,
+                  float gy, float gz, float damping, hipStream_t st);
+
+// stack traversal launcher (lbvh.hip); name is the kernel timer label
+int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
+                    const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
+                    const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
+                    int32_t* cand_n, int32_t* status, hipStream_t st);
+
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);
 void timer_end(int slot, hipStream_t s);

@@ -154,6 +154,29 @@ k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
   }
 }
 
+int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4* v4, const float* fixed,
+                  const float4* y_in, float4* y_out, float4* x_out, float4* v_out, const int32_t* slice_off,
+                  const int32_t* slice_width, const int32_t* entries, float k_s, float k_b, float dt, float gx,
+                  float gy, float gz, float damping, hipStream_t st) {
+  if (nv <= 0) return SLX_OK;
+  const float3 gdt2 = make_float3((gx * dt) * dt, (gy * dt) * dt, (gz * dt) * dt);
+  const dim3 grid(blocks_for(nv, 256)), blk(256);
+  const int2* E = (const int2*)entries;
+  if (pred && final)
+    SLX_LAUNCH("spring_pred_final", (k_spring<true, true>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
+               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+  else if (pred)
+    SLX_LAUNCH("spring_pred", (k_spring<true, false>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
+               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+  else if (final)
+    SLX_LAUNCH("spring_final", (k_spring<false, true>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
+               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+  else
+    SLX_LAUNCH("spring", (k_spring<false, false>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out, v_out,
+               slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+  return check_launch("k_spring");
+}
+
 }  // namespace slx
 
 using namespace slx;

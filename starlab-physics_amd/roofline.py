@@ -1,16 +1,17 @@
 """Algorithmic HBM byte model of the XPBD cloth step (FP32 / int32), the single
 source of truth for roofline figures.
 
-Per vertex-array pass each array is counted once; on-chip temporaries (dx, nc)
-are not counted.  V/E/B/F are the dynamic mesh's vertex, edge, bending and face
-counts; c is the mean number of broad-phase candidates per vertex.
+Each array is counted once per pass; on-chip temporaries (dx, nc) are not
+counted.  V/E/B/F are the dynamic mesh's vertex, edge, bending and face counts,
+Vs/Fs the static mesh's; cbar[set] is the measured mean number of broad-phase
+candidates per query of a query set.
 
   predict                      40 V   (read x, v, f; write y)
   one Jacobi spring pass       12 (E+B) + 40 V   (ids + rest length; read y, x, f*w; write y)
   velocity / position update   52 V   (read x, y, f; write v, x)
   refit                        76 F + 12 V
-  traversal                    16 V + 4 c V + 32 (2F - 1)
-  narrow phase                 16 c V + 24 V
+  traversal (nq queries, tree of Ft faces)   16 nq + 4 c nq + 32 (2 Ft - 1)
+  narrow phase                 16 c V + 24 V  (per query set of V queries)
   per-frame rebuild            Morton 20 F + 12 V, radix sort 68 F, Karras 16 F
 """
 from __future__ import annotations
@@ -34,36 +35,59 @@ def refit(V, F):
     return 76 * F + 12 * V
 
 
-def traversal(V, F, c):
-    return 16 * V + 4 * c * V + 32 * (2 * F - 1)
+def traversal(nq, Ft, c):
+    return 16 * nq + 4 * c * nq + 32 * (2 * Ft - 1)
 
 
-def narrow(V, c):
-    return 16 * c * V + 24 * V
+def narrow(nq, c):
+    return 16 * c * nq + 24 * nq
 
 
 def rebuild(V, F):
     return (20 * F + 12 * V) + 68 * F + 16 * F
 
 
-def kernel_bytes(name: str, V, E, B) -> int:
-    """Algorithmic bytes of one launch of a Jacobi kernel variant."""
+def kernel_bytes(name: str, n: dict):
+    """Algorithmic bytes of one launch of kernel `name` (None if not modelled)."""
+    V, E, B, F = n["V"], n["E"], n["B"], n["F"]
+    c = n.get("cbar", {})
     sp = spring_pass(V, E, B)
-    return {
+    table = {
         "compute_y": predict(V),
         "spring": sp,
         "spring_pred": predict(V) + sp,
         "spring_final": sp + update_vx(V),
         "spring_pred_final": predict(V) + sp + update_vx(V),
         "update_vx": update_vx(V),
-    }[name]
+        "tri_refit": refit(V, F),
+    }
+    if "Vs" in n:
+        table["traverse_vt_st"] = traversal(V, n["Fs"], c.get("vt_st", 0.0))
+        table["traverse_tv_st"] = traversal(n["Vs"], F, c.get("tv_st", 0.0))
+    table["traverse_vt_dy"] = traversal(V, F, c.get("vt_dy", 0.0))
+    nar = narrow(V, c.get("vt_dy", 0.0))
+    if "Vs" in n:
+        nar += narrow(V, c.get("vt_st", 0.0)) + narrow(n["Vs"], c.get("tv_st", 0.0))
+    table["contact_count"] = nar
+    table["contact_emit"] = nar
+    return table.get(name)
 
 
-def substep_bytes(V, E, B, F=0, n_iters=1, collision=False, c=0.0, substeps_per_frame=20):
-    """Bytes of one substep; with collision, the per-frame rebuild is amortised
-    over the frame's substeps."""
-    b = predict(V) + n_iters * spring_pass(V, E, B) + update_vx(V)
+def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
+    """Bytes of one frame (`substeps` substeps)."""
+    V, E, B, F = n["V"], n["E"], n["B"], n["F"]
+    sub = predict(V) + n_iters * spring_pass(V, E, B) + update_vx(V)
+    total = substeps * sub
     if collision:
-        b += refit(V, F) + n_iters * (traversal(V, F, c) + narrow(V, c))
-        b += rebuild(V, F) / substeps_per_frame
-    return b
+        c = n.get("cbar", {})
+        per_iter = traversal(V, F, c.get("vt_dy", 0.0)) + narrow(V, c.get("vt_dy", 0.0))
+        if "Vs" in n:
+            per_iter += traversal(V, n["Fs"], c.get("vt_st", 0.0)) + narrow(V, c.get("vt_st", 0.0))
+            per_iter += narrow(n["Vs"], c.get("tv_st", 0.0))
+            total += substeps * traversal(n["Vs"], F, c.get("tv_st", 0.0))
+        total += substeps * (refit(V, F) + n_iters * per_iter) + rebuild(V, F)
+    return total
+
+
+def substep_bytes(V, E, B, n_iters=1):
+    return predict(V) + n_iters * spring_pass(V, E, B) + update_vx(V)

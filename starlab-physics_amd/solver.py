@@ -2,14 +2,23 @@
 
 Mirrors framework/physics/XPBD.py:9-106 (constructor and attributes),
 forward (443-469), reset (108-112) and set_fixed_vertices (389-395).
-Extensions: forward(n_substeps, n_iters) runs n_iters spring passes per
-substep (the reference runs exactly one, XPBD.py:463); `step` is an alias.
+
+Extensions (all default to the reference's behaviour):
+  forward(n_substeps, n_iters=1)   n_iters spring passes per substep (the
+                                   reference runs one, XPBD.py:463)
+  bvh_mode = "cell" | "tri"        the reference's 32^3 cell LBVH, or one leaf
+                                   per triangle (needed for large cloths)
+  refit_every_substep              triangle tree: refit from x at each substep
+  cache_size (40)                  candidate capacity per query (XPBD.py:51)
+  status()                         device status word (overflow flags)
+`step` is an alias of `forward`.
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import _abi
+from .broadphase import DeviceTree
 
 
 def _f32(v) -> float:
@@ -42,13 +51,20 @@ class Solver:
         self.max_num_verts_st = mesh_st.num_verts if mesh_st is not None else 0
         self.max_num_edges_st = mesh_st.num_edges if mesh_st is not None else 0
         self.max_num_faces_st = mesh_st.num_faces if mesh_st is not None else 0
-        # build extensions
+        # extensions
         self.n_iters = 1
+        self.bvh_mode = "cell"
+        self.refit_every_substep = False
+        self.cache_size = 40
+        self.compliance_collision = 1e8          # XPBD.py:367
+        self.records_per_query = 8
         dev = mesh_dy.x4.device
+        self.device = dev
         nv = mesh_dy.num_verts
-        self._x_alt = torch.empty((nv, 4), dtype=torch.float32, device=dev)
-        self._v_alt = torch.empty((nv, 4), dtype=torch.float32, device=dev)
         self._y_alt = torch.empty((nv, 4), dtype=torch.float32, device=dev)
+        self._status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._cd = None            # collision buffers, allocated on first use
+        self._ws = None
 
     # ------------------------------------------------------------ reference API
     def reset(self):
@@ -66,9 +82,9 @@ class Solver:
                   _abi.ptr(m.v4), _abi.ptr(m.y4), _abi.stream_handle())
 
     def scalars(self, n_substeps):
-        """Per-substep scalars exactly as the reference forms them: dt_sub and
+        """Per-substep scalars as the reference forms them: dt_sub and
         stiffness*dt_sub^2 in double, rounded to float32 at the kernel call
-        (XPBD.py:446, 354-355 — stiffness names are swapped there)."""
+        (XPBD.py:446, 354-355 — the stiffness names are swapped there)."""
         dt_sub = self.dt / n_substeps
         return dict(dt=_f32(dt_sub), k_s=_f32(self.stiffness_bending * dt_sub * dt_sub),
                     k_b=_f32(self.stiffness_stretch * dt_sub * dt_sub), damping=_f32(self.damping),
@@ -76,20 +92,78 @@ class Solver:
 
     def forward(self, n_substeps, n_iters=None):
         n_iters = self.n_iters if n_iters is None else n_iters
-        if self.enable_collision_handling:
-            raise NotImplementedError("collision path not built yet")
-        self._forward_plain(n_substeps, n_iters)
+        d = self._desc(n_substeps, n_iters)
+        _abi.call("slx_frame", _abi.C.byref(d), _abi.stream_handle())
 
     step = forward
 
-    def _forward_plain(self, n_substeps, n_iters):
+    def status(self) -> int:
+        """Device status word (SLX_STATUS_* bits); synchronises."""
+        return int(self._status.item())
+
+    def clear_status(self):
+        self._status.zero_()
+
+    # --------------------------------------------------------------- internals
+    def _collision_buffers(self):
+        import torch
+        key = (self.bvh_mode, self.cache_size, self.records_per_query)
+        if self._cd is not None and self._cd["key"] == key:
+            return self._cd
+        m, s = self.mesh_dy, self.mesh_st
+        nv, nvs = m.num_verts, (s.num_verts if s is not None else 0)
+        dev = self.device
+        cap = self.cache_size
+        i32 = dict(dtype=torch.int32, device=dev)
+        cd = dict(key=key)
+        cd["dyn"] = DeviceTree(m, self.bvh_mode)
+        cd["st"] = DeviceTree(s, self.bvh_mode) if s is not None else None
+        cd["cand"] = torch.full((cap * (2 * nv + nvs) + 1,), -1, **i32)
+        cd["cand_n"] = torch.zeros(2 * nv + nvs + 1, **i32)
+        cd["pair_num"] = torch.zeros(2 * nv + nvs + 1, **i32)
+        cd["rec_cap"] = int(self.records_per_query * (2 * nv + nvs) + 1024)
+        cd["static_built"] = False
+        if s is not None:
+            s.xs4 = s.x4
+        self._cd = cd
+        return cd
+
+    def _desc(self, n_substeps, n_iters):
+        import torch
         m = self.mesh_dy
-        s = self.scalars(n_substeps)
-        _abi.call("slx_substeps_plain", m.num_verts, _abi.ptr(m.x4), _abi.ptr(m.v4),
-                  _abi.ptr(self._x_alt), _abi.ptr(self._v_alt), _abi.ptr(m.y4), _abi.ptr(self._y_alt),
-                  _abi.ptr(m.fixed), _abi.ptr(m.adj_slice_off), _abi.ptr(m.adj_slice_width),
-                  _abi.ptr(m.adj_entries), n_substeps, n_iters, s["dt"], *s["g"], s["k_s"], s["k_b"],
-                  s["damping"], _abi.stream_handle())
+        sc = self.scalars(n_substeps)
+        P = _abi.ptr
+        d = _abi.FrameDesc()
+        d.nv = m.num_verts
+        d.x4, d.v4, d.y4, d.y4_alt = P(m.x4), P(m.v4), P(m.y4), P(self._y_alt)
+        d.fixed, d.m_inv, d.faces = P(m.fixed), P(m.m_inv), P(m.face_indices)
+        d.adj_slice_off, d.adj_slice_width, d.adj_entries = P(m.adj_slice_off), P(m.adj_slice_width), P(m.adj_entries)
+        d.dt, d.k_s, d.k_b, d.damping = sc["dt"], sc["k_s"], sc["k_b"], sc["damping"]
+        d.gx, d.gy, d.gz = sc["g"]
+        d.padding, d.dhat, d.k_col = _f32(self.padding), _f32(self.dHat), _f32(self.compliance_collision)
+        d.n_substeps, d.n_iters = n_substeps, n_iters
+        d.collision = 1 if self.enable_collision_handling else 0
+        d.refit_every_substep = 1 if self.refit_every_substep else 0
+        d.status = P(self._status)
+        if self.enable_collision_handling:
+            cd = self._collision_buffers()
+            s = self.mesh_st
+            d.nvs = s.num_verts if s is not None else 0
+            if s is not None:
+                d.xs4, d.faces_st = P(s.x4), P(s.face_indices)
+                d.st = cd["st"].struct()
+                # the static mesh never moves: its tree is built once and reused
+                d.rebuild_static = 0 if cd["static_built"] else 1
+                cd["static_built"] = True
+            d.dyn = cd["dyn"].struct()
+            d.cap, d.rec_cap = self.cache_size, cd["rec_cap"]
+            d.cand, d.cand_n, d.pair_num = P(cd["cand"]), P(cd["cand_n"]), P(cd["pair_num"])
+        need = _abi.lib().slx_frame_ws_bytes(_abi.C.byref(d))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+        d.ws, d.ws_bytes = P(self._ws), self._ws.numel()
+        self._last_desc = d
+        return d
 
     # ----------------------------------------------------------- stage access
     def compute_y(self, dt):
@@ -99,7 +173,7 @@ class Solver:
                   _f32(dt), *g, _abi.stream_handle())
 
     def solve_spring_constraints(self, k_s, k_b):
-        """One spring pass on y (in place through the alternate buffer)."""
+        """One spring pass on y (through the alternate buffer)."""
         m = self.mesh_dy
         _abi.call("slx_spring_pass", m.num_verts, _abi.ptr(m.x4), _abi.ptr(m.fixed), _abi.ptr(m.y4),
                   _abi.ptr(self._y_alt), _abi.ptr(m.adj_slice_off), _abi.ptr(m.adj_slice_width),
@@ -110,3 +184,15 @@ class Solver:
         m = self.mesh_dy
         _abi.call("slx_update_vx", m.num_verts, _abi.ptr(m.x4), _abi.ptr(m.v4), _abi.ptr(m.y4),
                   _abi.ptr(m.fixed), _f32(self.damping), _f32(dt), _abi.stream_handle())
+
+    def candidates(self):
+        """Last broad-phase candidate lists: dict set -> (cand (cap, nq), counts)."""
+        cd = self._cd
+        m, s = self.mesh_dy, self.mesh_st
+        nv, nvs, cap = m.num_verts, (s.num_verts if s is not None else 0), self.cache_size
+        c, n = cd["cand"], cd["cand_n"]
+        out = {"vt_dy": (c[cap * nv:2 * cap * nv].view(cap, nv), n[nv:2 * nv])}
+        if nvs:
+            out["vt_st"] = (c[:cap * nv].view(cap, nv), n[:nv])
+            out["tv_st"] = (c[2 * cap * nv:2 * cap * nv + cap * nvs].view(cap, nvs), n[2 * nv:2 * nv + nvs])
+        return out

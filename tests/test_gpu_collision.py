@@ -1,0 +1,248 @@
+"""GPU parity of the broad and narrow phase against the oracle, bit for bit on
+the same float32 inputs: scan, radix sort, Morton codes, sorted order, Karras
+topology, cell bucketing, node boxes, candidate lists (content and order),
+contact projection, and whole frames with collision."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+NC = 32768
+
+
+def T(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def abi():
+    import importlib
+    return importlib.import_module("starlab-physics_amd._abi")
+
+
+@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100000, 1 << 21])
+def test_exclusive_scan(n, rng):
+    import torch
+    A = abi()
+    x = rng.integers(0, 9, n).astype(np.int32)
+    xi, out, tot = T(x), torch.empty(n, dtype=torch.int32, device="cuda"), torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = torch.empty(A.lib().slx_scan_ws_bytes(n), dtype=torch.uint8, device="cuda")
+    A.call("slx_exclusive_scan", n, None, A.ptr(xi), A.ptr(out), A.ptr(tot), A.ptr(ws), ws.numel(), A.stream_handle())
+    ex = np.concatenate([[0], np.cumsum(x)[:-1]])
+    np.testing.assert_array_equal(out.cpu().numpy(), ex)
+    assert tot.item() == x.sum()
+
+
+@pytest.mark.parametrize("n,bits,dev_n", [(1, 30, False), (1000, 30, False), (2048 * 3 + 5, 30, False),
+                                          (1 << 21, 30, False), (300000, 15, False), (50000, 30, True)])
+def test_radix_sort_equals_stable_argsort(n, bits, dev_n, rng):
+    import torch
+    A = abi()
+    keys = rng.integers(0, 1 << bits, n).astype(np.uint32)
+    keys[::5] = keys[1]
+    vals = np.arange(n, dtype=np.int32)
+    n_max = n + 777 if dev_n else n
+    kk = torch.zeros(n_max, dtype=torch.int32, device="cuda")
+    vv = torch.zeros(n_max, dtype=torch.int32, device="cuda")
+    kk[:n] = T(keys.view(np.int32))
+    vv[:n] = T(vals)
+    nd = T(np.array([n], np.int32)) if dev_n else None
+    ws = torch.empty(A.lib().slx_radix_sort_ws_bytes(n_max), dtype=torch.uint8, device="cuda")
+    A.call("slx_radix_sort_pairs", n_max, A.ptr(nd), A.ptr(kk), A.ptr(vv), bits, A.ptr(ws), ws.numel(),
+           A.stream_handle())
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(vv[:n].cpu().numpy(), order)
+    np.testing.assert_array_equal(kk[:n].cpu().numpy().view(np.uint32), keys[order])
+
+
+def test_cell_topology_bit_exact(slx, orc):
+    import importlib
+    bp = importlib.import_module("starlab-physics_amd.broadphase")
+    import torch
+    t = bp.cell_topology(torch.device("cuda"))
+    o = orc.cell_topology()
+    np.testing.assert_array_equal(t["codes"].cpu().numpy(), o.sorted_codes)
+    np.testing.assert_array_equal(t["sorted_ids"].cpu().numpy(), o.sorted_ids)
+    np.testing.assert_array_equal(t["child_a"].cpu().numpy(), o.child_a)
+    np.testing.assert_array_equal(t["child_b"].cpu().numpy(), o.child_b)
+    np.testing.assert_array_equal(t["parent"].cpu().numpy(), o.parent)
+    np.testing.assert_array_equal(t["range_l"].cpu().numpy(), o.range_l)
+    np.testing.assert_array_equal(t["range_r"].cpu().numpy(), o.range_r)
+
+
+def crumpled(slx, n, seed=0, jitter=5e-3):
+    g = slx.scene.grid_cloth(n, size=1.0, height=0.5, jitter=jitter, seed=seed)
+    x = g.x.astype(np.float64)
+    x[:, 1] += 0.1 * np.sin(7 * x[:, 0]) * np.cos(5 * x[:, 2])
+    return slx.Mesh(x=x.astype(np.float32), faces=g.faces)
+
+
+def tree_pair(slx, orc, mode, n=48, pad=0.01):
+    import importlib
+    bp = importlib.import_module("starlab-physics_amd.broadphase")
+    mesh = crumpled(slx, n)
+    gt = bp.DeviceTree(mesh, mode)
+    gt.build(pad)
+    Ot = orc.CellBVH if mode == "cell" else orc.TriBVH
+    ot = Ot(mesh.fid_np, mesh.positions(), pad)
+    return mesh, gt, ot
+
+
+@pytest.mark.parametrize("mode", ["cell", "tri"])
+def test_tree_build_bit_exact(slx, orc, mode):
+    mesh, gt, ot = tree_pair(slx, orc, mode)
+    nf = mesh.num_faces
+    np.testing.assert_array_equal(gt.fmin4[:nf, :3].cpu().numpy(), ot.fmin)
+    np.testing.assert_array_equal(gt.fmax4[:nf, :3].cpu().numpy(), ot.fmax)
+    np.testing.assert_array_equal(gt.box6[:3].cpu().numpy(), ot.lo)
+    np.testing.assert_array_equal(gt.box6[3:6].cpu().numpy(), ot.hi)
+    np.testing.assert_array_equal(gt.sorted_faces[:nf].cpu().numpy(), ot.sorted_faces[:nf])
+    nn = 2 * (NC if mode == "cell" else nf) - 1
+    np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin[:nn])
+    np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax[:nn])
+    if mode == "cell":
+        assert gt.status.item() == 0
+        np.testing.assert_array_equal(gt.face_cell[:nf].cpu().numpy(), ot.face_cell)
+        np.testing.assert_array_equal(gt.prefix.cpu().numpy(), ot.prefix)
+        np.testing.assert_array_equal(gt.num.cpu().numpy(), ot.num)
+        np.testing.assert_array_equal(gt.leaf_l.cpu().numpy(), ot.leaf_l)
+        np.testing.assert_array_equal(gt.leaf_r.cpu().numpy(), ot.leaf_r)
+    else:
+        np.testing.assert_array_equal(gt.codes[:nf].cpu().numpy(), np.sort(ot.codes))
+        np.testing.assert_array_equal(gt.child_a[:nf - 1].cpu().numpy(), ot.child_a)
+        np.testing.assert_array_equal(gt.child_b[:nf - 1].cpu().numpy(), ot.child_b)
+        np.testing.assert_array_equal(gt.parent[:nn].cpu().numpy(), ot.parent)
+
+
+def test_tri_refit_bit_exact(slx, orc):
+    mesh, gt, ot = tree_pair(slx, orc, "tri")
+    import torch
+    mesh.x4[:, :3] += torch.randn_like(mesh.x4[:, :3]) * 2e-3
+    gt.refit(0.01)
+    ot.refit(mesh.positions())
+    nn = 2 * mesh.num_faces - 1
+    np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin)
+    np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax)
+
+
+@pytest.mark.parametrize("mode,pad,cap", [("cell", 0.01, 40), ("tri", 0.01, 40), ("tri", 0.03, 8)])
+def test_traversal_bit_exact(slx, orc, rng, mode, pad, cap):
+    import torch
+    mesh, gt, ot = tree_pair(slx, orc, mode, pad=pad)
+    q = mesh.positions() + rng.normal(0, 0.01, (mesh.num_verts, 3)).astype(np.float32)
+    q4 = torch.zeros((len(q), 4), dtype=torch.float32, device="cuda")
+    q4[:, :3] = T(q)
+    gt.status.zero_()
+    cand, cnt = gt.query(q4, pad, cap)
+    qmin, qmax = q - np.float32(pad), q + np.float32(pad)
+    oc, on, fl = ot.query(qmin, qmax, cap)
+    cnt = cnt.cpu().numpy()
+    cand = cand.cpu().numpy()
+    np.testing.assert_array_equal(cnt, on)
+    for i in range(len(q)):
+        np.testing.assert_array_equal(cand[:cnt[i], i], oc[i, :on[i]])
+    assert (gt.status.item() != 0) == (fl > 0)
+
+
+def test_collision_pass_bit_exact(slx, orc, rng):
+    """Contact projection on the same y and the same candidate lists."""
+    import importlib
+    import torch
+    bp = importlib.import_module("starlab-physics_amd.broadphase")
+    A = abi()
+    cloth = crumpled(slx, 40, seed=2)
+    ico = slx.scene.icosphere(3, 0.25, center=(0.0, 0.45, 0.0))
+    obst = slx.Mesh(x=ico.x, faces=ico.faces, is_static=True)
+    pad, cap, dhat = 0.02, 40, 2.5e-4
+    y = cloth.positions() + rng.normal(0, 2e-3, (cloth.num_verts, 3)).astype(np.float32)
+    cloth.y4[:, :3] = T(y)
+    fixed = np.ones(cloth.num_verts, np.float32)
+    fixed[:3] = 0
+    cloth.fixed.copy_(T(fixed))
+    cloth._refresh_products()
+    td, ts = bp.DeviceTree(cloth, "tri"), bp.DeviceTree(obst, "tri")
+    td.build(pad)
+    ts.build(pad)
+    c0, n0 = ts.query(cloth.y4, pad, cap)
+    c1, n1 = td.query(cloth.y4, pad, cap)
+    c2, n2 = td.query(obst.x4, pad, cap)
+    nv, nvs = cloth.num_verts, obst.num_verts
+    rec_cap = 8 * (2 * nv + nvs)
+    ws = torch.empty(A.lib().slx_collision_ws_bytes(nv, nvs, rec_cap), dtype=torch.uint8, device="cuda")
+    pn = torch.zeros(2 * nv + nvs, dtype=torch.int32, device="cuda")
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    P = A.ptr
+    A.call("slx_collision_pass", nv, nvs, P(cloth.face_indices), P(obst.face_indices), P(cloth.y4), P(obst.x4),
+           P(cloth.fixed), P(cloth.m_inv), float(np.float32(dhat)), float(np.float32(1e8)), cap, P(c0), P(n0),
+           P(c1), P(n1), P(c2), P(n2), P(pn), rec_cap, P(st), P(ws), ws.numel(), A.stream_handle())
+    yo = y.copy()
+    to_host = lambda c, n: (np.ascontiguousarray(c.cpu().numpy().T), n.cpu().numpy())
+    out = orc.collision_pass(dict(faces=cloth.fid_np), obst.fid_np, yo, obst.positions(), fixed,
+                             cloth.m_inv_np, np.float32(dhat), np.float32(1e8), cap, to_host(c0, n0),
+                             to_host(c1, n1), to_host(c2, n2))
+    assert st.item() == 0
+    changed = np.any(yo != y, axis=1).sum()
+    assert changed > 20, "scene must produce contacts"
+    np.testing.assert_array_equal(cloth.y4[:, :3].cpu().numpy(), yo)
+    pn = pn.cpu().numpy()
+    np.testing.assert_array_equal(pn[:nv], out["pair_num"][0][:nv])
+    np.testing.assert_array_equal(pn[nv:2 * nv], out["pair_num"][1][:nv])
+    np.testing.assert_array_equal(pn[2 * nv:], out["pair_num"][2][:nvs])
+
+
+def collision_scene(slx, n=32, mode="cell", refit=False, layers=1, seed=0):
+    import torch
+    if layers == 1:
+        g = slx.scene.grid_cloth(n, size=1.0, height=0.35, jitter=1e-3, seed=seed)
+    else:
+        g = slx.scene.stacked_layers(n, layers, 0.02, size=1.0, height=0.35, jitter=1e-3, seed=seed)
+    cloth = slx.Mesh(x=g.x, faces=g.faces)
+    ico = slx.scene.icosphere(3, 0.3)
+    obst = slx.Mesh(x=ico.x, faces=ico.faces, is_static=True)
+    sim = slx.Solver(cloth, obst, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3,
+                     g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+    sim.enable_collision_handling = True
+    sim.bvh_mode = mode
+    sim.refit_every_substep = refit
+    sim.padding = 0.02
+    sel = torch.zeros(cloth.num_verts, device="cuda")
+    sel[0] = 1
+    sel[n - 1] = 1
+    sim.set_fixed_vertices(sel)
+    ref = orc_cloth(cloth, obst, sim)
+    return cloth, sim, ref
+
+
+def orc_cloth(cloth, obst, sim):
+    from oracle import oracle as orc
+    return orc.OracleCloth(cloth.host_dict(), dict(x=obst.positions(), faces=obst.fid_np), dhat=sim.dHat,
+                           stiffness_stretch=sim.stiffness_stretch, stiffness_bending=sim.stiffness_bending,
+                           g=sim.g, dt=sim.dt, padding=sim.padding, collision=True, bvh=sim.bvh_mode,
+                           cap=sim.cache_size, refit_every_substep=sim.refit_every_substep)
+
+
+@pytest.mark.parametrize("mode,refit,layers,iters", [("cell", False, 1, 1), ("tri", False, 1, 1),
+                                                      ("tri", True, 1, 2), ("tri", True, 2, 1)])
+def test_frames_with_collision_bit_exact(slx, mode, refit, layers, iters):
+    cloth, sim, ref = collision_scene(slx, n=24, mode=mode, refit=refit, layers=layers)
+    contacts = 0
+    for frame in range(12):
+        sim.forward(10, iters)
+        ref.forward(10, iters)
+        contacts += int((sim.candidates()["vt_st"][1] > 0).sum().item())
+    np.testing.assert_array_equal(cloth.positions(), ref.x)
+    assert sim.status() == 0 and ref.stats["overflow"] == 0
+    assert contacts > 0, "the cloth must reach the sphere"
+
+
+def test_frames_plain_through_frame_entry(slx):
+    """collision off: slx_frame runs the fused collision-free substeps."""
+    cloth, sim, ref = collision_scene(slx, n=20, mode="tri")
+    sim.enable_collision_handling = False
+    ref.collision = False
+    for _ in range(3):
+        sim.forward(20, 2)
+        ref.forward(20, 2)
+    np.testing.assert_array_equal(cloth.positions(), ref.x)
