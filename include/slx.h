@@ -187,6 +187,25 @@ int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* 
                        int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                        slx_stream_t stream);
 
+/* ---------------- graph-coloured Gauss-Seidel ---------------- */
+
+/* Greedy colouring of the stretch edges (HOST pointers; graph_coloring.py:131-149):
+ * edges in index order, each takes the lowest colour not used by an already-
+ * coloured edge sharing a vertex; order[] lists edges colour by colour, ascending
+ * id inside a colour (stable counting sort, graph_coloring.py:172-201);
+ * offsets[n_colors+1].  Returns n_colors, or a negative error. */
+int slx_greedy_edge_coloring(int nv, int ne, const int32_t* edges, int max_colors,
+                             int32_t* color, int32_t* order, int32_t* offsets);
+
+/* One Gauss-Seidel iteration in place on y4: the bending constraints (on x) as a
+ * Jacobi-averaged pass, then the stretch edges colour by colour (gs_edges: int4
+ * (v0, v1, l0 bits, 0) in colour order; color_off_host: HOST [n_colors+1]).
+ * The reference colours edges but has no sweep; this sweep is the package's. */
+int slx_gs_pass(int nv, const float* x4, const float* fixed, float* y4, const int32_t* slice_off,
+                const int32_t* slice_width, const int32_t* entries, const int32_t* gs_edges,
+                const int32_t* color_off_host, int n_colors, float k_s, float k_b,
+                slx_stream_t stream);
+
 /* ---------------- one frame (Solver.forward, XPBD.py:443-469) ---------------- */
 
 typedef struct slx_tree {
@@ -234,6 +253,10 @@ typedef struct slx_frame_desc {
   int32_t refit_every_substep;           /* triangle tree: refit from x at each substep */
   int32_t rebuild_static;                /* rebuild the static tree this frame */
   int32_t cap, rec_cap;
+  int32_t method;                        /* 0 = Jacobi, 1 = graph-coloured Gauss-Seidel */
+  const int32_t* gs_edges;               /* GS: int4 (v0, v1, l0 bits, 0) per stretch edge, colour order */
+  const int32_t* gs_color_off;           /* GS: HOST array [n_colors+1] of colour starts */
+  int32_t n_colors;
   slx_tree dyn, st;
   int32_t* cand;                         /* [cap*nv] vt_st, [cap*nv] vt_dy, [cap*nvs] tv_st */
   int32_t* cand_n;                       /* [2nv + nvs] */

@@ -31,6 +31,12 @@ extern "C" {
   void orc_spring_pass_##S(int nv, int ne, int nb, const int* edges, const R* l0,           \
                            const int* bend, const R* bl0, const R* x, R* y, const R* fixed, \
                            const R* m_inv, R k_s, R k_b, R* dx, R* nc);                      \
+  /* Gauss-Seidel iteration: bending Jacobi pass, then stretch colours in order           \
+     (colouring of graph_coloring.py:131-201; the sweep is this package's definition) */   \
+  void orc_gs_pass_##S(int nv, int nb, const int* bend, const R* bl0, const R* x, R* y,     \
+                       const R* fixed, const R* m_inv, R k_b, const int* gs_order,          \
+                       const int* edges, const R* l0, const int* color_off, int n_colors,   \
+                       R k_s, R* dx, R* nc);                                                \
   /* XPBD.py:340-343 compute_velocity then XPBD.py:332-337 update_x */                       \
   void orc_update_vx_##S(int nv, R* x, R* v, const R* y, const R* fixed, R damping, R dt);  \
   /* distance.py:4-47 d_type_PT */                                                           \

@@ -73,6 +73,20 @@ def spring_pass(mesh, x, y, fixed, m_inv, k_s, k_b):
     return y
 
 
+def gs_pass(mesh, x, y, fixed, m_inv, k_s, k_b, order, offsets):
+    ct, s = _real(x.dtype)
+    dt = x.dtype
+    dx = np.zeros_like(x)
+    nc = np.zeros(len(fixed), dt)
+    l0 = mesh["l0"].astype(dt)
+    bl0 = mesh["bl0"].astype(dt)
+    getattr(lib(), "orc_gs_pass_" + s)(
+        len(fixed), len(bl0), P(mesh["bend"]), P(bl0), P(x), P(y), P(fixed), P(m_inv), ct(k_b),
+        P(np.ascontiguousarray(order, np.int32)), P(mesh["edges"]), P(l0),
+        P(np.ascontiguousarray(offsets, np.int32)), len(offsets) - 1, ct(k_s), P(dx), P(nc))
+    return y
+
+
 def update_vx(x, v, y, fixed, damping, dt):
     ct, s = _real(x.dtype)
     getattr(lib(), "orc_update_vx_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
@@ -290,7 +304,8 @@ class OracleCloth:
 
     def __init__(self, mesh_dy, mesh_st=None, *, dhat, stiffness_stretch, stiffness_bending,
                  g, dt, dtype=np.float32, damping=0.001, padding=0.05, collision=False,
-                 bvh="cell", cap=40, refit_every_substep=False):
+                 bvh="cell", cap=40, refit_every_substep=False, method="jacobi"):
+        self.method = method
         self.dt_ = np.dtype(dtype)
         self.m = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh_dy.items()}
         self.m["edges"] = np.ascontiguousarray(self.m["edges"], np.int32)
@@ -352,8 +367,14 @@ class OracleCloth:
             if self.collision and self.bvh == "tri" and self.refit_every_substep:
                 self.tree_dy.refit(self.x.astype(np.float32))
             self.y = compute_y(self.x, self.v, self.fixed, dts, g)
+            if self.method == "gs" and not hasattr(self, "_colors"):
+                _, order, off = greedy_coloring(len(self.fixed), self.m["edges"])
+                self._colors = (order, off)
             for _ in range(n_iters):
-                spring_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b)
+                if self.method == "gs":
+                    gs_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b, *self._colors)
+                else:
+                    spring_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b)
                 if self.collision:
                     cands = self._broadphase()
                     xs = None if self.st is None else np.asarray(self.st["x"], self.dt_)

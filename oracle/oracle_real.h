@@ -78,6 +78,46 @@ void FN(spring_pass)(int nv, int ne, int nb, const int* edges, const R* l0, cons
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
 }
 
+/* Graph-coloured Gauss-Seidel iteration (the reference colours stretch edges,
+   graph_coloring.py:131-201, but has no sweep): bending constraints on x as one
+   Jacobi-averaged pass, then for each colour, each edge of the colour in order,
+   the stretch projection applied to both endpoints at once with nc = 1. */
+void FN(gs_pass)(int nv, int nb, const int* bend, const R* bl0, const R* x, R* y, const R* fixed,
+                 const R* m_inv, R k_b, const int* gs_order, const int* edges, const R* l0,
+                 const int* color_off, int n_colors, R k_s, R* dx, R* nc) {
+  for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  for (int i = 0; i < nv; ++i) nc[i] = 0;
+  for (int i = 0; i < nb; ++i)
+    CAT(spring_one_, S)(bend[2 * i], bend[2 * i + 1], bl0[i], x, k_b, fixed, m_inv, dx, nc);
+  CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+  for (int c = 0; c < n_colors; ++c) {
+    for (int k = color_off[c]; k < color_off[c + 1]; ++k) {
+      int e = gs_order[k], v0 = edges[2 * e], v1 = edges[2 * e + 1];
+      R d1[6] = {0, 0, 0, 0, 0, 0}, n1[2] = {0, 0};
+      /* reuse the scatter form on a two-vertex scratch: dx/nc of just this edge */
+      R* p = y;
+      R d[3];
+      SUB(p + 3 * v0, p + 3 * v1, d);
+      R len = SQRT(DOT(d, d));
+      R C = len - l0[e];
+      R inv = (R)1 / len;
+      R n[3] = {inv * d[0], inv * d[1], inv * d[2]};
+      R fw0 = fixed[v0] * m_inv[v0], fw1 = fixed[v1] * m_inv[v1];
+      R schur = (fw0 + fw1) * DOT(n, n);
+      R ld = (k_s * C) / ((k_s * schur) + (R)1);
+      for (int q = 0; q < 3; ++q) {
+        d1[q] = d1[q] - (fw0 * ld) * n[q];
+        d1[3 + q] = d1[3 + q] + (fw1 * ld) * n[q];
+      }
+      n1[0] = n1[1] = (R)1;
+      for (int q = 0; q < 3; ++q) {
+        y[3 * v0 + q] = y[3 * v0 + q] + fixed[v0] * (d1[q] / n1[0]);
+        y[3 * v1 + q] = y[3 * v1 + q] + fixed[v1] * (d1[3 + q] / n1[1]);
+      }
+    }
+  }
+}
+
 /* XPBD.py:340-343 then 332-337:
    v = (1 - damping) * f * (y - x) / dt;  x += dt * v. */
 void FN(update_vx)(int nv, R* x, R* v, const R* y, const R* fixed, R damping, R dt) {

@@ -52,6 +52,8 @@ SIGNATURES = {
                            PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
     "slx_frame_ws_bytes": [PTR],
     "slx_frame": [PTR, STREAM],
+    "slx_greedy_edge_coloring": [I32, I32, PTR, I32, PTR, PTR, PTR],
+    "slx_gs_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, F32, F32, STREAM],
 }
 _RET = {"slx_last_error": C.c_char_p, "slx_scan_ws_bytes": C.c_size_t,
         "slx_radix_sort_ws_bytes": C.c_size_t, "slx_mesh_aabb_ws_bytes": C.c_size_t,
@@ -76,7 +78,8 @@ class FrameDesc(C.Structure):
                 ("dt", F32), ("gx", F32), ("gy", F32), ("gz", F32), ("k_s", F32), ("k_b", F32),
                 ("damping", F32), ("padding", F32), ("dhat", F32), ("k_col", F32),
                 ("n_substeps", I32), ("n_iters", I32), ("collision", I32), ("refit_every_substep", I32),
-                ("rebuild_static", I32), ("cap", I32), ("rec_cap", I32), ("dyn", Tree), ("st", Tree),
+                ("rebuild_static", I32), ("cap", I32), ("rec_cap", I32), ("method", I32), ("gs_edges", PTR),
+                ("gs_color_off", PTR), ("n_colors", I32), ("dyn", Tree), ("st", Tree),
                 ("cand", PTR), ("cand_n", PTR), ("pair_num", PTR), ("status", PTR), ("ws", PTR),
                 ("ws_bytes", C.c_size_t)]
 
@@ -147,4 +150,14 @@ def timer_names():
 
 
 def greedy_coloring(nv: int, edges: np.ndarray, max_colors: int = 64):
-    raise NotImplementedError
+    """Host greedy edge colouring in libslx (slx_greedy_edge_coloring)."""
+    e = np.ascontiguousarray(edges, np.int32).reshape(-1, 2)
+    ne = len(e)
+    color = np.zeros(max(ne, 1), np.int32)
+    order = np.zeros(max(ne, 1), np.int32)
+    offsets = np.zeros(max_colors + 1, np.int32)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    nc = lib().slx_greedy_edge_coloring(nv, ne, P(e), max_colors, P(color), P(order), P(offsets))
+    if nc < 0:
+        raise SlxError(f"slx_greedy_edge_coloring failed: {last_error()}")
+    return color[:ne], order[:ne], offsets[:nc + 1].copy()

@@ -77,7 +77,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                   d->adj_slice_width && d->adj_entries,
               "slx_frame: null state pointer");
   hipStream_t st = (hipStream_t)stream;
-  if (!d->collision) {
+  SLX_REQUIRE(d->method == 0 || d->method == 1, "slx_frame: method must be 0 (Jacobi) or 1 (Gauss-Seidel)");
+  SLX_REQUIRE(d->method == 0 || d->n_colors == 0 || (d->gs_edges && d->gs_color_off),
+              "slx_frame: Gauss-Seidel colouring missing");
+  if (!d->collision && d->method == 0) {
     // x4_alt / v4_alt: the caller's y4_alt doubles as one ping-pong buffer is not
     // possible (four buffers are needed); use the workspace for the alternates.
     size_t need = 2 * sizeof(float4) * (size_t)d->nv;
@@ -88,16 +91,19 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                               d->adj_slice_width, d->adj_entries, d->n_substeps, d->n_iters, d->dt, d->gx, d->gy,
                               d->gz, d->k_s, d->k_b, d->damping, stream);
   }
-  SLX_REQUIRE(d->faces && d->cand && d->cand_n && d->ws, "slx_frame: null collision pointer");
-  SLX_REQUIRE(d->dyn.mode == 0 || d->dyn.mode == 1, "slx_frame: bad tree mode");
-  SLX_REQUIRE(d->nvs == 0 || (d->xs4 && d->faces_st), "slx_frame: static mesh missing");
+  const bool coll = d->collision != 0;
+  SLX_REQUIRE(!coll || (d->faces && d->cand && d->cand_n && d->ws), "slx_frame: null collision pointer");
+  SLX_REQUIRE(!coll || d->dyn.mode == 0 || d->dyn.mode == 1, "slx_frame: bad tree mode");
+  SLX_REQUIRE(!coll || d->nvs == 0 || (d->xs4 && d->faces_st), "slx_frame: static mesh missing");
   SLX_REQUIRE(d->ws_bytes >= slx_frame_ws_bytes(d), "slx_frame: workspace too small");
-  const int nv = d->nv, nvs = d->nvs, cap = d->cap;
+  const int nv = d->nv, nvs = coll ? d->nvs : 0, cap = d->cap;
   const float pad = d->padding;
   int rc;
   // per-frame rebuild from x (XPBD.py:448-458)
-  rc = build_tree(d->dyn, nv, d->x4, d->faces, pad, d->status, d->ws, d->ws_bytes, stream);
-  if (rc) return rc;
+  if (coll) {
+    rc = build_tree(d->dyn, nv, d->x4, d->faces, pad, d->status, d->ws, d->ws_bytes, stream);
+    if (rc) return rc;
+  }
   if (nvs > 0 && d->rebuild_static) {
     rc = build_tree(d->st, nvs, d->xs4, d->faces_st, pad, d->status, d->ws, d->ws_bytes, stream);
     if (rc) return rc;
@@ -109,7 +115,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   float4* yalt = (float4*)d->y4_alt;
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
   for (int s = 0; s < d->n_substeps; ++s) {
-    if (s > 0 && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0) {
+    if (coll && s > 0 && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0) {
       rc = slx_face_aabbs(d->dyn.nf, d->faces, d->x4, pad, d->dyn.fmin4, d->dyn.fmax4, stream);
       if (rc) return rc;
       rc = slx_tri_refit(d->dyn.nf, d->dyn.fmin4, d->dyn.fmax4, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
@@ -117,13 +123,24 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (rc) return rc;
       tv_fresh = false;
     }
-    for (int it = 0; it < d->n_iters; ++it) {
-      // spring pass (compute_y fused into the first)
-      rc = spring_launch(it == 0, false, nv, (const float4*)d->x4, (const float4*)d->v4, d->fixed, ycur, yalt,
-                         nullptr, nullptr, d->adj_slice_off, d->adj_slice_width, d->adj_entries, d->k_s, d->k_b, d->dt,
-                         d->gx, d->gy, d->gz, d->damping, st);
+    if (d->method == 1) {
+      rc = slx_compute_y(nv, d->x4, d->v4, (float*)ycur, d->dt, d->gx, d->gy, d->gz, stream);
       if (rc) return rc;
-      float4* t = ycur; ycur = yalt; yalt = t;
+    }
+    for (int it = 0; it < d->n_iters; ++it) {
+      if (d->method == 0) {
+        // Jacobi spring pass (compute_y fused into the first)
+        rc = spring_launch(it == 0, false, nv, (const float4*)d->x4, (const float4*)d->v4, d->fixed, ycur, yalt,
+                           nullptr, nullptr, d->adj_slice_off, d->adj_slice_width, d->adj_entries, d->k_s, d->k_b,
+                           d->dt, d->gx, d->gy, d->gz, d->damping, st);
+        if (rc) return rc;
+        float4* t = ycur; ycur = yalt; yalt = t;
+      } else {
+        rc = gs_pass(nv, (const float4*)d->x4, d->fixed, ycur, d->adj_slice_off, d->adj_slice_width, d->adj_entries,
+                     (const int4*)d->gs_edges, d->gs_color_off, d->n_colors, d->k_s, d->k_b, st);
+        if (rc) return rc;
+      }
+      if (!coll) continue;
       // broad phase (XPBD.py:186-215)
       if (nvs > 0) {
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, pad, cap, c0, n0, d->status, stream);

@@ -325,7 +325,8 @@ __global__ void __launch_bounds__(kTravBS)
 k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32_t* __restrict__ num,
            const int32_t* __restrict__ sorted, const float4* __restrict__ nlo, const float4* __restrict__ nhi,
            const float4* __restrict__ fmin, const float4* __restrict__ fmax, const float* __restrict__ box,
-           int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status) {
+           int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status,
+           const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst) {
   __shared__ int32_t s_stack[kStack * kTravBS];
   const int q = blockIdx.x * kTravBS + threadIdx.x;
   if (q >= nq) return;
@@ -346,7 +347,7 @@ k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32
     return;
   }
   int sp = 0;
-  stk[0] = 0;
+  stk[0] = roots ? roots[q_inst[q]] : 0;   // batched trees: the query's instance subtree
   sp = 1;
   while (sp > 0) {
     const int idx = stk[(--sp) * kTravBS];
@@ -519,7 +520,9 @@ namespace slx {
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
-                    int32_t* cand_n, int32_t* status, hipStream_t st) {
+                    int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
+                    const int32_t* q_inst) {
+  SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
   if (nq == 0) return SLX_OK;
@@ -529,11 +532,11 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   if (mode == 0)
     SLX_LAUNCH(name, k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
-               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
+               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst);
   else
     SLX_LAUNCH(name, k_traverse<false>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
-               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status);
+               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst);
   return check_launch("slx_traverse");
 }
 }  // namespace slx
@@ -543,5 +546,6 @@ extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, in
                             const float* fmax4, const float* box6, int cap, int32_t* cand, int32_t* cand_n,
                             int32_t* status, slx_stream_t stream) {
   return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
-                         sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream));
+                         sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream), nullptr,
+                         nullptr);
 }

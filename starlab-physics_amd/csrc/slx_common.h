@@ -59,11 +59,17 @@ int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4*
                   const int32_t* slice_width, const int32_t* entries, float k_s, float k_b, float dt, float gx,
                   float gy, float gz, float damping, hipStream_t st);
 
+// one Gauss-Seidel iteration in place on y4 (gs.hip); color_off_host is a host array
+int gs_pass(int nv, const float4* x4, const float* fixed, float4* y4, const int32_t* slice_off,
+            const int32_t* slice_width, const int32_t* entries, const int4* gs_edges, const int32_t* color_off_host,
+            int n_colors, float k_s, float k_b, hipStream_t st);
+
 // stack traversal launcher (lbvh.hip); name is the kernel timer label
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
-                    int32_t* cand_n, int32_t* status, hipStream_t st);
+                    int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
+                    const int32_t* q_inst = nullptr);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);

@@ -58,6 +58,9 @@ class Solver:
         self.cache_size = 40
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_query = 8
+        self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
+        self.max_colors = 64
+        self._gs = None
         dev = mesh_dy.x4.device
         self.device = dev
         nv = mesh_dy.num_verts
@@ -145,6 +148,13 @@ class Solver:
         d.collision = 1 if self.enable_collision_handling else 0
         d.refit_every_substep = 1 if self.refit_every_substep else 0
         d.status = P(self._status)
+        if self.method not in ("jacobi", "gs"):
+            raise ValueError("method must be 'jacobi' or 'gs'")
+        if self.method == "gs":
+            gs = self.coloring()
+            d.method, d.n_colors = 1, gs["n_colors"]
+            d.gs_edges = P(gs["edges"])
+            d.gs_color_off = gs["offsets"].ctypes.data_as(_abi.C.c_void_p)
         if self.enable_collision_handling:
             cd = self._collision_buffers()
             s = self.mesh_st
@@ -164,6 +174,21 @@ class Solver:
         d.ws, d.ws_bytes = P(self._ws), self._ws.numel()
         self._last_desc = d
         return d
+
+    def coloring(self):
+        """Greedy stretch-edge colouring (graph_coloring.py:131-201), uploaded as
+        int4 (v0, v1, l0 bits, 0) per edge in colour order."""
+        import torch
+        if self._gs is None:
+            m = self.mesh_dy
+            color, order, offsets = _abi.greedy_coloring(m.num_verts, m.eid_np, self.max_colors)
+            e = m.eid_np[order]
+            packed = np.zeros((len(order), 4), np.int32)
+            packed[:, 0], packed[:, 1] = e[:, 0], e[:, 1]
+            packed[:, 2] = m.l0_np[order].view(np.int32)
+            self._gs = dict(color=color, order=order, offsets=np.ascontiguousarray(offsets, np.int32),
+                            n_colors=len(offsets) - 1, edges=torch.from_numpy(packed).to(self.device))
+        return self._gs
 
     # ----------------------------------------------------------- stage access
     def compute_y(self, dt):
