@@ -91,6 +91,11 @@ int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, float* v4_al
                        int n_iters, float dt, float gx, float gy, float gz, float k_s, float k_b,
                        float damping, slx_stream_t stream);
 
+/* Launch shape of every spring-pass kernel (no reference counterpart: set by the
+ * autotuner): threads per block {64, 128, 256} x vertices per thread {1, 2, 4}.
+ * Process-wide; results do not depend on it. */
+int slx_set_spring_launch(int block, int vpt);
+
 /* (f, m_inv) -> x4.w = y4.w = f*m_inv, v4.w = f (set_fixed_vertices,
  * XPBD.py:389-395, with the per-vertex products the kernels read). */
 int slx_set_fixed(int nv, const float* fixed, const float* m_inv, float* x4, float* v4,
@@ -170,6 +175,25 @@ int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves,
                  const float* nhi4, const float* fmin4, const float* fmax4, const float* box6,
                  int cap, int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
 
+/* Batched instances (no reference counterpart beyond the mesh concatenation of
+ * framework/meshio/concat.py:11-58): instances packed CSR-style (vert_base,
+ * face_base [n_inst+1], face_inst [nf]); per-instance boxes box6[6*n_inst];
+ * Morton codes in each instance's box; stable sort by code then by instance
+ * (one segmented sort); one Karras tree over the composite keys
+ * (instance, code, index) in which roots[i] is the subtree holding exactly
+ * instance i's faces; refit.  Includes the face boxes (x4, padding). */
+size_t slx_seg_build_ws_bytes(int nf, int n_inst);
+int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float* x4, float padding,
+                  const int32_t* vert_base, const int32_t* face_base, const int32_t* face_inst,
+                  float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
+                  int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
+                  int32_t* range_l, int32_t* range_r, int32_t* roots, float* nlo4, float* nhi4,
+                  void* ws, size_t ws_bytes, slx_stream_t stream);
+/* Traversal of a batched tree: query q starts at roots[q_inst[q]]. */
+int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots,
+                     float padding, int nf, const float* nlo4, const float* nhi4, int cap,
+                     int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
+
 /* ---------------- narrow phase (collision_constraints_x.py, distance.py) ---------------- */
 
 /* solve_collision_constraints_x + update_dx, XPBD.py:236-264, 383-387, with
@@ -226,6 +250,16 @@ typedef struct slx_tree {
   int32_t* leaf_r;
   float* nlo4;               /* float4 [nodes] */
   float* nhi4;
+  /* batched instances (triangle mode with n_inst > 1, see slx_seg_build) */
+  int32_t n_inst;            /* 0 or 1: a single mesh */
+  const int32_t* vert_base;  /* [n_inst+1] */
+  const int32_t* face_base;  /* [n_inst+1] */
+  const int32_t* face_inst;  /* [nf] */
+  const int32_t* vert_inst;  /* [nv] instance of each vertex of this tree's mesh (its queries) */
+  int32_t* sorted_inst;      /* [nf] */
+  int32_t* range_l;          /* [nf-1] */
+  int32_t* range_r;
+  int32_t* roots;            /* [n_inst] */
 } slx_tree;
 
 typedef struct slx_frame_desc {

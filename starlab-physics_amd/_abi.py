@@ -31,6 +31,7 @@ SIGNATURES = {
     "slx_substeps_plain": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, I32,
                            F32, F32, F32, F32, F32, F32, F32, STREAM],
     "slx_set_fixed": [I32, PTR, PTR, PTR, PTR, PTR, STREAM],
+    "slx_set_spring_launch": [I32, I32],
     "slx_scan_ws_bytes": [I32],
     "slx_exclusive_scan": [I32, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
     "slx_radix_sort_ws_bytes": [I32],
@@ -52,6 +53,10 @@ SIGNATURES = {
                            PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
     "slx_frame_ws_bytes": [PTR],
     "slx_frame": [PTR, STREAM],
+    "slx_seg_build_ws_bytes": [I32, I32],
+    "slx_seg_build": [I32, I32, I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
+                      PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_traverse_seg": [I32, PTR, PTR, PTR, F32, I32, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_greedy_edge_coloring": [I32, I32, PTR, I32, PTR, PTR, PTR],
     "slx_gs_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, F32, F32, STREAM],
 }
@@ -59,7 +64,7 @@ _RET = {"slx_last_error": C.c_char_p, "slx_scan_ws_bytes": C.c_size_t,
         "slx_radix_sort_ws_bytes": C.c_size_t, "slx_mesh_aabb_ws_bytes": C.c_size_t,
         "slx_cell_topology_ws_bytes": C.c_size_t, "slx_cell_build_ws_bytes": C.c_size_t,
         "slx_tri_build_ws_bytes": C.c_size_t, "slx_collision_ws_bytes": C.c_size_t,
-        "slx_frame_ws_bytes": C.c_size_t}
+        "slx_frame_ws_bytes": C.c_size_t, "slx_seg_build_ws_bytes": C.c_size_t}
 
 
 class Tree(C.Structure):
@@ -67,7 +72,9 @@ class Tree(C.Structure):
     _fields_ = [("mode", I32), ("nf", I32), ("fmin4", PTR), ("fmax4", PTR), ("box6", PTR),
                 ("codes", PTR), ("sorted_faces", PTR), ("child_a", PTR), ("child_b", PTR),
                 ("parent", PTR), ("num", PTR), ("face_cell", PTR), ("prefix", PTR), ("leaf_l", PTR),
-                ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR)]
+                ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR), ("n_inst", I32), ("vert_base", PTR),
+                ("face_base", PTR), ("face_inst", PTR), ("vert_inst", PTR), ("sorted_inst", PTR),
+                ("range_l", PTR), ("range_r", PTR), ("roots", PTR)]
 
 
 class FrameDesc(C.Structure):

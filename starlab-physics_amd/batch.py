@@ -1,0 +1,126 @@
+"""Batched cloth instances in one solver, one captured HIP graph per frame.
+
+Many independent cloths of uneven sizes, each with its own obstacle, are packed
+CSR-style into one dynamic and one static mesh (Mesh.from_instances; the
+reference's only multi-mesh mechanism is the concatenation of
+framework/meshio/concat.py:11-58).  Constraints never cross instances, so the
+packed mesh runs through the ordinary frame (slx_frame); the broad phase keeps
+one subtree per instance of a single LBVH built with one segmented sort
+(slx_seg_build), so no instance sees another's triangles.
+
+A frame is captured once into a HIP graph (torch.cuda.CUDAGraph) and replayed,
+which removes every per-kernel launch from the host loop.  `autotune` picks the
+spring kernels' launch shape (threads per block x vertices per thread) by timing
+replays of the captured frame.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _abi
+from . import scene as _scene
+from .mesh import Mesh
+from .solver import Solver
+
+
+def instance_sides(n_inst=256, lo=16, hi=256, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi + 1, n_inst)
+
+
+def make_instances(n_inst=256, side_lo=16, side_hi=256, seed=0, size=1.0, sphere_subdiv=3,
+                   r_lo=0.2, r_hi=0.5):
+    """Cloth instance i: side_i x side_i vertex grid (side ~ U{side_lo..side_hi}),
+    1 m square at height r_i + 0.05 above its own sphere of radius r_i ~ U(r_lo, r_hi)
+    at the origin.  Instances share the space but never interact."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(side_lo, side_hi + 1, n_inst)
+    radii = rng.uniform(r_lo, r_hi, n_inst)
+    base = _scene.icosphere(sphere_subdiv, 1.0)
+    cloths, spheres = [], []
+    for i in range(n_inst):
+        r = float(radii[i])
+        cloths.append(_scene.grid_cloth(int(sides[i]), size=size, height=r + 0.05, jitter=1e-4, seed=seed + i))
+        spheres.append(_scene.TriMesh((base.x.astype(np.float64) * r).astype(np.float32), base.faces))
+    return cloths, spheres, sides, radii
+
+
+class BatchedScene:
+    def __init__(self, cloths, spheres, *, dt=1.0 / 60.0, stiffness_stretch=1e6, stiffness_bending=1e3,
+                 thickness=5e-3, padding=None, g=(0.0, -9.8, 0.0), device="cuda"):
+        import torch
+        self.cloth = Mesh.from_instances(cloths, device=device)
+        self.obst = Mesh.from_instances(spheres, is_static=True, device=device)
+        self.n_inst = len(cloths)
+        sim = Solver(self.cloth, self.obst, dHat=thickness * thickness, stiffness_stretch=stiffness_stretch,
+                     stiffness_bending=stiffness_bending, g=g, dt=dt)
+        sim.enable_collision_handling = True
+        sim.bvh_mode = "tri"
+        sim.refit_every_substep = True
+        sim.padding = 0.5 * thickness if padding is None else padding
+        # pin two corners of every instance
+        sel = torch.zeros(self.cloth.num_verts, device=device)
+        vb = self.cloth.inst_np["vert_base"]
+        for i, c in enumerate(cloths):
+            n = int(round(np.sqrt(len(c.x))))
+            sel[int(vb[i])] = 1
+            sel[int(vb[i]) + n - 1] = 1
+        sim.set_fixed_vertices(sel)
+        self.sim = sim
+        self.graph = None
+        self.shape = (256, 1)
+
+    # ------------------------------------------------------------------ frames
+    def forward(self, n_substeps, n_iters):
+        self.sim.forward(n_substeps, n_iters)
+
+    def capture(self, n_substeps, n_iters):
+        """Capture one frame into a HIP graph (buffers are allocated and the static
+        tree built by an eager warm-up frame first)."""
+        import torch
+        self.sim.forward(n_substeps, n_iters)          # warm-up: allocations + static tree
+        torch.cuda.synchronize()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.sim.forward(n_substeps, n_iters)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = g
+        self.graph_shape = (n_substeps, n_iters)
+        return g
+
+    def replay(self):
+        self.graph.replay()
+
+    # --------------------------------------------------------------- autotune
+    def autotune(self, n_substeps, n_iters, blocks=(64, 128, 256), vpts=(1, 2, 4), reps=3):
+        """Time `reps` graph replays of one frame for every launch shape of the
+        spring kernels; keep the fastest.  Each shape is captured afresh (the
+        launch shape is baked into the graph)."""
+        import torch
+        results = {}
+        state = [t.clone() for t in (self.cloth.x4, self.cloth.v4)]
+        for bs in blocks:
+            for vpt in vpts:
+                _abi.call("slx_set_spring_launch", bs, vpt)
+                self.cloth.x4.copy_(state[0])
+                self.cloth.v4.copy_(state[1])
+                self.capture(n_substeps, n_iters)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    self.replay()
+                torch.cuda.synchronize()
+                results[(bs, vpt)] = (time.perf_counter() - t0) / reps
+        best = min(results, key=results.get)
+        _abi.call("slx_set_spring_launch", *best)
+        self.cloth.x4.copy_(state[0])
+        self.cloth.v4.copy_(state[1])
+        self.capture(n_substeps, n_iters)
+        self.shape = best
+        return best, results

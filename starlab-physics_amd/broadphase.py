@@ -72,16 +72,38 @@ class DeviceTree:
             ws = _abi.lib().slx_tri_build_ws_bytes(nf)
         self.nlo4 = torch.zeros((nn, 4), **f32)
         self.nhi4 = torch.zeros((nn, 4), **f32)
+        # batched instances: one subtree per instance (slx_seg_build)
+        self.inst = getattr(mesh, "inst", None)
+        self.n_inst = 0
+        if self.inst is not None and self.inst["n"] > 1:
+            if mode != "tri":
+                raise ValueError("batched instances need the triangle LBVH (mode='tri')")
+            ni = self.inst["n"]
+            self.n_inst = ni
+            self.box6 = torch.zeros(6 * ni + 2, **f32)
+            self.sorted_inst = torch.zeros(max(nf, 1), **i32)
+            self.range_l = torch.zeros(max(nf - 1, 1), **i32)
+            self.range_r = torch.zeros(max(nf - 1, 1), **i32)
+            self.roots = torch.zeros(ni, **i32)
+            ws = max(ws, _abi.lib().slx_seg_build_ws_bytes(nf, ni))
         ws = max(ws, _abi.lib().slx_mesh_aabb_ws_bytes())
         self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
         self.status = torch.zeros(1, **i32)
 
     def struct(self) -> _abi.Tree:
         P = _abi.ptr
-        return _abi.Tree(0 if self.mode == "cell" else 1, self.nf, P(self.fmin4), P(self.fmax4), P(self.box6),
-                         P(self.codes), P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent),
-                         P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
-                         P(self.nlo4), P(self.nhi4))
+        t = _abi.Tree(0 if self.mode == "cell" else 1, self.nf, P(self.fmin4), P(self.fmax4), P(self.box6),
+                      P(self.codes), P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent),
+                      P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
+                      P(self.nlo4), P(self.nhi4))
+        if self.n_inst > 1:
+            I = self.inst
+            t.n_inst = self.n_inst
+            t.vert_base, t.face_base = P(I["vert_base"]), P(I["face_base"])
+            t.face_inst, t.vert_inst = P(I["face_inst"]), P(I["vert_inst"])
+            t.sorted_inst, t.range_l, t.range_r, t.roots = (P(self.sorted_inst), P(self.range_l), P(self.range_r),
+                                                            P(self.roots))
+        return t
 
     # --- stages (the frame driver calls the same entry points itself) ---
     def face_aabbs(self, padding, x4=None):
@@ -97,9 +119,18 @@ class DeviceTree:
     def build(self, padding, x4=None):
         """computeAABB_faces + computeAABB + LBVH build from x (XPBD.py:450-452)."""
         P = _abi.ptr
+        s = _abi.stream_handle()
+        if self.n_inst > 1:
+            x4 = self.mesh.x4 if x4 is None else x4
+            I = self.inst
+            _abi.call("slx_seg_build", self.mesh.num_verts, self.nf, self.n_inst, P(self.faces), P(x4),
+                      float(padding), P(I["vert_base"]), P(I["face_base"]), P(I["face_inst"]), P(self.fmin4),
+                      P(self.fmax4), P(self.box6), P(self.codes), P(self.sorted_faces), P(self.sorted_inst),
+                      P(self.child_a), P(self.child_b), P(self.parent), P(self.range_l), P(self.range_r),
+                      P(self.roots), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
+            return
         self.face_aabbs(padding, x4)
         self.mesh_aabb(padding, x4)
-        s = _abi.stream_handle()
         if self.mode == "cell":
             _abi.call("slx_cell_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.child_a),
                       P(self.child_b), P(self.parent), P(self.face_cell), P(self.prefix), P(self.sorted_faces),
@@ -119,7 +150,7 @@ class DeviceTree:
                   P(self.child_b), P(self.parent), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(),
                   _abi.stream_handle())
 
-    def query(self, q4, padding, cap, cand=None, cand_n=None, status=None):
+    def query(self, q4, padding, cap, cand=None, cand_n=None, status=None, q_inst=None):
         """Candidate faces of boxes q4.xyz -+ padding, column-major (cap, nq)."""
         nq = q4.shape[0]
         dev = q4.device
@@ -129,6 +160,12 @@ class DeviceTree:
             cand_n = torch.zeros(max(nq, 1), dtype=torch.int32, device=dev)
         st = self.status if status is None else status
         P = _abi.ptr
+        if self.n_inst > 1:
+            if q_inst is None:
+                raise ValueError("batched tree queries need q_inst (instance of each query)")
+            _abi.call("slx_traverse_seg", nq, P(q4), P(q_inst), P(self.roots), float(padding), self.nf,
+                      P(self.nlo4), P(self.nhi4), cap, P(cand), P(cand_n), P(st), _abi.stream_handle())
+            return cand, cand_n
         _abi.call("slx_traverse", 0 if self.mode == "cell" else 1, nq, P(q4), float(padding),
                   self.n_leaves if self.nf > 0 else 0, P(self.num), P(self.sorted_faces), P(self.nlo4),
                   P(self.nhi4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n), P(st),

@@ -26,14 +26,21 @@ namespace {
 
 using namespace slx;
 
+bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
+
 size_t tree_ws(const slx_tree& t) {
   if (t.nf <= 0) return 0;
+  if (batched(t)) return slx_seg_build_ws_bytes(t.nf, t.n_inst);
   return t.mode == 0 ? slx_cell_build_ws_bytes(t.nf) : slx_tri_build_ws_bytes(t.nf);
 }
 
 int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces, float pad, int32_t* status,
                void* ws, size_t ws_bytes, slx_stream_t s) {
   if (t.nf <= 0) return SLX_OK;
+  if (batched(t))
+    return slx_seg_build(nv, t.nf, t.n_inst, faces, x4, pad, t.vert_base, t.face_base, t.face_inst, t.fmin4, t.fmax4,
+                         t.box6, t.codes, t.sorted_faces, t.sorted_inst, t.child_a, t.child_b, t.parent, t.range_l,
+                         t.range_r, t.roots, t.nlo4, t.nhi4, ws, ws_bytes, s);
   int rc = slx_face_aabbs(t.nf, faces, x4, pad, t.fmin4, t.fmax4, s);
   if (rc) return rc;
   rc = slx_mesh_aabb(nv, x4, pad, t.box6, ws, ws_bytes, s);
@@ -45,12 +52,15 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
                        t.nlo4, t.nhi4, ws, ws_bytes, s);
 }
 
-int query(const char* name, const slx_tree& t, int nq, const float* q4, float pad, int cap, int32_t* cand,
-          int32_t* cand_n, int32_t* status, slx_stream_t s) {
+// queries of a mesh whose vertices belong to instances q_inst (batched trees only)
+int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst, float pad, int cap,
+          int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t s) {
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
+  const bool seg = batched(t);
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4, t.fmin4, t.fmax4,
-                         t.box6, cap, cand, cand_n, status, (hipStream_t)s);
+                         t.box6, cap, cand, cand_n, status, (hipStream_t)s, seg ? t.roots : nullptr,
+                         seg ? q_inst : nullptr);
 }
 
 }  // namespace
@@ -95,6 +105,9 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(!coll || (d->faces && d->cand && d->cand_n && d->ws), "slx_frame: null collision pointer");
   SLX_REQUIRE(!coll || d->dyn.mode == 0 || d->dyn.mode == 1, "slx_frame: bad tree mode");
   SLX_REQUIRE(!coll || d->nvs == 0 || (d->xs4 && d->faces_st), "slx_frame: static mesh missing");
+  SLX_REQUIRE(!coll || !batched(d->dyn) || (d->dyn.vert_inst && d->dyn.roots && (d->nvs == 0 || (batched(d->st) &&
+                                                                                               d->st.vert_inst))),
+              "slx_frame: batched trees need instance ids on both meshes");
   SLX_REQUIRE(d->ws_bytes >= slx_frame_ws_bytes(d), "slx_frame: workspace too small");
   const int nv = d->nv, nvs = coll ? d->nvs : 0, cap = d->cap;
   const float pad = d->padding;
@@ -143,13 +156,15 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (!coll) continue;
       // broad phase (XPBD.py:186-215)
       if (nvs > 0) {
-        rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, pad, cap, c0, n0, d->status, stream);
+        rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, pad, cap, c0, n0, d->status,
+                   stream);
         if (rc) return rc;
       }
-      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, pad, cap, c1, n1, d->status, stream);
+      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, pad, cap, c1, n1, d->status,
+                 stream);
       if (rc) return rc;
       if (nvs > 0 && !tv_fresh) {
-        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, pad, cap, c2, n2, d->status, stream);
+        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, pad, cap, c2, n2, d->status, stream);
         if (rc) return rc;
         tv_fresh = true;
       }

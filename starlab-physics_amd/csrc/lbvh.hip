@@ -347,7 +347,12 @@ k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32
     return;
   }
   int sp = 0;
-  stk[0] = roots ? roots[q_inst[q]] : 0;   // batched trees: the query's instance subtree
+  const int root = roots ? roots[q_inst[q]] : 0;   // batched trees: the query's instance subtree
+  if (root < 0) {
+    cand_n[q] = 0;
+    return;
+  }
+  stk[0] = root;
   sp = 1;
   while (sp > 0) {
     const int idx = stk[(--sp) * kTravBS];
@@ -548,4 +553,199 @@ extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, in
   return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
                          sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream), nullptr,
                          nullptr);
+}
+
+// ===================== batched instances: one tree, one subtree per instance =====================
+//
+// Instances are packed CSR-style: vertices of instance i are [vert_base[i],
+// vert_base[i+1]), faces [face_base[i], face_base[i+1]) (framework/meshio/
+// concat.py:11-58 packs meshes the same way).  Each face gets a Morton code in its
+// own instance's box; a stable sort by code followed by a stable sort by instance
+// orders faces by (instance, code, index), the composite 64-bit key
+// inst<<54 | code<<24 | index.  The Karras tree over those keys contains, for
+// every instance, exactly one node whose range is that instance's faces (the
+// keys sharing the instance prefix); roots[i] is that node, and a query of
+// instance i starts its traversal there, so instances never see each other.
+namespace slx {
+
+constexpr int kSegInstBits = 10;      // up to 1024 instances
+constexpr int kSegIndexBits = 24;     // up to 16M faces
+
+__global__ void __launch_bounds__(256)
+k_seg_box(const int32_t* __restrict__ vert_base, const float4* __restrict__ x4, float pad, float* __restrict__ box) {
+  __shared__ float s[6][256];
+  const int inst = blockIdx.x;
+  const int b = vert_base[inst], e = vert_base[inst + 1];
+  float lo[3] = {1e5f, 1e5f, 1e5f}, hi[3] = {-1e5f, -1e5f, -1e5f};
+  for (int i = b + (int)threadIdx.x; i < e; i += 256) {
+    float4 p = x4[i];
+    lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+    hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+  }
+  for (int c = 0; c < 3; ++c) { s[c][threadIdx.x] = lo[c]; s[3 + c][threadIdx.x] = hi[c]; }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = fminf(s[c][threadIdx.x], s[c][threadIdx.x + o]);
+        s[3 + c][threadIdx.x] = fmaxf(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + o]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) box[6 * inst + threadIdx.x] = s[threadIdx.x][0] - pad * 1.0f;
+  else if (threadIdx.x < 6) box[6 * inst + threadIdx.x] = s[threadIdx.x][0] + pad * 1.0f;
+}
+
+__global__ void k_face_morton_seg(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                                  const int32_t* __restrict__ face_inst, const float* __restrict__ box,
+                                  uint32_t* __restrict__ codes, uint32_t* __restrict__ codes_keep,
+                                  int32_t* __restrict__ ids) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const float* bx = box + 6 * face_inst[f];
+  float4 a = fmin[f], b = fmax[f];
+  float3 c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+  uint32_t code = morton_in_box(c, make_float3(bx[0], bx[1], bx[2]), make_float3(bx[3], bx[4], bx[5]));
+  codes[f] = code;
+  codes_keep[f] = code;
+  ids[f] = f;
+}
+
+__global__ void k_gather_inst(int nf, const int32_t* __restrict__ ids, const int32_t* __restrict__ face_inst,
+                              uint32_t* __restrict__ key) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nf) key[k] = (uint32_t)face_inst[ids[k]];
+}
+
+__global__ void k_gather_codes(int nf, const int32_t* __restrict__ ids, const uint32_t* __restrict__ codes_keep,
+                               const int32_t* __restrict__ face_inst, uint32_t* __restrict__ codes,
+                               int32_t* __restrict__ sinst) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  int f = ids[k];
+  codes[k] = codes_keep[f];
+  sinst[k] = face_inst[f];
+}
+
+// composite key; the tie-break index is instance-local so every instance's
+// subtree has exactly the shape of a tree built over that instance alone
+__device__ __forceinline__ uint64_t seg_key(const uint32_t* codes, const int32_t* sinst, const int32_t* fbase, int i) {
+  const int inst = sinst[i];
+  return ((uint64_t)(uint32_t)inst << (64 - kSegInstBits)) | ((uint64_t)codes[i] << kSegIndexBits) |
+         (uint32_t)(i - fbase[inst]);
+}
+__device__ __forceinline__ int seg_prefix(const uint32_t* codes, const int32_t* sinst, const int32_t* fbase, int n,
+                                          int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  return __clzll(seg_key(codes, sinst, fbase, i) ^ seg_key(codes, sinst, fbase, j));
+}
+
+// Karras 2012 over the composite keys (distinct by construction)
+__global__ void k_karras_seg(int n, const uint32_t* __restrict__ codes, const int32_t* __restrict__ sinst,
+                             const int32_t* __restrict__ fbase, int32_t* __restrict__ ca, int32_t* __restrict__ cb, int32_t* __restrict__ parent,
+                             int32_t* __restrict__ rl, int32_t* __restrict__ rr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) parent[0] = -1;
+  if (i >= n - 1) return;
+  int dl = seg_prefix(codes, sinst, fbase, n, i, i - 1), dr = seg_prefix(codes, sinst, fbase, n, i, i + 1);
+  int d = (dr < dl) ? -1 : 1;
+  int dmin = (dr < dl) ? dr : dl;
+  int hi = 2;
+  while (seg_prefix(codes, sinst, fbase, n, i, i + hi * d) > dmin) hi *= 2;
+  int len = 0;
+  for (int t = hi / 2; t > 0; t /= 2)
+    if (seg_prefix(codes, sinst, fbase, n, i, i + (len + t) * d) > dmin) len += t;
+  int j = i + len * d;
+  int first = d > 0 ? i : j, last = d > 0 ? j : i;
+  int node_prefix = seg_prefix(codes, sinst, fbase, n, first, last);
+  int split = first, step = last - first;
+  do {
+    step = (step + 1) / 2;
+    int c = split + step;
+    if (c < last && seg_prefix(codes, sinst, fbase, n, first, c) > node_prefix) split = c;
+  } while (step > 1);
+  int a = (split == first) ? split + (n - 1) : split;
+  int b = (split + 1 == last) ? split + 1 + (n - 1) : split + 1;
+  ca[i] = a;
+  cb[i] = b;
+  parent[a] = i;
+  parent[b] = i;
+  rl[i] = first;
+  rr[i] = last;
+}
+
+// roots[i]: the node covering exactly instance i's sorted faces
+__global__ void k_seg_roots_leaf(int n_inst, int n, const int32_t* __restrict__ face_base, int32_t* __restrict__ roots) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  int c = face_base[i + 1] - face_base[i];
+  roots[i] = c == 1 ? face_base[i] + (n - 1) : -1;
+}
+__global__ void k_seg_roots_internal(int n, const int32_t* __restrict__ rl, const int32_t* __restrict__ rr,
+                                     const int32_t* __restrict__ sinst, const int32_t* __restrict__ face_base,
+                                     int32_t* __restrict__ roots) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int l = rl[i], r = rr[i], inst = sinst[l];
+  if (l == face_base[inst] && r == face_base[inst + 1] - 1) roots[inst] = i;
+}
+
+}  // namespace slx
+
+extern "C" size_t slx_seg_build_ws_bytes(int nf, int n_inst) {
+  size_t n = (size_t)(nf > 0 ? nf : 1);
+  (void)n_inst;
+  return 16 * n + radix_sort_ws_bytes(nf) + 2048;
+}
+
+extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float* x4, float padding,
+                             const int32_t* vert_base, const int32_t* face_base, const int32_t* face_inst,
+                             float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
+                             int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
+                             int32_t* range_l, int32_t* range_r, int32_t* roots, float* nlo4, float* nhi4, void* ws,
+                             size_t ws_bytes, slx_stream_t stream) {
+  SLX_REQUIRE(nv >= 0 && nf >= 0 && n_inst >= 1 && n_inst <= (1 << kSegInstBits), "slx_seg_build: bad sizes");
+  SLX_REQUIRE(nf < (1 << kSegIndexBits), "slx_seg_build: too many faces for 24-bit indices");
+  if (nf == 0) return SLX_OK;
+  SLX_REQUIRE(faces && x4 && vert_base && face_base && face_inst && fmin4 && fmax4 && box6 && codes && sorted_faces &&
+                  sorted_inst && child_a && child_b && parent && range_l && range_r && roots && nlo4 && nhi4 && ws,
+              "slx_seg_build: null pointer");
+  if (ws_bytes < slx_seg_build_ws_bytes(nf, n_inst)) return fail(SLX_EWS, "slx_seg_build: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* p = (char*)ws;
+  uint32_t* keep = (uint32_t*)p; p += 4 * (size_t)nf;
+  uint32_t* key2 = (uint32_t*)p; p += 4 * (size_t)nf;
+  p = (char*)ws + 16 * (size_t)nf + 1024;
+  void* sort_ws = p;
+  size_t sort_bytes = ws_bytes - (16 * (size_t)nf + 1024);
+  int rc = slx_face_aabbs(nf, faces, x4, padding, fmin4, fmax4, stream);
+  if (rc) return rc;
+  SLX_LAUNCH("seg_box", k_seg_box, dim3(n_inst), dim3(256), 0, st, vert_base, (const float4*)x4, padding, box6);
+  SLX_LAUNCH("face_morton", k_face_morton_seg, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const float4*)fmin4,
+             (const float4*)fmax4, face_inst, box6, (uint32_t*)codes, keep, sorted_faces);
+  rc = radix_sort_pairs(nf, nullptr, (uint32_t*)codes, sorted_faces, 30, sort_ws, sort_bytes, st);
+  if (rc) return rc;
+  SLX_LAUNCH("gather", k_gather_inst, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, sorted_faces, face_inst, key2);
+  rc = radix_sort_pairs(nf, nullptr, key2, sorted_faces, kSegInstBits, sort_ws, sort_bytes, st);
+  if (rc) return rc;
+  SLX_LAUNCH("gather", k_gather_codes, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, sorted_faces, keep, face_inst,
+             (uint32_t*)codes, sorted_inst);
+  SLX_LAUNCH("karras", k_karras_seg, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const uint32_t*)codes,
+             sorted_inst, face_base, child_a, child_b, parent, range_l, range_r);
+  SLX_LAUNCH("seg_roots", k_seg_roots_leaf, dim3(blocks_for(n_inst, 256)), dim3(256), 0, st, n_inst, nf, face_base,
+             roots);
+  if (nf > 1)
+    SLX_LAUNCH("seg_roots", k_seg_roots_internal, dim3(blocks_for(nf - 1, 256)), dim3(256), 0, st, nf, range_l,
+               range_r, sorted_inst, face_base, roots);
+  rc = check_launch("slx_seg_build");
+  if (rc) return rc;
+  return slx_tri_refit(nf, fmin4, fmax4, sorted_faces, child_a, child_b, parent, nlo4, nhi4, ws, ws_bytes, stream);
+}
+
+extern "C" int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots, float padding,
+                                int nf, const float* nlo4, const float* nhi4, int cap, int32_t* cand, int32_t* cand_n,
+                                int32_t* status, slx_stream_t stream) {
+  SLX_REQUIRE(roots && q_inst, "slx_traverse_seg: roots and q_inst required");
+  return traverse_launch("traverse_seg", 1, nq, q4, padding, nf, nullptr, nullptr, nlo4, nhi4, nullptr, nullptr,
+                         nullptr, cap, cand, cand_n, status, S(stream), roots, q_inst);
 }

@@ -94,15 +94,12 @@ __global__ void k_set_fixed(int nv, const float* __restrict__ fixed, const float
 //         (compute_velocity + update_x fused in) and written to x_out/v_out;
 //         otherwise it is written to y_out.
 template <bool PRED, bool FINAL>
-__global__ void __launch_bounds__(256)
-k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
-         const float* __restrict__ fixed, const float4* __restrict__ y_in,
-         float4* __restrict__ y_out, float4* __restrict__ x_out, float4* __restrict__ v_out,
-         const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_width,
-         const int2* __restrict__ ent, float k_s, float k_b, float dt, float3 gdt2,
-         float damping) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
+__device__ __forceinline__ void spring_vertex(int v, const float4* __restrict__ x4, const float4* __restrict__ v4,
+                                              const float* __restrict__ fixed, const float4* __restrict__ y_in,
+                                              float4* __restrict__ y_out, float4* __restrict__ x_out,
+                                              float4* __restrict__ v_out, const int32_t* __restrict__ slice_off,
+                                              const int32_t* __restrict__ slice_width, const int2* __restrict__ ent,
+                                              float k_s, float k_b, float dt, float3 gdt2, float damping) {
   const int s = v >> 6, lane = v & 63;
   const int base = slice_off[s] + lane, width = slice_width[s];
   const float4 xv = x4[v];
@@ -154,26 +151,56 @@ k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
   }
 }
 
+// VPT vertices per thread: block b covers [b*blockDim*VPT, (b+1)*blockDim*VPT), thread t
+// takes t, t + blockDim, ... so every load instruction stays coalesced per wave.
+template <bool PRED, bool FINAL, int VPT>
+__global__ void __launch_bounds__(256)
+k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
+         const float* __restrict__ fixed, const float4* __restrict__ y_in,
+         float4* __restrict__ y_out, float4* __restrict__ x_out, float4* __restrict__ v_out,
+         const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_width,
+         const int2* __restrict__ ent, float k_s, float k_b, float dt, float3 gdt2,
+         float damping) {
+  const int base = blockIdx.x * blockDim.x * VPT + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    const int v = base + j * (int)blockDim.x;
+    if (v < nv)
+      spring_vertex<PRED, FINAL>(v, x4, v4, fixed, y_in, y_out, x_out, v_out, slice_off, slice_width, ent, k_s, k_b,
+                                 dt, gdt2, damping);
+  }
+}
+
+// launch shape of the spring kernels (autotuned by the host: block x vertices/thread)
+static int g_spring_block = 256, g_spring_vpt = 1;
+
 int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4* v4, const float* fixed,
                   const float4* y_in, float4* y_out, float4* x_out, float4* v_out, const int32_t* slice_off,
                   const int32_t* slice_width, const int32_t* entries, float k_s, float k_b, float dt, float gx,
                   float gy, float gz, float damping, hipStream_t st) {
   if (nv <= 0) return SLX_OK;
   const float3 gdt2 = make_float3((gx * dt) * dt, (gy * dt) * dt, (gz * dt) * dt);
-  const dim3 grid(blocks_for(nv, 256)), blk(256);
+  const int bs = g_spring_block, vpt = g_spring_vpt;
+  const dim3 grid(blocks_for(nv, bs * vpt)), blk(bs);
   const int2* E = (const int2*)entries;
-  if (pred && final)
-    SLX_LAUNCH("spring_pred_final", (k_spring<true, true>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
-               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
-  else if (pred)
-    SLX_LAUNCH("spring_pred", (k_spring<true, false>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
-               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
-  else if (final)
-    SLX_LAUNCH("spring_final", (k_spring<false, true>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out,
-               v_out, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
-  else
-    SLX_LAUNCH("spring", (k_spring<false, false>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out, v_out,
-               slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+#define SLX_SPRING(NAME, P, F, V)                                                                              \
+  SLX_LAUNCH(NAME, (k_spring<P, F, V>), grid, blk, 0, st, nv, x4, v4, fixed, y_in, y_out, x_out, v_out, slice_off, \
+             slice_width, E, k_s, k_b, dt, gdt2, damping)
+#define SLX_SPRING_V(NAME, P, F)   \
+  do {                             \
+    if (vpt == 4)                  \
+      SLX_SPRING(NAME, P, F, 4);   \
+    else if (vpt == 2)             \
+      SLX_SPRING(NAME, P, F, 2);   \
+    else                           \
+      SLX_SPRING(NAME, P, F, 1);   \
+  } while (0)
+  if (pred && final) SLX_SPRING_V("spring_pred_final", true, true);
+  else if (pred) SLX_SPRING_V("spring_pred", true, false);
+  else if (final) SLX_SPRING_V("spring_final", false, true);
+  else SLX_SPRING_V("spring", false, false);
+#undef SLX_SPRING_V
+#undef SLX_SPRING
   return check_launch("k_spring");
 }
 
@@ -223,11 +250,9 @@ extern "C" int slx_spring_pass(int nv, const float* x4, const float* fixed, cons
   SLX_REQUIRE(x4 && fixed && y4_in && y4_out && slice_off && slice_width && entries,
               "slx_spring_pass: null pointer");
   SLX_REQUIRE(y4_in != y4_out, "slx_spring_pass: y4_in and y4_out must differ (Jacobi)");
-  SLX_LAUNCH("spring", (k_spring<false, false>), dim3(blocks_for(nv, 256)), dim3(256), 0, S(stream),
-                     nv, (const float4*)x4, (const float4*)nullptr, fixed, (const float4*)y4_in,
-                     (float4*)y4_out, (float4*)nullptr, (float4*)nullptr, slice_off, slice_width,
-                     (const int2*)entries, k_s, k_b, 0.f, make_float3(0.f, 0.f, 0.f), 0.f);
-  return check_launch("k_spring");
+  return spring_launch(false, false, nv, (const float4*)x4, nullptr, fixed, (const float4*)y4_in, (float4*)y4_out,
+                       nullptr, nullptr, slice_off, slice_width, entries, k_s, k_b, 0.f, 0.f, 0.f, 0.f, 0.f,
+                       S(stream));
 }
 
 extern "C" int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, float* v4_alt,
@@ -241,36 +266,42 @@ extern "C" int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, f
   SLX_REQUIRE(x4 && v4 && x4_alt && v4_alt && fixed && slice_off && slice_width && entries,
               "slx_substeps_plain: null pointer");
   SLX_REQUIRE(n_iters == 1 || (y4a && y4b), "slx_substeps_plain: y buffers required");
-  const float3 gdt2 = make_float3((gx * dt) * dt, (gy * dt) * dt, (gz * dt) * dt);
-  const dim3 grid(blocks_for(nv, 256)), blk(256);
   hipStream_t st = S(stream);
   float4 *cx = (float4*)x4, *cv = (float4*)v4, *ax = (float4*)x4_alt, *av = (float4*)v4_alt;
   float4 *ya = (float4*)y4a, *yb = (float4*)y4b;
-  const int2* E = (const int2*)entries;
-  for (int s = 0; s < n_substeps; ++s) {
+  int rc = SLX_OK;
+  for (int s = 0; s < n_substeps && rc == SLX_OK; ++s) {
     if (n_iters == 1) {
-      SLX_LAUNCH("spring_pred_final", (k_spring<true, true>), grid, blk, 0, st, nv, cx, cv, fixed, nullptr,
-                         nullptr, ax, av, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+      rc = spring_launch(true, true, nv, cx, cv, fixed, nullptr, nullptr, ax, av, slice_off, slice_width, entries,
+                         k_s, k_b, dt, gx, gy, gz, damping, st);
     } else {
-      SLX_LAUNCH("spring_pred", (k_spring<true, false>), grid, blk, 0, st, nv, cx, cv, fixed, nullptr, ya,
-                         nullptr, nullptr, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
-      for (int it = 1; it + 1 < n_iters; ++it) {
-        SLX_LAUNCH("spring", (k_spring<false, false>), grid, blk, 0, st, nv, cx, cv, fixed, ya, yb,
-                           nullptr, nullptr, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+      rc = spring_launch(true, false, nv, cx, cv, fixed, nullptr, ya, nullptr, nullptr, slice_off, slice_width,
+                         entries, k_s, k_b, dt, gx, gy, gz, damping, st);
+      for (int it = 1; it + 1 < n_iters && rc == SLX_OK; ++it) {
+        rc = spring_launch(false, false, nv, cx, cv, fixed, ya, yb, nullptr, nullptr, slice_off, slice_width,
+                           entries, k_s, k_b, dt, gx, gy, gz, damping, st);
         float4* t = ya; ya = yb; yb = t;
       }
-      SLX_LAUNCH("spring_final", (k_spring<false, true>), grid, blk, 0, st, nv, cx, cv, fixed, ya, nullptr,
-                         ax, av, slice_off, slice_width, E, k_s, k_b, dt, gdt2, damping);
+      if (rc == SLX_OK)
+        rc = spring_launch(false, true, nv, cx, cv, fixed, ya, nullptr, ax, av, slice_off, slice_width, entries,
+                           k_s, k_b, dt, gx, gy, gz, damping, st);
     }
     float4* t = cx; cx = ax; ax = t;
     t = cv; cv = av; av = t;
   }
-  int rc = check_launch("k_spring (substeps)");
   if (rc) return rc;
   if (cx != (float4*)x4) {
     if (hipMemcpyAsync(x4, cx, sizeof(float4) * (size_t)nv, hipMemcpyDeviceToDevice, st) != hipSuccess ||
         hipMemcpyAsync(v4, cv, sizeof(float4) * (size_t)nv, hipMemcpyDeviceToDevice, st) != hipSuccess)
       return fail(SLX_EHIP, "slx_substeps_plain: copy-back failed");
   }
+  return SLX_OK;
+}
+
+extern "C" int slx_set_spring_launch(int block, int vpt) {
+  SLX_REQUIRE(block == 64 || block == 128 || block == 256, "slx_set_spring_launch: block must be 64, 128 or 256");
+  SLX_REQUIRE(vpt == 1 || vpt == 2 || vpt == 4, "slx_set_spring_launch: vertices per thread must be 1, 2 or 4");
+  slx::g_spring_block = block;
+  slx::g_spring_vpt = vpt;
   return SLX_OK;
 }

@@ -23,7 +23,7 @@ from . import topology as _topo
 class Mesh:
     def __init__(self, model_dir=None, model_name=None, offsets=(0,), trans=(0.0, 0.0, 0.0),
                  rot=(0.0, 0.0, 0.0), scale=1.0, is_static=False, *, x=None, faces=None,
-                 device="cuda"):
+                 device="cuda", topology=None):
         if x is None:
             if model_dir is None or model_name is None:
                 raise ValueError("Mesh needs (model_dir, model_name) or x= and faces=")
@@ -35,17 +35,21 @@ class Mesh:
         x = _scene.apply_transform(np.asarray(x, np.float32), trans, rot, scale)
         self.fid_np = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
         self.x0_np = np.ascontiguousarray(x, np.float32)
-        self.eid_np = _topo.edges_from_faces(self.fid_np)
-        self.l0_np = _topo.rest_lengths(self.x0_np, self.eid_np)
         self.num_verts = len(self.x0_np)
-        self.num_edges = len(self.eid_np)
         self.num_faces = len(self.fid_np)
-        self.m_inv_np = _topo.lumped_inverse_mass(self.num_verts, self.eid_np, self.l0_np)
-        if is_static:
-            self.bend_np = np.zeros((0, 2), np.int32)
+        if topology is not None:
+            # precomputed (edges, l0, m_inv, bending pairs, bending l0), e.g. per packed instance
+            self.eid_np, self.l0_np, self.m_inv_np, self.bend_np, self.bending_l0_np = topology
         else:
-            self.bend_np = _topo.bending_pairs(self.fid_np)
-        self.bending_l0_np = _topo.rest_lengths(self.x0_np, self.bend_np)
+            self.eid_np = _topo.edges_from_faces(self.fid_np)
+            self.l0_np = _topo.rest_lengths(self.x0_np, self.eid_np)
+            self.m_inv_np = _topo.lumped_inverse_mass(self.num_verts, self.eid_np, self.l0_np)
+            if is_static:
+                self.bend_np = np.zeros((0, 2), np.int32)
+            else:
+                self.bend_np = _topo.bending_pairs(self.fid_np)
+            self.bending_l0_np = _topo.rest_lengths(self.x0_np, self.bend_np)
+        self.num_edges = len(self.eid_np)
         self.bending_constraint_count = len(self.bend_np)
         self.adj = None if is_static else _topo.Adjacency(
             self.num_verts, self.eid_np, self.l0_np, self.bend_np, self.bending_l0_np)
@@ -137,6 +141,46 @@ class Mesh:
     @classmethod
     def from_trimesh(cls, tm, is_static=False, device="cuda", **kw):
         return cls(x=tm.x, faces=tm.faces, is_static=is_static, device=device, **kw)
+
+    @classmethod
+    def from_instances(cls, parts, is_static=False, device="cuda"):
+        """Pack independent meshes into one, CSR-style: vertex and face offsets per
+        instance (the concatenation of framework/meshio/concat.py:11-58), plus the
+        instance id of every vertex and face.  Constraints never cross instances,
+        so the solver runs on the packed mesh unchanged; the broad phase keeps one
+        subtree per instance (slx_seg_build)."""
+        tm = _scene.concat(parts)
+        # topology per instance, offset and concatenated: instance vertex ranges are
+        # increasing, so this equals the topology of the packed mesh, in the same order
+        E, L, W, Bp, BL = [], [], [], [], []
+        off = 0
+        for p in parts:
+            px = np.ascontiguousarray(p.x, np.float32)
+            e = _topo.edges_from_faces(p.faces)
+            l0 = _topo.rest_lengths(px, e)
+            b = np.zeros((0, 2), np.int32) if is_static else _topo.bending_pairs(p.faces)
+            E.append(e + off)
+            L.append(l0)
+            W.append(_topo.lumped_inverse_mass(len(px), e, l0))
+            Bp.append(b + off)
+            BL.append(_topo.rest_lengths(px, b))
+            off += len(p.x)
+        topo = (np.concatenate(E).astype(np.int32), np.concatenate(L), np.concatenate(W),
+                np.concatenate(Bp).astype(np.int32).reshape(-1, 2), np.concatenate(BL))
+        m = cls(x=tm.x, faces=tm.faces, is_static=is_static, device=device, topology=topo)
+        nvs = np.array([len(p.x) for p in parts], np.int64)
+        nfs = np.array([len(p.faces) for p in parts], np.int64)
+        ids = np.arange(len(parts), dtype=np.int32)
+        m.inst_np = dict(n=len(parts),
+                         vert_base=np.concatenate([[0], np.cumsum(nvs)]).astype(np.int32),
+                         face_base=np.concatenate([[0], np.cumsum(nfs)]).astype(np.int32),
+                         face_inst=np.repeat(ids, nfs).astype(np.int32),
+                         vert_inst=np.repeat(ids, nvs).astype(np.int32))
+        if device is not None:
+            import torch
+            m.inst = {k: (torch.from_numpy(v).to(m.torch_device) if isinstance(v, np.ndarray) else v)
+                      for k, v in m.inst_np.items()}
+        return m
 
     def __len__(self):
         return self.num_verts

@@ -149,3 +149,27 @@ def test_concat_offsets(slx):
     c = slx.scene.concat([a, b])
     assert slx.scene.concat_offsets([a, b]) == [0, 9, 25]
     np.testing.assert_array_equal(c.faces[len(a.faces):], b.faces + 9)
+
+
+def test_packed_instances_equal_global_topology(slx):
+    """Per-instance topology, offset and concatenated (Mesh.from_instances), equals
+    the topology of the concatenated mesh computed as a whole."""
+    import importlib
+    batch = importlib.import_module("starlab-physics_amd.batch")
+    cloths, _, sides, _ = batch.make_instances(5, 5, 13, seed=2, sphere_subdiv=1)
+    m = slx.Mesh.from_instances(cloths, device=None)
+    tm = slx.scene.concat(cloths)
+    g = slx.Mesh(x=tm.x, faces=tm.faces, device=None)
+    for a in ("eid_np", "l0_np", "m_inv_np", "bend_np", "bending_l0_np"):
+        np.testing.assert_array_equal(getattr(m, a), getattr(g, a))
+    np.testing.assert_array_equal(m.adj.entries, g.adj.entries)
+    vb = m.inst_np["vert_base"]
+    assert vb[-1] == m.num_verts and np.all(np.diff(vb) == sides ** 2)
+    assert np.all(m.inst_np["face_inst"][m.inst_np["face_base"][:-1]] == np.arange(5))
+
+
+def test_config4_instance_sizes(slx):
+    import importlib
+    batch = importlib.import_module("starlab-physics_amd.batch")
+    sides = batch.instance_sides(256, 16, 256, seed=0)
+    assert int((sides.astype(np.int64) ** 2).sum()) == 6416788
