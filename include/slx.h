@@ -230,6 +230,40 @@ int slx_gs_pass(int nv, const float* x4, const float* fixed, float* y4, const in
                 const int32_t* color_off_host, int n_colors, float k_s, float k_b,
                 slx_stream_t stream);
 
+/* ---------------- learned warm start (no reference counterpart) ---------------- */
+
+/* Number of FP32 parameters of the per-vertex MLP (24 -> 64 -> 64 -> 64 -> 3,
+ * GELU; layout W1[64][32] b1 W2[64][64] b2 W3[64][64] b3 W4[32][64] b4[32],
+ * weights [out][in], pads zero). */
+int slx_mlp_num_params(void);
+/* 24 features per vertex (FP32, feat[nv][24]) from x4, v4 and the predicted y4
+ * over the SELL adjacency; inv_s = 1 / mean rest length. */
+int slx_features(int nv, const float* x4, const float* v4, const float* y4, const int32_t* slice_off,
+                 const int32_t* slice_width, const int32_t* entries, float dt, float inv_s,
+                 float* feat, slx_stream_t stream);
+/* Warm start of a substep's Jacobi iterate: features of y4_pred -> MLP on BF16
+ * MFMA (w_bf16 weights, w_f32 biases, feature mean / inverse std) ->
+ * y4_out = y4_pred + fixed * out * out_scale. */
+int slx_warm_start(int nv, const float* x4, const float* v4, const float* y4_pred, float* y4_out,
+                   const float* fixed, const int32_t* slice_off, const int32_t* slice_width,
+                   const int32_t* entries, float dt, float inv_s, const void* w_bf16,
+                   const float* w_f32, const float* fmean, const float* fistd, float out_scale,
+                   slx_stream_t stream);
+/* Gradient of mean_n sum_3 (out - target)^2 w.r.t. the parameters (forward and
+ * backward on BF16 MFMA, FP32 accumulation): per-block partials part[n_blocks][P],
+ * loss partials loss_part[n_blocks], and (nullable) their sum in block order. */
+int slx_mlp_grad(int n, const float* feat, const float* target, const void* w_bf16,
+                 const float* w_f32, const float* fmean, const float* fistd, int n_blocks,
+                 float* part, float* loss_part, float* grad, slx_stream_t stream);
+/* Adam step on the FP32 master weights from the summed partials; refreshes the
+ * BF16 copy. step counts from 1. */
+int slx_mlp_adam(int n_blocks, const float* part, float* w_f32, void* w_bf16, float* m1, float* m2,
+                 float lr, float beta1, float beta2, float eps, int step, slx_stream_t stream);
+int slx_mlp_to_bf16(const float* w_f32, void* w_bf16, slx_stream_t stream);
+/* *out = max(*out, max_v |a - b|_inf) as float bits (caller zeroes *out): the
+ * Jacobi update residual. */
+int slx_max_update(int nv, const float* a4, const float* b4, uint32_t* out, slx_stream_t stream);
+
 /* ---------------- one frame (Solver.forward, XPBD.py:443-469) ---------------- */
 
 typedef struct slx_tree {

@@ -57,6 +57,13 @@ SIGNATURES = {
     "slx_seg_build": [I32, I32, I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
                       PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
     "slx_traverse_seg": [I32, PTR, PTR, PTR, F32, I32, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_mlp_num_params": [],
+    "slx_features": [I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, PTR, STREAM],
+    "slx_warm_start": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, PTR, PTR, PTR, PTR, F32, STREAM],
+    "slx_mlp_grad": [I32, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_mlp_adam": [I32, PTR, PTR, PTR, PTR, PTR, F32, F32, F32, F32, I32, STREAM],
+    "slx_mlp_to_bf16": [PTR, PTR, STREAM],
+    "slx_max_update": [I32, PTR, PTR, PTR, STREAM],
     "slx_greedy_edge_coloring": [I32, I32, PTR, I32, PTR, PTR, PTR],
     "slx_gs_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, F32, F32, STREAM],
 }

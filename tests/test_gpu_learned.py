@@ -1,0 +1,156 @@
+"""GPU checks of the learned warm start: features and MLP forward against the
+float64 reference (BF16 operands emulated; tolerance 2e-2 relative on outputs,
+1e-5 on features), gradients of the MFMA backward against autograd (relative
+error of the whole gradient <= 3e-2), training reduces the loss, and the trained
+network reaches the residual in fewer Jacobi iterations than plain Jacobi, with
+the plain solver as fallback."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def L():
+    return importlib.import_module("starlab-physics_amd.learned")
+
+
+def cloth(slx, n=48, seed=0):
+    import torch
+    g = slx.scene.grid_cloth(n, size=1.0, height=0.5, jitter=1e-3, seed=seed)
+    mesh = slx.Mesh(x=g.x, faces=g.faces)
+    sim = slx.Solver(mesh, None, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3,
+                     g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+    sel = torch.zeros(mesh.num_verts, device="cuda")
+    sel[0] = 1
+    sel[n - 1] = 1
+    sim.set_fixed_vertices(sel)
+    return mesh, sim
+
+
+def test_mfma_probe_layout():
+    """The MLP's MFMA operand layout assumption, on a known product."""
+    import torch
+    learned = L()
+    md = learned.WarmStart(seed=1)
+    rng = np.random.default_rng(0)
+    feat = torch.from_numpy(rng.normal(0, 1, (96, 24)).astype(np.float32)).cuda()
+    tgt = torch.zeros((96, 3), dtype=torch.float32, device="cuda")
+    g, loss = md.grad(feat, tgt)
+    from oracle import mlp_ref
+    out, _ = mlp_ref.forward(mlp_ref.unpack(md.w.cpu().numpy()), feat.cpu().numpy(), np.zeros(24), np.ones(24))
+    ref_loss = float((out[:, :3] ** 2).sum() / 96)
+    assert abs(loss.item() - ref_loss) <= 2e-2 * ref_loss
+
+
+def test_features_match_reference(slx):
+    import torch
+    from oracle import mlp_ref
+    mesh, sim = cloth(slx, 24)
+    for _ in range(3):
+        sim.forward(10, 1)
+    rs = L().ResidualSolver(sim)
+    s = sim.scalars(10)
+    rs.predict(s)
+    F = rs.features(s).cpu().numpy()
+    ref = mlp_ref.features(mesh.positions(), mesh.v4[:, :3].cpu().numpy(), rs.ypred[:, :3].cpu().numpy(),
+                           mesh.adj, s["dt"], rs.inv_s)
+    np.testing.assert_allclose(F, ref, rtol=1e-4, atol=1e-5)
+    assert np.abs(ref).max() > 1e-3
+
+
+def test_gradient_matches_autograd():
+    import torch
+    from oracle import mlp_ref
+    learned = L()
+    md = learned.WarmStart(seed=2)
+    rng = np.random.default_rng(1)
+    n = 1000
+    feat = rng.normal(0, 1, (n, 24)).astype(np.float32)
+    tgt = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    fm, fi = rng.normal(0, 0.1, 24).astype(np.float32), rng.uniform(0.8, 1.2, 24).astype(np.float32)
+    md.fmean.copy_(torch.from_numpy(fm))
+    md.fistd.copy_(torch.from_numpy(fi))
+    g, loss = md.grad(torch.from_numpy(feat).cuda(), torch.from_numpy(tgt).cuda())
+    L_ref, g_ref = mlp_ref.loss_and_grad(md.w.cpu().numpy(), feat, tgt, fm, fi)
+    g = g.cpu().numpy()
+    assert abs(loss.item() - L_ref) <= 1e-2 * L_ref
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"MFMA backward vs float64 autograd: relative error {rel:.3e}")
+    assert rel < 3e-2
+    # per layer, and padded parameters have zero gradient
+    o = 0
+    for name, shp in mlp_ref.LAYOUT:
+        k = int(np.prod(shp))
+        a, b = g[o:o + k], g_ref[o:o + k]
+        assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b) + 1e-7, name
+        o += k
+    P = mlp_ref.unpack(g)
+    assert np.all(P["W1"][:, 24:].numpy() == 0) and np.all(P["W4"][3:].numpy() == 0)
+
+
+def test_forward_matches_reference_through_warm_start(slx):
+    import torch
+    from oracle import mlp_ref
+    mesh, sim = cloth(slx, 20)
+    learned = L()
+    md = learned.WarmStart(seed=3)
+    md.out_scale = 1e-3
+    rs = learned.ResidualSolver(sim, md)
+    s = sim.scalars(10)
+    rs.predict(s)
+    F = rs.features(s).cpu().numpy()
+    rs.warm(s)
+    got = (rs.ya[:, :3] - rs.ypred[:, :3]).cpu().numpy() / md.out_scale
+    out, _ = mlp_ref.forward(mlp_ref.unpack(md.w.cpu().numpy()), F, md.fmean.cpu().numpy(), md.fistd.cpu().numpy())
+    ref = out[:, :3].detach().numpy() * mesh.fixed.cpu().numpy()[:, None]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-2, err
+
+
+def test_training_and_warm_start_gain(slx):
+    """Train on converged plain-Jacobi substeps of seeded variants; on a held-out
+    seed the warm start needs fewer Jacobi iterations to reach the residual."""
+    import torch
+    learned = L()
+    rng = np.random.default_rng(0)
+    feats, tgts = [], []
+    for seed in (1, 2):
+        mesh, sim = cloth(slx, 32, seed)
+        rs = learned.ResidualSolver(sim)
+        for _ in range(40):
+            rs.collect(10, 1e-6, 3000, 800, rng, feats, tgts)
+    feat = torch.cat(feats).contiguous()
+    tgt = torch.cat(tgts).contiguous()
+    md = learned.WarmStart(seed=0)
+    md.set_normalisation(feat, tgt)
+    losses = md.train(feat, tgt, epochs=30, batch=4096, lr=2e-3)
+    assert losses[-1] < 0.5 * losses[0]
+    res = {}
+    for learned_on in (False, True):
+        mesh, sim = cloth(slx, 32, seed=7)
+        rs = learned.ResidualSolver(sim, md)
+        it = 0
+        for _ in range(30):
+            k, _ = rs.substep(10, tol=1e-5, max_iters=3000, learned=learned_on)
+            it += k
+        res[learned_on] = it
+    print("iterations plain / learned:", res[False], res[True])
+    assert res[True] < res[False]
+
+
+def test_fallback_when_the_network_is_wrong(slx):
+    learned = L()
+    mesh, sim = cloth(slx, 24)
+    md = learned.WarmStart(seed=5)
+    md.out_scale = float("nan")   # a broken network: the residual is not finite
+    rs = learned.ResidualSolver(sim, md)
+    k, fell = rs.substep(10, tol=1e-5, max_iters=3000, learned=True)
+    assert fell
+    assert np.isfinite(mesh.positions()).all()
+    # the fallback substep equals a plain substep from the same state
+    mesh2, sim2 = cloth(slx, 24)
+    rs2 = learned.ResidualSolver(sim2)
+    rs2.substep(10, tol=1e-5, max_iters=3000, learned=False)
+    np.testing.assert_array_equal(mesh.positions(), mesh2.positions())
