@@ -5,8 +5,8 @@ vertex cloth, 1 m square (1,048,576 vertices, 2,093,058 triangles, 3,141,633
 stretch and 3,137,541 bending constraints), FP32, draped over a static
 142 x 142 height-field terrain (39,762 triangles, sum of 8 seeded sinusoids,
 amplitude 0.1 m).  One frame = 20 substeps x 2 Jacobi iterations at
-dt = 1/60 s, with vertex-triangle collision (terrain and self, thickness 1 mm:
-dHat = 1e-6 m^2 on squared distance, padding 0.5 mm), a per-triangle LBVH
+dt = 1/60 s, with vertex-triangle collision (terrain and self, thickness 0.5 mm:
+dHat = 2.5e-7 m^2 on squared distance, padding 0.25 mm), a per-triangle LBVH
 rebuilt every frame and refit every substep.  A "step" is one frame.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-collision] [--no-cpu-baseline]
@@ -39,10 +39,18 @@ def parse():
     p.add_argument("--no-collision", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--workload", default="cloth", choices=["cloth", "batched", "learned"],
+                   help="cloth: the 1M-vertex cloth (default); batched: 256 uneven instances in one "
+                        "captured graph; learned: time to residual, plain vs learned warm start")
+    p.add_argument("--instances", type=int, default=256)
     return p.parse_args()
 
 
-SCENE = dict(size=1.0, height=0.12, thickness=1e-3, padding=5e-4, stiffness_stretch=1e6,
+# thickness 0.5 mm < 0.707 x edge length (0.98 mm): the nearest non-incident
+# triangle of a flat grid vertex is 0.707 h away, so a flat cloth does not
+# self-collide at rest; padding = thickness / 2 keeps every pair within the
+# thickness in the broad phase
+SCENE = dict(size=1.0, height=0.12, thickness=5e-4, padding=2.5e-4, stiffness_stretch=1e6,
              stiffness_bending=1e3, g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
 
 
@@ -96,6 +104,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is a design, not built)"}))
         sys.exit(2)
+    if a.workload == "batched":
+        return bench_batched(a)
+    if a.workload == "learned":
+        return bench_learned(a)
     import torch
     slx = importlib.import_module("starlab-physics_amd")
     rl = importlib.import_module("starlab-physics_amd.roofline")
@@ -161,6 +173,111 @@ def main():
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cloth, terr, sim, a.iters, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(a):
+    """256 cloth instances, side ~ U{16..256} (seed 0: 6,416,788 vertices), each
+    on its own sphere (r ~ U(0.2, 0.5)), packed CSR-style; one frame (20 substeps
+    x 2 Jacobi iterations, per-instance LBVHs rebuilt per frame and refit per
+    substep) captured in a HIP graph; spring launch shape autotuned."""
+    import torch
+    batch = importlib.import_module("starlab-physics_amd.batch")
+    t0 = time.perf_counter()
+    cloths, spheres, sides, radii = batch.make_instances(a.instances, 16, 256, seed=0)
+    bs = batch.BatchedScene(cloths, spheres, thickness=5e-3)
+    setup = time.perf_counter() - t0
+    best, tune = bs.autotune(a.substeps, a.iters, reps=2)
+    for _ in range(a.warmup):
+        bs.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        bs.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    V = bs.cloth.num_verts
+    out = {
+        "metric": "vertex-substeps/s", "value": V * a.substeps * a.steps / el, "unit": "vertex-substeps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded instances)",
+        "config": {"workload": f"{a.instances} batched cloth instances ({V} vertices total, sides 16..256), "
+                               f"own sphere each, {a.substeps} substeps x {a.iters} iterations per frame, "
+                               "one captured HIP graph per frame",
+                   "instances": a.instances, "vertices": V},
+        "substeps_per_s": a.substeps * a.steps / el,
+        "instance_substeps_per_s": a.instances * a.substeps * a.steps / el,
+        "autotune": {"best": list(best), "frame_ms": {f"{k[0]}x{k[1]}": round(v * 1e3, 3) for k, v in tune.items()}},
+        "status": bs.sim.status(), "setup_s": setup,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(a):
+    """Time to residual 1e-5 m per substep, plain Jacobi vs learned warm start, on
+    a 256 x 256 cloth (one layer of the stacked-layers scene family, collision
+    off).  Training data: converged (1e-6) plain substeps of seeded variants."""
+    import torch
+    slx = importlib.import_module("starlab-physics_amd")
+    learned = importlib.import_module("starlab-physics_amd.learned")
+    n = 256
+
+    def scene(seed):
+        g = slx.scene.grid_cloth(n, size=1.0, height=0.5, jitter=1e-3, seed=seed)
+        mesh = slx.Mesh(x=g.x, faces=g.faces)
+        sim = slx.Solver(mesh, None, dHat=2.5e-5, stiffness_stretch=1e6, stiffness_bending=1e3,
+                         g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+        sel = torch.zeros(mesh.num_verts, device="cuda")
+        sel[0] = 1
+        sel[n - 1] = 1
+        sim.set_fixed_vertices(sel)
+        return mesh, sim
+
+    rng = np.random.default_rng(0)
+    feats, tgts = [], []
+    t0 = time.perf_counter()
+    seeds = (1, 2, 3, 4)
+    per = 200000 // (len(seeds) * 25) + 1
+    for seed in seeds:
+        mesh, sim = scene(seed)
+        rs = learned.ResidualSolver(sim)
+        for _ in range(25):
+            rs.collect(a.substeps, 1e-6, 20000, per, rng, feats, tgts)
+    feat = torch.cat(feats)[:200000].contiguous()
+    tgt = torch.cat(tgts)[:200000].contiguous()
+    collect_s = time.perf_counter() - t0
+    md = learned.WarmStart(seed=0)
+    md.set_normalisation(feat, tgt)
+    t0 = time.perf_counter()
+    losses = md.train(feat, tgt, epochs=60, batch=8192, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    res = {}
+    for mode in ("plain", "learned"):
+        mesh, sim = scene(99)
+        rs = learned.ResidualSolver(sim, md)
+        for _ in range(3):                       # same warm-up state for both
+            rs.substep(a.substeps, tol=1e-5, max_iters=20000, learned=False)
+        torch.cuda.synchronize()
+        iters, falls = 0, 0
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            k, fell = rs.substep(a.substeps, tol=1e-5, max_iters=20000, learned=(mode == "learned"))
+            iters += k
+            falls += int(fell)
+        torch.cuda.synchronize()
+        res[mode] = dict(wall_s=time.perf_counter() - t0, iterations=iters, fallbacks=falls)
+    speed = res["plain"]["wall_s"] / res["learned"]["wall_s"]
+    out = {
+        "metric": "time-to-residual 1e-5 (plain / learned)", "value": speed, "unit": "x", "n_gpus": 1,
+        "steps": a.steps, "warmup": 3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 solve, bf16 MLP", "data": "synthetic (seeded cloth variants)",
+        "config": {"workload": f"{n}x{n} cloth, {a.steps} substeps of dt/{a.substeps}, Jacobi to update residual "
+                               "<= 1e-5 m; MLP 24-64-64-64-3 GELU, 200k samples at residual 1e-6"},
+        "plain": res["plain"], "learned": res["learned"], "train_loss_first_last": [losses[0], losses[-1]],
+        "collect_s": collect_s, "train_s": train_s,
+    }
     print(json.dumps(out), flush=True)
 
 
