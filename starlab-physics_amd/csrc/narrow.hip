@@ -186,25 +186,36 @@ __device__ __forceinline__ bool query_of(const QuerySets& Q, int qg, int& set, i
 }
 
 // Pass 1: number of contact records per query (and recorded pair counts).
+// Records of the first set (dynamic vertex vs static triangle) only ever move
+// the query vertex itself and come first in its serial order, so that thread
+// sums them in candidate order right here (self_acc = (sum, count)); only the
+// other two sets go through the emit / sort path.
 __global__ void k_contact_count(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
                                 const float4* __restrict__ y4, const float4* __restrict__ xs4,
                                 const float* __restrict__ fixed, const float* __restrict__ m_inv, float dhat,
-                                float k, int32_t* __restrict__ cnt, int32_t* __restrict__ pair_num) {
+                                float k, int32_t* __restrict__ cnt, int32_t* __restrict__ pair_num,
+                                float4* __restrict__ self_acc) {
   const int nqt = 2 * Q.nv + Q.nvs;
   int qg = blockIdx.x * blockDim.x + threadIdx.x;
   if (qg >= nqt) return;
   int set, qv, ncand, nq;
   const int32_t* cand;
   int recs = 0, pairs = 0;
+  float3 dx = make_float3(0.f, 0.f, 0.f);
   if (query_of(Q, qg, set, qv, cand, ncand, nq)) {
     for (int j = 0; j < ncand; ++j) {
       int f = cand[(size_t)j * nq + qv];
       if (set == 1 && (fdy[3 * f] == qv || fdy[3 * f + 1] == qv || fdy[3 * f + 2] == qv)) continue;
       Contact c;
       contact(set, qv, f, fdy, fst, y4, xs4, fixed, m_inv, dhat, k, c);
+      if (set == 0 && c.mask) dx = make_float3(dx.x + c.delta[0].x, dx.y + c.delta[0].y, dx.z + c.delta[0].z);
       recs += __popc(c.mask);
       pairs += c.record ? 1 : 0;
     }
+  }
+  if (qg < Q.nv) {
+    self_acc[qg] = make_float4(dx.x, dx.y, dx.z, (float)recs);
+    recs = 0;
   }
   cnt[qg] = recs;
   if (pair_num) pair_num[qg] = pairs < Q.cap ? pairs : Q.cap;
@@ -217,7 +228,7 @@ __global__ void k_contact_emit(QuerySets Q, const int32_t* __restrict__ fdy, con
                                float k, const int32_t* __restrict__ off, int rec_cap, uint32_t* __restrict__ rkey,
                                int32_t* __restrict__ ridx, float4* __restrict__ rval, int32_t* status) {
   const int nqt = 2 * Q.nv + Q.nvs;
-  int qg = blockIdx.x * blockDim.x + threadIdx.x;
+  int qg = Q.nv + blockIdx.x * blockDim.x + threadIdx.x;     // the first set is summed in pass 1
   if (qg >= nqt) return;
   int set, qv, ncand, nq;
   const int32_t* cand;
@@ -260,17 +271,20 @@ __global__ void k_seg_bounds(int rec_cap, const int32_t* __restrict__ total, con
 // update_dx over the contact records: y += f * (dx / nc) (XPBD.py:383-387)
 __global__ void k_contact_apply(int nv, const int32_t* __restrict__ seg_lo, const int32_t* __restrict__ seg_hi,
                                 const int32_t* __restrict__ ridx, const float4* __restrict__ rval,
-                                const float* __restrict__ fixed, float4* __restrict__ y4) {
+                                const float4* __restrict__ self_acc, const float* __restrict__ fixed,
+                                float4* __restrict__ y4) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
+  const float4 sa = self_acc[v];            // first-set records, already summed in order
   int lo = seg_lo[v], hi = seg_hi[v];
-  if (hi <= lo) return;
-  float3 dx = make_float3(0.f, 0.f, 0.f);
+  if (hi < lo) hi = lo;
+  const float nc = sa.w + (float)(hi - lo);
+  if (!(nc > 0.f)) return;
+  float3 dx = make_float3(sa.x, sa.y, sa.z);
   for (int i = lo; i < hi; ++i) {
     float4 r = rval[ridx[i]];
     dx = make_float3(dx.x + r.x, dx.y + r.y, dx.z + r.z);
   }
-  float nc = (float)(hi - lo);
   float f = fixed[v];
   float4 y = y4[v];
   y4[v] = make_float4(y.x + f * (dx.x / nc), y.y + f * (dx.y / nc), y.z + f * (dx.z / nc), y.w);
@@ -291,6 +305,7 @@ struct CollisionWs {
   int32_t *cnt, *off, *total, *seg_lo, *seg_hi, *ridx;
   uint32_t* rkey;
   float4* rval;
+  float4* self_acc;
   void* scan_ws;
   size_t scan_bytes;
   void* sort_ws;
@@ -314,6 +329,7 @@ static size_t layout(int nv, int nvs, int rec_cap, char* base, CollisionWs* w) {
   t.rkey = (uint32_t*)take(4 * rc);
   t.ridx = (int32_t*)take(4 * rc);
   t.rval = (float4*)take(16 * rc);
+  t.self_acc = (float4*)take(16 * (size_t)nv);
   t.scan_bytes = scan_ws_bytes((int)nq);
   t.scan_ws = take(t.scan_bytes);
   t.sort_bytes = radix_sort_ws_bytes((int)rc);
@@ -350,10 +366,10 @@ extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, cons
   QuerySets Q{nv, c_tv_st ? nvs : 0, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
   const int nqt = 2 * nv + Q.nvs;
   SLX_LAUNCH("contact_count", k_contact_count, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w.cnt, pair_num);
+             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w.cnt, pair_num, w.self_acc);
   int rc = exclusive_scan(nqt, nullptr, w.cnt, w.off, w.total, w.scan_ws, w.scan_bytes, st);
   if (rc) return rc;
-  SLX_LAUNCH("contact_emit", k_contact_emit, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
+  SLX_LAUNCH("contact_emit", k_contact_emit, dim3(blocks_for(nqt - nv, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
              (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w.off, rec_cap, w.rkey, w.ridx,
              w.rval, status);
   SLX_LAUNCH("clamp_total", k_clamp_total, dim3(1), dim3(64), 0, st, w.total, rec_cap);
@@ -366,6 +382,6 @@ extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, cons
     SLX_LAUNCH("seg_bounds", k_seg_bounds, dim3(blocks_for(rec_cap, 256)), dim3(256), 0, st, rec_cap, w.total,
                w.rkey, w.seg_lo, w.seg_hi);
   SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.seg_lo, w.seg_hi,
-             w.ridx, w.rval, fixed, (float4*)y4);
+             w.ridx, w.rval, w.self_acc, fixed, (float4*)y4);
   return check_launch("slx_collision_pass");
 }
