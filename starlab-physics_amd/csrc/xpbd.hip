@@ -109,35 +109,52 @@ __device__ __forceinline__ void spring_vertex(int v, const float4* __restrict__ 
   const float fwv = xv.w;
   float3 dx = make_float3(0.f, 0.f, 0.f);
   float nc = 0.f;
-  for (int k = 0; k < width; ++k) {
-    int2 e = ent[base + 64 * k];
-    if (e.x == -1) break;
-    const int u = e.x & 0x7fffffff;
-    const float l0 = __int_as_float(e.y);
-    float3 pv, pu;
-    float fwu, kk;
-    if (e.x < 0) {                 // bending: acts on x (XPBD.py:169)
-      float4 xu = x4[u];
-      pv = xyz(xv);
-      pu = xyz(xu);
-      fwu = xu.w;
-      kk = k_b;
-    } else {                       // stretch: acts on y (XPBD.py:150)
-      float4 pu4;
-      if constexpr (PRED) {
-        pu4 = x4[u];
-        pu = predict(pu4, v4[u], dt, gdt2);
-      } else {
-        pu4 = y_in[u];
-        pu = xyz(pu4);
+  // Entries are consumed in groups of 4: all entry loads, then all neighbour
+  // gathers, then the arithmetic in entry order, so four independent gathers are
+  // in flight per lane.  Rows are left-aligned: an entry of -1 ends the row.
+  constexpr int G = 4;
+  for (int k0 = 0; k0 < width; k0 += G) {
+    int2 e[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) e[q] = (k0 + q < width) ? ent[base + 64 * (k0 + q)] : make_int2(-1, 0);
+    float4 pa[G], pb[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int u = e[q].x == -1 ? v : (e[q].x & 0x7fffffff);
+      if (e[q].x < 0) {            // bending (or padding): x of the neighbour (XPBD.py:169)
+        pa[q] = x4[u];
+      } else if constexpr (PRED) { // stretch on the predicted y, from x and v
+        pa[q] = x4[u];
+        pb[q] = v4[u];
+      } else {                     // stretch on y (XPBD.py:150)
+        pa[q] = y_in[u];
       }
-      pv = yv;
-      fwu = pu4.w;
-      kk = k_s;
     }
-    float3 t = constraint_term(pv, pu, fwv, fwu, l0, kk);
-    dx = sub3(dx, t);
-    nc = nc + 1.0f;
+    bool done = false;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      if (e[q].x == -1) {
+        done = true;
+        break;
+      }
+      const float l0 = __int_as_float(e[q].y);
+      float3 pv, pu;
+      float kk;
+      if (e[q].x < 0) {
+        pv = xyz(xv);
+        pu = xyz(pa[q]);
+        kk = k_b;
+      } else {
+        if constexpr (PRED) pu = predict(pa[q], pb[q], dt, gdt2);
+        else pu = xyz(pa[q]);
+        pv = yv;
+        kk = k_s;
+      }
+      float3 t = constraint_term(pv, pu, fwv, pa[q].w, l0, kk);
+      dx = sub3(dx, t);
+      nc = nc + 1.0f;
+    }
+    if (done) break;
   }
   const float f = fixed[v];
   float3 yn = jacobi_apply(yv, dx, nc, f);
