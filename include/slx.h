@@ -44,6 +44,7 @@ extern "C" {
 #define SLX_STATUS_CAND_OVERFLOW 1   /* a candidate list exceeded its capacity */
 #define SLX_STATUS_STACK_OVERFLOW 2  /* a traversal stack exceeded its depth */
 #define SLX_STATUS_CELL_RANGE 4      /* a face centre fell outside the 32^3 cell grid */
+#define SLX_STATUS_REC_OVERFLOW 8    /* a vertex received more than rec_cap contact deltas (dropped) */
 
 typedef void* slx_stream_t; /* hipStream_t */
 
@@ -135,8 +136,23 @@ int slx_mesh_aabb(int nv, const float* x4, float padding, float* box6, void* ws,
  * [32767], parent [65535]. */
 size_t slx_cell_topology_ws_bytes(void);
 int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int32_t* child_b,
-                      int32_t* parent, int32_t* range_l, int32_t* range_r, void* ws,
+                      int32_t* parent, int32_t* range_l, int32_t* range_r, int32_t* sched, void* ws,
                       size_t ws_bytes, slx_stream_t stream);
+
+/* Refit schedule of a fixed topology over n_leaves leaves (sched holds
+ * slx_refit_sched_ints(n_leaves) int32; the topology entry points above and below
+ * write it).  The reference refits with one thread per leaf and global arrival
+ * counters (lbvh_cell.py:490-522); the schedule instead cuts the tree into subtrees
+ * of at most sub_leaves leaves, each refit by one workgroup in LDS, and the nodes
+ * above them, refit by one workgroup, so no device-wide fence is needed.  The
+ * boxes are the same.  slx_set_refit_shape sets sub_leaves (1..2048, default 2048)
+ * and how many top nodes keep their state in LDS (0..2048, default 2048) for
+ * schedules made afterwards. */
+size_t slx_refit_sched_ints(int n_leaves);
+int slx_refit_schedule(int n_leaves, const int32_t* child_b, const int32_t* parent,
+                       const int32_t* range_l, const int32_t* range_r, int32_t* sched,
+                       slx_stream_t stream);
+int slx_set_refit_shape(int sub_leaves, int top_lds);
 
 /* LBVH_CELL.build, lbvh_cell.py:637-653: faces bucketed by cell (stable, ascending
  * face id inside a cell), inclusive cell prefix, leaf slot i = row-major cell i
@@ -146,7 +162,7 @@ int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int
 size_t slx_cell_build_ws_bytes(int nf);
 int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
                    const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
-                   int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces, int32_t* num,
+                   const int32_t* sched, int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces, int32_t* num,
                    int32_t* leaf_l, int32_t* leaf_r, float* nlo4, float* nhi4, int32_t* status,
                    void* ws, size_t ws_bytes, slx_stream_t stream);
 
@@ -158,12 +174,16 @@ int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* 
 size_t slx_tri_build_ws_bytes(int nf);
 int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
                   int32_t* codes, int32_t* sorted_faces, int32_t* child_a, int32_t* child_b,
-                  int32_t* parent, float* nlo4, float* nhi4, void* ws, size_t ws_bytes,
-                  slx_stream_t stream);
-/* Refit of the triangle LBVH over new face boxes (topology kept). ws >= 4*nf bytes. */
-int slx_tri_refit(int nf, const float* fmin4, const float* fmax4, const int32_t* sorted_faces,
-                  const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
+                  int32_t* parent, int32_t* range_l, int32_t* range_r, int32_t* sched,
                   float* nlo4, float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream);
+/* Refit of the triangle LBVH (topology and schedule kept): leaf k gets the padded
+ * box of face sorted_faces[k] computed from x4 (the face boxes of
+ * computeAABB_faces, meshTaichiWrapper.py:350-361, fused in; fmin4/fmax4 are not
+ * rewritten), internal nodes the union of their children. */
+int slx_tri_refit(int nf, const int32_t* faces, const float* x4, float padding,
+                  const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b,
+                  const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4,
+                  slx_stream_t stream);
 
 /* traverse_cell_bvh_single, lbvh_cell.py:655-714, for nq query points p -+ padding
  * (p = xyz of q4).  mode 0 = cell tree (queries whose centre cell id exceeds 32767
@@ -187,8 +207,8 @@ int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float*
                   const int32_t* vert_base, const int32_t* face_base, const int32_t* face_inst,
                   float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
                   int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
-                  int32_t* range_l, int32_t* range_r, int32_t* roots, float* nlo4, float* nhi4,
-                  void* ws, size_t ws_bytes, slx_stream_t stream);
+                  int32_t* range_l, int32_t* range_r, int32_t* roots, int32_t* sched, float* nlo4,
+                  float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream);
 /* Traversal of a batched tree: query q starts at roots[q_inst[q]]. */
 int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots,
                      float padding, int nf, const float* nlo4, const float* nhi4, int cap,
@@ -201,7 +221,8 @@ int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32
  * candidate sets (any may be null) in the reference's loop order; deltas are
  * accumulated per vertex in that serial order without float atomics; y4 updated
  * in place.  pair_num [2nv+nvs] (nullable) receives the recorded pair counts.
- * rec_cap bounds the contact records of one pass (overflow -> status bit). */
+ * rec_cap = contact deltas kept per dynamic vertex and pass from the second and
+ * third sets (more -> dropped, status bit); (nv+nvs)*cap*4 must stay below 2^32. */
 size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap);
 int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st,
                        float* y4, const float* xs4, const float* fixed, const float* m_inv,
@@ -291,9 +312,10 @@ typedef struct slx_tree {
   const int32_t* face_inst;  /* [nf] */
   const int32_t* vert_inst;  /* [nv] instance of each vertex of this tree's mesh (its queries) */
   int32_t* sorted_inst;      /* [nf] */
-  int32_t* range_l;          /* [nf-1] */
+  int32_t* range_l;          /* triangle: leaf range of each internal node [nf-1] */
   int32_t* range_r;
   int32_t* roots;            /* [n_inst] */
+  int32_t* sched;            /* refit schedule, slx_refit_sched_ints(leaves) (cell: of the static topology) */
 } slx_tree;
 
 typedef struct slx_frame_desc {
@@ -320,7 +342,7 @@ typedef struct slx_frame_desc {
   int32_t collision;                     /* enable_collision_handling */
   int32_t refit_every_substep;           /* triangle tree: refit from x at each substep */
   int32_t rebuild_static;                /* rebuild the static tree this frame */
-  int32_t cap, rec_cap;
+  int32_t cap, rec_cap;                  /* candidates per query; contact deltas per vertex */
   int32_t method;                        /* 0 = Jacobi, 1 = graph-coloured Gauss-Seidel */
   const int32_t* gs_edges;               /* GS: int4 (v0, v1, l0 bits, 0) per stretch edge, colour order */
   const int32_t* gs_color_off;           /* GS: HOST array [n_colors+1] of colour starts */

@@ -40,13 +40,17 @@ SIGNATURES = {
     "slx_mesh_aabb_ws_bytes": [],
     "slx_mesh_aabb": [I32, PTR, F32, PTR, PTR, C.c_size_t, STREAM],
     "slx_cell_topology_ws_bytes": [],
-    "slx_cell_topology": [PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_cell_topology": [PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_refit_sched_ints": [I32],
+    "slx_refit_schedule": [I32, PTR, PTR, PTR, PTR, PTR, STREAM],
+    "slx_set_refit_shape": [I32, I32],
     "slx_cell_build_ws_bytes": [I32],
     "slx_cell_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
-                       PTR, C.c_size_t, STREAM],
+                       PTR, PTR, C.c_size_t, STREAM],
     "slx_tri_build_ws_bytes": [I32],
-    "slx_tri_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
-    "slx_tri_refit": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_tri_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t,
+                      STREAM],
+    "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
     "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
@@ -55,7 +59,7 @@ SIGNATURES = {
     "slx_frame": [PTR, STREAM],
     "slx_seg_build_ws_bytes": [I32, I32],
     "slx_seg_build": [I32, I32, I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
-                      PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+                      PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
     "slx_traverse_seg": [I32, PTR, PTR, PTR, F32, I32, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_mlp_num_params": [],
     "slx_features": [I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, PTR, STREAM],
@@ -71,7 +75,8 @@ _RET = {"slx_last_error": C.c_char_p, "slx_scan_ws_bytes": C.c_size_t,
         "slx_radix_sort_ws_bytes": C.c_size_t, "slx_mesh_aabb_ws_bytes": C.c_size_t,
         "slx_cell_topology_ws_bytes": C.c_size_t, "slx_cell_build_ws_bytes": C.c_size_t,
         "slx_tri_build_ws_bytes": C.c_size_t, "slx_collision_ws_bytes": C.c_size_t,
-        "slx_frame_ws_bytes": C.c_size_t, "slx_seg_build_ws_bytes": C.c_size_t}
+        "slx_frame_ws_bytes": C.c_size_t, "slx_seg_build_ws_bytes": C.c_size_t,
+        "slx_refit_sched_ints": C.c_size_t}
 
 
 class Tree(C.Structure):
@@ -81,7 +86,7 @@ class Tree(C.Structure):
                 ("parent", PTR), ("num", PTR), ("face_cell", PTR), ("prefix", PTR), ("leaf_l", PTR),
                 ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR), ("n_inst", I32), ("vert_base", PTR),
                 ("face_base", PTR), ("face_inst", PTR), ("vert_inst", PTR), ("sorted_inst", PTR),
-                ("range_l", PTR), ("range_r", PTR), ("roots", PTR)]
+                ("range_l", PTR), ("range_r", PTR), ("roots", PTR), ("sched", PTR)]
 
 
 class FrameDesc(C.Structure):

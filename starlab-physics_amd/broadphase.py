@@ -24,11 +24,12 @@ def cell_topology(device):
         t = dict(codes=torch.empty(NCELLS, **i32), sorted_ids=torch.empty(NCELLS, **i32),
                  child_a=torch.empty(NCELLS - 1, **i32), child_b=torch.empty(NCELLS - 1, **i32),
                  parent=torch.empty(2 * NCELLS - 1, **i32), range_l=torch.empty(NCELLS - 1, **i32),
-                 range_r=torch.empty(NCELLS - 1, **i32))
+                 range_r=torch.empty(NCELLS - 1, **i32),
+                 sched=torch.empty(_abi.lib().slx_refit_sched_ints(NCELLS), **i32))
         ws = torch.empty(_abi.lib().slx_cell_topology_ws_bytes(), dtype=torch.uint8, device=device)
         _abi.call("slx_cell_topology", *[_abi.ptr(t[k]) for k in
                                          ("codes", "sorted_ids", "child_a", "child_b", "parent", "range_l",
-                                          "range_r")], _abi.ptr(ws), ws.numel(), _abi.stream_handle())
+                                          "range_r", "sched")], _abi.ptr(ws), ws.numel(), _abi.stream_handle())
         _TOPO[key] = t
     return _TOPO[key]
 
@@ -52,6 +53,7 @@ class DeviceTree:
         if mode == "cell":
             topo = cell_topology(dev)
             self.child_a, self.child_b, self.parent = topo["child_a"], topo["child_b"], topo["parent"]
+            self.range_l, self.range_r, self.sched = topo["range_l"], topo["range_r"], topo["sched"]
             nn = 2 * NCELLS - 1
             self.num = torch.zeros(nn, **i32)
             self.face_cell = torch.zeros(max(nf, 1), **i32)
@@ -67,6 +69,9 @@ class DeviceTree:
             self.child_b = torch.zeros(max(nf - 1, 1), **i32)
             self.parent = torch.zeros(nn, **i32)
             self.codes = torch.zeros(max(nf, 1), **i32)
+            self.range_l = torch.zeros(max(nf - 1, 1), **i32)
+            self.range_r = torch.zeros(max(nf - 1, 1), **i32)
+            self.sched = torch.zeros(_abi.lib().slx_refit_sched_ints(max(nf, 1)), **i32)
             self.num = self.face_cell = self.prefix = self.leaf_l = self.leaf_r = None
             self.n_leaves = nf
             ws = _abi.lib().slx_tri_build_ws_bytes(nf)
@@ -82,8 +87,6 @@ class DeviceTree:
             self.n_inst = ni
             self.box6 = torch.zeros(6 * ni + 2, **f32)
             self.sorted_inst = torch.zeros(max(nf, 1), **i32)
-            self.range_l = torch.zeros(max(nf - 1, 1), **i32)
-            self.range_r = torch.zeros(max(nf - 1, 1), **i32)
             self.roots = torch.zeros(ni, **i32)
             ws = max(ws, _abi.lib().slx_seg_build_ws_bytes(nf, ni))
         ws = max(ws, _abi.lib().slx_mesh_aabb_ws_bytes())
@@ -96,13 +99,13 @@ class DeviceTree:
                       P(self.codes), P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent),
                       P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
                       P(self.nlo4), P(self.nhi4))
+        t.range_l, t.range_r, t.sched = P(self.range_l), P(self.range_r), P(self.sched)
         if self.n_inst > 1:
             I = self.inst
             t.n_inst = self.n_inst
             t.vert_base, t.face_base = P(I["vert_base"]), P(I["face_base"])
             t.face_inst, t.vert_inst = P(I["face_inst"]), P(I["vert_inst"])
-            t.sorted_inst, t.range_l, t.range_r, t.roots = (P(self.sorted_inst), P(self.range_l), P(self.range_r),
-                                                            P(self.roots))
+            t.sorted_inst, t.roots = P(self.sorted_inst), P(self.roots)
         return t
 
     # --- stages (the frame driver calls the same entry points itself) ---
@@ -127,27 +130,29 @@ class DeviceTree:
                       float(padding), P(I["vert_base"]), P(I["face_base"]), P(I["face_inst"]), P(self.fmin4),
                       P(self.fmax4), P(self.box6), P(self.codes), P(self.sorted_faces), P(self.sorted_inst),
                       P(self.child_a), P(self.child_b), P(self.parent), P(self.range_l), P(self.range_r),
-                      P(self.roots), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
+                      P(self.roots), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
             return
         self.face_aabbs(padding, x4)
         self.mesh_aabb(padding, x4)
         if self.mode == "cell":
             _abi.call("slx_cell_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.child_a),
-                      P(self.child_b), P(self.parent), P(self.face_cell), P(self.prefix), P(self.sorted_faces),
+                      P(self.child_b), P(self.parent), P(self.sched), P(self.face_cell), P(self.prefix), P(self.sorted_faces),
                       P(self.num), P(self.leaf_l), P(self.leaf_r), P(self.nlo4), P(self.nhi4), P(self.status),
                       P(self.ws), self.ws.numel(), s)
         else:
             _abi.call("slx_tri_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.codes),
-                      P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent), P(self.nlo4),
-                      P(self.nhi4), P(self.ws), self.ws.numel(), s)
+                      P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent), P(self.range_l),
+                      P(self.range_r), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
 
     def refit(self, padding, x4=None):
-        """Triangle tree: new face boxes, same topology."""
-        assert self.mode == "tri"
+        """Triangle tree: node boxes from new positions, same topology (the face
+        boxes fmin4/fmax4 keep the values of the last build)."""
+        if self.mode != "tri":
+            raise ValueError("refit needs the triangle LBVH")
         P = _abi.ptr
-        self.face_aabbs(padding, x4)
-        _abi.call("slx_tri_refit", self.nf, P(self.fmin4), P(self.fmax4), P(self.sorted_faces), P(self.child_a),
-                  P(self.child_b), P(self.parent), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(),
+        x4 = self.mesh.x4 if x4 is None else x4
+        _abi.call("slx_tri_refit", self.nf, P(self.faces), P(x4), float(padding), P(self.sorted_faces),
+                  P(self.child_a), P(self.child_b), P(self.parent), P(self.sched), P(self.nlo4), P(self.nhi4),
                   _abi.stream_handle())
 
     def query(self, q4, padding, cap, cand=None, cand_n=None, status=None, q_inst=None):

@@ -40,16 +40,16 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
   if (batched(t))
     return slx_seg_build(nv, t.nf, t.n_inst, faces, x4, pad, t.vert_base, t.face_base, t.face_inst, t.fmin4, t.fmax4,
                          t.box6, t.codes, t.sorted_faces, t.sorted_inst, t.child_a, t.child_b, t.parent, t.range_l,
-                         t.range_r, t.roots, t.nlo4, t.nhi4, ws, ws_bytes, s);
+                         t.range_r, t.roots, t.sched, t.nlo4, t.nhi4, ws, ws_bytes, s);
   int rc = slx_face_aabbs(t.nf, faces, x4, pad, t.fmin4, t.fmax4, s);
   if (rc) return rc;
   rc = slx_mesh_aabb(nv, x4, pad, t.box6, ws, ws_bytes, s);
   if (rc) return rc;
   if (t.mode == 0)
-    return slx_cell_build(t.nf, t.fmin4, t.fmax4, t.box6, t.child_a, t.child_b, t.parent, t.face_cell, t.prefix,
-                          t.sorted_faces, t.num, t.leaf_l, t.leaf_r, t.nlo4, t.nhi4, status, ws, ws_bytes, s);
+    return slx_cell_build(t.nf, t.fmin4, t.fmax4, t.box6, t.child_a, t.child_b, t.parent, t.sched, t.face_cell,
+                          t.prefix, t.sorted_faces, t.num, t.leaf_l, t.leaf_r, t.nlo4, t.nhi4, status, ws, ws_bytes, s);
   return slx_tri_build(t.nf, t.fmin4, t.fmax4, t.box6, t.codes, t.sorted_faces, t.child_a, t.child_b, t.parent,
-                       t.nlo4, t.nhi4, ws, ws_bytes, s);
+                       t.range_l, t.range_r, t.sched, t.nlo4, t.nhi4, ws, ws_bytes, s);
 }
 
 // queries of a mesh whose vertices belong to instances q_inst (batched trees only)
@@ -129,10 +129,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
   for (int s = 0; s < d->n_substeps; ++s) {
     if (coll && s > 0 && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0) {
-      rc = slx_face_aabbs(d->dyn.nf, d->faces, d->x4, pad, d->dyn.fmin4, d->dyn.fmax4, stream);
-      if (rc) return rc;
-      rc = slx_tri_refit(d->dyn.nf, d->dyn.fmin4, d->dyn.fmax4, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
-                         d->dyn.parent, d->dyn.nlo4, d->dyn.nhi4, d->ws, d->ws_bytes, stream);
+      rc = slx_tri_refit(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
+                         d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, stream);
       if (rc) return rc;
       tv_fresh = false;
     }

@@ -30,17 +30,24 @@ constexpr int kTravBS = 128;     // traversal block size
 
 // ----------------------------------------------------------------- AABBs
 
-// computeAABB_faces, meshTaichiWrapper.py:350-361
+// computeAABB_faces, meshTaichiWrapper.py:350-361: componentwise min/max of the
+// three corners, minus/plus the padding
+__device__ __forceinline__ void face_box(const int32_t* __restrict__ faces, const float4* __restrict__ x4, int f,
+                                         float pad, float3& lo, float3& hi) {
+  float3 a = xyz(x4[faces[3 * f]]), b = xyz(x4[faces[3 * f + 1]]), c = xyz(x4[faces[3 * f + 2]]);
+  const float p = pad * 1.0f;
+  lo = make_float3(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p, fminf(fminf(a.z, b.z), c.z) - p);
+  hi = make_float3(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p, fmaxf(fmaxf(a.z, b.z), c.z) + p);
+}
+
 __global__ void k_face_aabbs(int nf, const int32_t* __restrict__ faces, const float4* __restrict__ x4,
                              float pad, float4* __restrict__ fmin, float4* __restrict__ fmax) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  float3 a = xyz(x4[faces[3 * f]]), b = xyz(x4[faces[3 * f + 1]]), c = xyz(x4[faces[3 * f + 2]]);
-  const float p = pad * 1.0f;
-  fmin[f] = make_float4(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p,
-                        fminf(fminf(a.z, b.z), c.z) - p, __int_as_float(f));
-  fmax[f] = make_float4(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p,
-                        fmaxf(fmaxf(a.z, b.z), c.z) + p, __int_as_float(f));
+  float3 lo, hi;
+  face_box(faces, x4, f, pad, lo, hi);
+  fmin[f] = make_float4(lo.x, lo.y, lo.z, __int_as_float(f));
+  fmax[f] = make_float4(hi.x, hi.y, hi.z, __int_as_float(f));
 }
 
 // computeAABB, meshTaichiWrapper.py:334-347: min/max reduction (order-independent,
@@ -201,91 +208,330 @@ __global__ void k_fill_i32(int n, int32_t* __restrict__ p, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-// lbvh_cell.py:499-515: union of non-empty children; a node whose children are
-// both empty gets the degenerate box 0.25*(min0+max0+min1+max1)
-__device__ __forceinline__ void refit_internal(int i, int nl, const int32_t* ca, const int32_t* cb,
-                                               int32_t* num, float4* nlo, float4* nhi) {
-  int l = ca[i], r = cb[i];
-  float4 a0 = nlo[l], a1 = nhi[l], b0 = nlo[r], b1 = nhi[r];
-  int nlv = num ? num[l] : 1, nrv = num ? num[r] : 1;
-  float3 mn, mx;
-  if (nlv > 0 && nrv > 0) {
-    mn = make_float3(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z));
-    mx = make_float3(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z));
-  } else if (nlv > 0) {
-    mn = xyz(a0); mx = xyz(a1);
-  } else if (nrv > 0) {
-    mn = xyz(b0); mx = xyz(b1);
+// Bottom-up refit without device-wide synchronisation.
+//
+// The reference refits with one thread per leaf walking up and atomic arrival
+// counters in global memory (lbvh_cell.py:490-522).  On a GPU whose L2 is split
+// over several dies every step of such a walk needs a device-scope release and
+// acquire, i.e. an L2 write-back and invalidate.  Here a schedule, made once per
+// topology, cuts the tree into maximal subtrees of at most T leaves (a leaf range
+// of an LBVH node is contiguous, and every internal node's index lies inside its
+// own range), each refit by one workgroup in LDS with workgroup-scope arrival
+// counters, and the top part (nodes over more than T leaves), refit by a single
+// workgroup whose per-node state also lives in LDS.  The stream orders the two
+// kernels; no device-scope fence is executed.  Every node is still the union of its
+// two children in the same operand order as the serial refit, so the boxes are
+// bit-identical.
+//
+// Schedule (int32), for a tree over n leaves:
+//   [0] n_sub  [1] n_top  [2] T  [3] top nodes with LDS state  [4..7] unused
+//   sub[n]      int4 {subtree root, first leaf, last leaf, up}
+//   top_slot[n] internal node -> top slot (-1 below the top part)
+//   top_node[n] top slot -> node
+//   top_up[n]   top slot -> up link
+//   gflag[n]    arrival counters of top slots without LDS state (left at zero)
+// An up link is (top slot of the parent) << 1 | (1 if the node is the parent's
+// second child), or -1 for the root.
+constexpr int kSubMax = 2048;   // leaves of one subtree (LDS of k_refit_sub)
+constexpr int kSubBS = 1024;
+constexpr int kSubGrid = 2048;
+constexpr int kTopMax = 2048;   // top nodes with LDS state (LDS of k_refit_top)
+constexpr int kTopBS = 1024;
+int g_refit_sub = kSubMax;
+int g_refit_top = kTopMax;
+
+struct Sched {
+  const int32_t* p;
+  int n;
+  __device__ int n_sub() const { return p[0]; }
+  __device__ int n_top() const { return p[1]; }
+  __device__ int top_lds() const { return p[3]; }
+  __device__ int4 sub(int i) const { return reinterpret_cast<const int4*>(p + 8)[i]; }
+  __device__ int top_slot(int v) const { return p[8 + 4 * (size_t)n + v]; }
+  __device__ int top_node(int s) const { return p[8 + 5 * (size_t)n + s]; }
+  __device__ int top_up(int s) const { return p[8 + 6 * (size_t)n + s]; }
+  __device__ int32_t* gflag(int s) const { return const_cast<int32_t*>(p) + 8 + 7 * (size_t)n + s; }
+};
+
+__global__ void k_sched_header(int32_t* sched, int T, int top_lds) {
+  if (threadIdx.x == 0) {
+    sched[0] = 0;
+    sched[1] = 0;
+    sched[2] = T;
+    sched[3] = top_lds;
+  }
+}
+
+__device__ __forceinline__ int node_leaves(int j, int n, const int32_t* rl, const int32_t* rr) {
+  return j < n - 1 ? rr[j] - rl[j] + 1 : 1;
+}
+
+// classify every node: top node (internal, more than T leaves) or subtree root
+// (at most T leaves under a top node or at the root)
+__global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const int32_t* __restrict__ rl,
+                             const int32_t* __restrict__ rr, int32_t* __restrict__ sched) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= 2 * n - 1) return;
+  const int T = sched[2];
+  const size_t N = (size_t)n;
+  const int sz = node_leaves(j, n, rl, rr);
+  if (j < n - 1) {
+    int slot = -1;
+    if (sz > T) {
+      slot = atomicAdd(&sched[1], 1);
+      sched[8 + 5 * N + slot] = j;
+    }
+    sched[8 + 4 * N + j] = slot;
+    sched[8 + 7 * N + j] = 0;
+  }
+  const int p = parent[j];
+  if (sz <= T && (p < 0 || node_leaves(p, n, rl, rr) > T)) {
+    const int e = atomicAdd(&sched[0], 1);
+    const int first = j < n - 1 ? rl[j] : j - (n - 1);
+    reinterpret_cast<int4*>(sched + 8)[e] = make_int4(j, first, first + sz - 1, p);
+  }
+}
+
+__device__ __forceinline__ int up_link(int j, int p, const int32_t* cb, const int32_t* sched, size_t N) {
+  return p < 0 ? -1 : (sched[8 + 4 * N + p] << 1) | (cb[p] == j ? 1 : 0);
+}
+
+// up links of subtree roots and top nodes (needs every top slot assigned)
+__global__ void k_sched_link(int n, const int32_t* __restrict__ cb, const int32_t* __restrict__ parent,
+                             int32_t* __restrict__ sched) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = (size_t)n;
+  if (i < sched[0]) {
+    int4* e = reinterpret_cast<int4*>(sched + 8) + i;
+    e->w = up_link(e->x, e->w, cb, sched, N);
+  }
+  if (i < sched[1]) {
+    const int j = sched[8 + 5 * N + i];
+    sched[8 + 6 * N + i] = up_link(j, parent[j], cb, sched, N);
+  }
+}
+
+struct NodeBox {
+  float3 lo, hi;
+  int num;
+};
+
+// lbvh_cell.py:499-515: union of the non-empty children; a node whose children
+// are both empty gets the degenerate box 0.25*(min0+max0+min1+max1).  Triangle
+// trees have no empty nodes (COUNTED = false).
+template <bool COUNTED>
+__device__ __forceinline__ NodeBox unite(const NodeBox& a, const NodeBox& b) {
+  NodeBox o;
+  const int na = COUNTED ? a.num : 1, nb = COUNTED ? b.num : 1;
+  if (na > 0 && nb > 0) {
+    o.lo = make_float3(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z));
+    o.hi = make_float3(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z));
+  } else if (na > 0) {
+    o.lo = a.lo;
+    o.hi = a.hi;
+  } else if (nb > 0) {
+    o.lo = b.lo;
+    o.hi = b.hi;
   } else {
-    mn = make_float3(0.25f * (((a0.x + a1.x) + b0.x) + b1.x), 0.25f * (((a0.y + a1.y) + b0.y) + b1.y),
-                     0.25f * (((a0.z + a1.z) + b0.z) + b1.z));
-    mx = mn;
+    o.lo = make_float3(0.25f * (((a.lo.x + a.hi.x) + b.lo.x) + b.hi.x), 0.25f * (((a.lo.y + a.hi.y) + b.lo.y) + b.hi.y),
+                       0.25f * (((a.lo.z + a.hi.z) + b.lo.z) + b.hi.z));
+    o.hi = o.lo;
   }
-  if (num) num[i] = nlv + nrv;
-  nlo[i] = make_float4(mn.x, mn.y, mn.z, __int_as_float(l));
-  nhi[i] = make_float4(mx.x, mx.y, mx.z, __int_as_float(r));
+  o.num = na + nb;
+  return o;
 }
 
-// Walk up from leaf `node`; the second child to arrive at a parent refits it.
-__device__ void refit_up(int node, int nl, const int32_t* ca, const int32_t* cb, const int32_t* parent,
-                         int32_t* num, float4* nlo, float4* nhi, int32_t* flags) {
-  __threadfence();
-  int p = parent[node];
-  while (p >= 0) {
-    int prev = atomicAdd(&flags[p], 1);
-    if (prev == 0) return;           // sibling subtree not finished: it will continue
-    __threadfence();                 // see the sibling's writes
-    refit_internal(p, nl, ca, cb, num, nlo, nhi);
-    __threadfence();
-    p = parent[p];
+template <bool COUNTED>
+__device__ __forceinline__ NodeBox load_node(int j, const int32_t* num, const float4* nlo, const float4* nhi) {
+  NodeBox b;
+  b.lo = xyz(nlo[j]);
+  b.hi = xyz(nhi[j]);
+  b.num = COUNTED ? num[j] : 1;
+  return b;
+}
+
+// node boxes keep the child ids (internal) or the face / face range (leaf) in .w
+template <bool COUNTED>
+__device__ __forceinline__ void store_node(int j, const NodeBox& b, int wl, int wr, int32_t* num, float4* nlo,
+                                           float4* nhi) {
+  nlo[j] = make_float4(b.lo.x, b.lo.y, b.lo.z, __int_as_float(wl));
+  nhi[j] = make_float4(b.hi.x, b.hi.y, b.hi.z, __int_as_float(wr));
+  if (COUNTED) num[j] = b.num;
+}
+
+// where leaf boxes come from
+enum LeafKind { kLeafFaceBoxes = 0, kLeafVertices = 1, kLeafCells = 2 };
+struct LeafSrc {
+  const float4* fmin;       // face boxes (kLeafFaceBoxes, kLeafCells)
+  const float4* fmax;
+  const int32_t* faces;     // kLeafVertices: boxes from the corners
+  const float4* x4;
+  float pad;
+  const int32_t* sorted;    // triangle: leaf k holds face sorted[k]; cell: faces sorted by cell
+  const float* box;         // kLeafCells
+  const int32_t* prefix;
+  int32_t* leaf_l;
+  int32_t* leaf_r;
+};
+
+// Leaf k: its box, written to the node arrays as well.  Cell leaves follow
+// assign_leaf_cell_nodes (lbvh_cell.py:601-634): leaf slot k holds row-major cell
+// k, its face range from the inclusive prefix, box = cell centre united with its
+// faces' boxes.
+template <int LEAF>
+__device__ __forceinline__ NodeBox leaf_box(int k, int n, const LeafSrc& s, int32_t* num, float4* nlo, float4* nhi) {
+  NodeBox b;
+  const int node = n - 1 + k;
+  if constexpr (LEAF == kLeafCells) {
+    const float3 lo = make_float3(s.box[0], s.box[1], s.box[2]);
+    const float3 cs = cell_size_of(s.box);
+    const int x0 = k / (kGrid * kGrid), y0 = (k % (kGrid * kGrid)) / kGrid, z0 = k % kGrid;
+    float3 mn = make_float3(((cs.x * (float)x0) + (0.5f * cs.x)) + lo.x, ((cs.y * (float)y0) + (0.5f * cs.y)) + lo.y,
+                            ((cs.z * (float)z0) + (0.5f * cs.z)) + lo.z);
+    float3 mx = mn;
+    const int l = k ? s.prefix[k - 1] : 0, r = s.prefix[k] - 1;
+    for (int j = l; j <= r; ++j) {
+      const int f = s.sorted[j];
+      const float4 a = s.fmin[f], c = s.fmax[f];
+      mn = make_float3(fminf(mn.x, a.x), fminf(mn.y, a.y), fminf(mn.z, a.z));
+      mx = make_float3(fmaxf(mx.x, c.x), fmaxf(mx.y, c.y), fmaxf(mx.z, c.z));
+    }
+    s.leaf_l[k] = l;
+    s.leaf_r[k] = r;
+    b.lo = mn;
+    b.hi = mx;
+    b.num = r - l + 1;
+    store_node<true>(node, b, l, r, num, nlo, nhi);
+  } else {
+    const int f = s.sorted[k];
+    if constexpr (LEAF == kLeafVertices) {
+      face_box(s.faces, s.x4, f, s.pad, b.lo, b.hi);
+    } else {
+      b.lo = xyz(s.fmin[f]);
+      b.hi = xyz(s.fmax[f]);
+    }
+    b.num = 1;
+    store_node<false>(node, b, f, f, num, nlo, nhi);
+  }
+  return b;
+}
+
+// One workgroup per subtree: leaves into LDS, then every thread walks up from its
+// leaves; the second child to arrive at a node refits it.
+template <int LEAF>
+__global__ void __launch_bounds__(kSubBS)
+k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t* __restrict__ ca,
+            const int32_t* __restrict__ cb, const int32_t* __restrict__ parent, int32_t* num, float4* nlo,
+            float4* nhi) {
+  constexpr bool COUNTED = LEAF == kLeafCells;
+  __shared__ NodeBox s_leaf[kSubMax];
+  __shared__ NodeBox s_int[kSubMax];
+  __shared__ int s_flag[kSubMax];
+  const Sched S{sched, n};
+  const int n_sub = S.n_sub();
+  for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
+    const int4 e = S.sub(si);
+    const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
+    for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+      s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
+      s_flag[k] = 0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+      int node = n - 1 + first + k;
+      while (node != root) {
+        const int p = parent[node];          // inside the subtree: first <= p <= last
+        __threadfence_block();
+        if (atomicAdd(&s_flag[p - first], 1) == 0) break;
+        __threadfence_block();
+        const int l = ca[p], r = cb[p];
+        const NodeBox A = l >= n - 1 ? s_leaf[l - (n - 1) - first] : s_int[l - first];
+        const NodeBox B = r >= n - 1 ? s_leaf[r - (n - 1) - first] : s_int[r - first];
+        const NodeBox P = unite<COUNTED>(A, B);
+        s_int[p - first] = P;
+        store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
+        node = p;
+      }
+    }
+    __syncthreads();
   }
 }
 
-// Triangle leaves: leaf k holds face sorted[k]
-__global__ void k_tri_refit(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
-                            const int32_t* __restrict__ sorted, const int32_t* __restrict__ ca,
-                            const int32_t* __restrict__ cb, const int32_t* __restrict__ parent,
-                            float4* nlo, float4* nhi, int32_t* flags) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nf) return;
-  int f = sorted[k], node = k + nf - 1;
-  float4 a = fmin[f], b = fmax[f];
-  nlo[node] = make_float4(a.x, a.y, a.z, __int_as_float(f));
-  nhi[node] = make_float4(b.x, b.y, b.z, __int_as_float(f));
-  refit_up(node, nf, ca, cb, parent, nullptr, nlo, nhi, flags);
+// The top part in one workgroup.  Each top slot with LDS state has two child
+// cells; a finished node puts its box into its parent's cell, then arrives at the
+// parent.  Slots beyond the LDS capacity read their children from the node arrays
+// and count arrivals in global memory; one workgroup runs, so workgroup scope
+// orders both.
+template <bool COUNTED>
+__global__ void __launch_bounds__(kTopBS)
+k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
+            const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi) {
+  __shared__ NodeBox s_cell[kTopMax][2];
+  __shared__ int s_flag[kTopMax];
+  __shared__ int s_up[kTopMax];
+  const Sched S{sched, n};
+  const int n_sub = S.n_sub(), n_top = S.n_top();
+  const int nl = n_top < S.top_lds() ? n_top : S.top_lds();
+  for (int i = threadIdx.x; i < nl; i += kTopBS) {
+    s_flag[i] = 0;
+    s_up[i] = S.top_up(i);
+  }
+  for (int i = threadIdx.x; i < n_sub; i += kTopBS) {
+    const int4 e = S.sub(i);
+    if (e.w >= 0 && (e.w >> 1) < nl) s_cell[e.w >> 1][e.w & 1] = load_node<COUNTED>(e.x, num, nlo, nhi);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_sub; i += kTopBS) {
+    int up = S.sub(i).w;
+    while (up >= 0) {
+      const int slot = up >> 1;
+      const bool lds = slot < nl;
+      int prev;
+      if (lds) {
+        __threadfence_block();
+        prev = atomicAdd(&s_flag[slot], 1);
+      } else {
+        prev = __hip_atomic_fetch_add(S.gflag(slot), 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (prev == 0) break;
+      const int p = S.top_node(slot);
+      const int l = ca[p], r = cb[p];
+      NodeBox A, B;
+      if (lds) {
+        __threadfence_block();
+        A = s_cell[slot][0];
+        B = s_cell[slot][1];
+      } else {
+        __hip_atomic_store(S.gflag(slot), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        A = load_node<COUNTED>(l, num, nlo, nhi);
+        B = load_node<COUNTED>(r, num, nlo, nhi);
+      }
+      const NodeBox P = unite<COUNTED>(A, B);
+      store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
+      up = lds ? s_up[slot] : S.top_up(slot);
+      if (up >= 0 && (up >> 1) < nl) s_cell[up >> 1][up & 1] = P;
+    }
+  }
 }
 
-// Cell leaves (assign_leaf_cell_nodes, lbvh_cell.py:601-634): leaf slot i holds
-// row-major cell i: range from the inclusive prefix, box = cell centre united
-// with its faces' boxes.
-__global__ void k_cell_leaves(int nf, const float4* __restrict__ fmin, const float4* __restrict__ fmax,
-                              const float* __restrict__ box, const int32_t* __restrict__ prefix,
-                              const int32_t* __restrict__ sorted, const int32_t* __restrict__ ca,
-                              const int32_t* __restrict__ cb, const int32_t* __restrict__ parent,
-                              int32_t* leaf_l, int32_t* leaf_r, int32_t* num, float4* nlo, float4* nhi,
-                              int32_t* flags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= kCells) return;
-  const float3 lo = make_float3(box[0], box[1], box[2]);
-  const float3 cs = cell_size_of(box);
-  int x0 = i / (kGrid * kGrid), y0 = (i % (kGrid * kGrid)) / kGrid, z0 = i % kGrid;
-  float3 mn = make_float3(((cs.x * (float)x0) + (0.5f * cs.x)) + lo.x, ((cs.y * (float)y0) + (0.5f * cs.y)) + lo.y,
-                          ((cs.z * (float)z0) + (0.5f * cs.z)) + lo.z);
-  float3 mx = mn;
-  int l = i ? prefix[i - 1] : 0, r = prefix[i] - 1;
-  for (int j = l; j <= r; ++j) {
-    int f = sorted[j];
-    float4 a = fmin[f], b = fmax[f];
-    mn = make_float3(fminf(mn.x, a.x), fminf(mn.y, a.y), fminf(mn.z, a.z));
-    mx = make_float3(fmaxf(mx.x, b.x), fmaxf(mx.y, b.y), fmaxf(mx.z, b.z));
-  }
-  int node = i + kCells - 1;
-  leaf_l[i] = l;
-  leaf_r[i] = r;
-  num[node] = r - l + 1;
-  nlo[node] = make_float4(mn.x, mn.y, mn.z, __int_as_float(l));
-  nhi[node] = make_float4(mx.x, mx.y, mx.z, __int_as_float(r));
-  refit_up(node, kCells, ca, cb, parent, num, nlo, nhi, flags);
+size_t refit_sched_ints(int n) { return 8 + 8 * (size_t)(n > 0 ? n : 1); }
+
+int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_t* rl, const int32_t* rr,
+                   int32_t* sched, hipStream_t st) {
+  SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, g_refit_sub, g_refit_top);
+  SLX_LAUNCH("sched", k_sched_mark, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, st, n, parent, rl, rr, sched);
+  SLX_LAUNCH("sched", k_sched_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cb, parent, sched);
+  return check_launch("refit_schedule");
+}
+
+template <int LEAF>
+int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, const int32_t* ca, const int32_t* cb,
+          const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st) {
+  constexpr bool COUNTED = LEAF == kLeafCells;
+  const int grid = n < kSubGrid ? n : kSubGrid;
+  SLX_LAUNCH(name, k_refit_sub<LEAF>, dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb, parent, num, nlo, nhi);
+  SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi);
+  return check_launch(name);
 }
 
 // face -> cell id (assign_face_cell_ids, lbvh_cell.py:576-586) + histogram
@@ -422,10 +668,29 @@ extern "C" int slx_mesh_aabb(int nv, const float* x4, float padding, float* box6
 
 extern "C" size_t slx_cell_topology_ws_bytes(void) { return radix_sort_ws_bytes(kCells); }
 
+extern "C" size_t slx_refit_sched_ints(int n_leaves) { return refit_sched_ints(n_leaves); }
+
+extern "C" int slx_set_refit_shape(int sub_leaves, int top_lds) {
+  SLX_REQUIRE(sub_leaves >= 1 && sub_leaves <= kSubMax, "slx_set_refit_shape: sub_leaves outside [1, 2048]");
+  SLX_REQUIRE(top_lds >= 0 && top_lds <= kTopMax, "slx_set_refit_shape: top_lds outside [0, 2048]");
+  g_refit_sub = sub_leaves;
+  g_refit_top = top_lds;
+  return SLX_OK;
+}
+
+extern "C" int slx_refit_schedule(int n_leaves, const int32_t* child_b, const int32_t* parent,
+                                  const int32_t* range_l, const int32_t* range_r, int32_t* sched,
+                                  slx_stream_t stream) {
+  SLX_REQUIRE(n_leaves >= 1, "slx_refit_schedule: n_leaves < 1");
+  SLX_REQUIRE(child_b && parent && range_l && range_r && sched, "slx_refit_schedule: null pointer");
+  return refit_schedule(n_leaves, child_b, parent, range_l, range_r, sched, S(stream));
+}
+
 extern "C" int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int32_t* child_b,
-                                 int32_t* parent, int32_t* range_l, int32_t* range_r, void* ws, size_t ws_bytes,
-                                 slx_stream_t stream) {
-  SLX_REQUIRE(codes && sorted_ids && child_a && child_b && parent && ws, "slx_cell_topology: null pointer");
+                                 int32_t* parent, int32_t* range_l, int32_t* range_r, int32_t* sched, void* ws,
+                                 size_t ws_bytes, slx_stream_t stream) {
+  SLX_REQUIRE(codes && sorted_ids && child_a && child_b && parent && range_l && range_r && sched && ws,
+              "slx_cell_topology: null pointer");
   hipStream_t st = S(stream);
   SLX_LAUNCH("cell_morton", k_cell_morton, dim3(blocks_for(kCells, 256)), dim3(256), 0, st,
              make_float3(0.f, 0.f, 0.f), make_float3(1e3f, 1e3f, 1e3f), (uint32_t*)codes, sorted_ids);
@@ -433,7 +698,9 @@ extern "C" int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* c
   if (rc) return rc;
   SLX_LAUNCH("karras", k_karras, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells,
              (const uint32_t*)codes, 0, child_a, child_b, parent, range_l, range_r);
-  return check_launch("slx_cell_topology");
+  rc = check_launch("slx_cell_topology");
+  if (rc) return rc;
+  return refit_schedule(kCells, child_b, parent, range_l, range_r, sched, st);
 }
 
 extern "C" size_t slx_cell_build_ws_bytes(int nf) {
@@ -443,11 +710,11 @@ extern "C" size_t slx_cell_build_ws_bytes(int nf) {
 
 extern "C" int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
                               const int32_t* child_a, const int32_t* child_b, const int32_t* parent,
-                              int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces, int32_t* num,
-                              int32_t* leaf_l, int32_t* leaf_r, float* nlo4, float* nhi4, int32_t* status,
-                              void* ws, size_t ws_bytes, slx_stream_t stream) {
+                              const int32_t* sched, int32_t* face_cell, int32_t* prefix, int32_t* sorted_faces,
+                              int32_t* num, int32_t* leaf_l, int32_t* leaf_r, float* nlo4, float* nhi4,
+                              int32_t* status, void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nf >= 0, "slx_cell_build: nf < 0");
-  SLX_REQUIRE(box6 && child_a && child_b && parent && face_cell && prefix && sorted_faces && num && leaf_l &&
+  SLX_REQUIRE(box6 && child_a && child_b && parent && sched && face_cell && prefix && sorted_faces && num && leaf_l &&
                   leaf_r && nlo4 && nhi4 && ws,
               "slx_cell_build: null pointer");
   SLX_REQUIRE(nf == 0 || (fmin4 && fmax4), "slx_cell_build: null face boxes");
@@ -457,11 +724,10 @@ extern "C" int slx_cell_build(int nf, const float* fmin4, const float* fmax4, co
   size_t n = (size_t)(nf > 0 ? nf : 1);
   uint32_t* keys = (uint32_t*)p; p += 4 * n;
   int32_t* hist = (int32_t*)p; p += 4 * (size_t)kCells;
-  int32_t* flags = (int32_t*)p; p += 4 * (size_t)kCells;
+  p += 4 * (size_t)kCells;
   void* sort_ws = p; size_t sort_bytes = radix_sort_ws_bytes(nf); p += sort_bytes;
   void* scan_ws = p; size_t scan_bytes = scan_ws_bytes(kCells);
   SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, hist, 0);
-  SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, flags, 0);
   if (nf > 0) {
     SLX_LAUNCH("face_cells", k_face_cells, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf,
                (const float4*)fmin4, (const float4*)fmax4, box6, keys, sorted_faces, face_cell, hist, status);
@@ -472,10 +738,16 @@ extern "C" int slx_cell_build(int nf, const float* fmin4, const float* fmax4, co
   int rc = exclusive_scan(kCells, nullptr, hist, prefix, nullptr, scan_ws, scan_bytes, st);
   if (rc) return rc;
   SLX_LAUNCH("add", k_add_in_place, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, kCells, hist, prefix);
-  SLX_LAUNCH("cell_leaves", k_cell_leaves, dim3(blocks_for(kCells, 256)), dim3(256), 0, st, nf,
-             (const float4*)fmin4, (const float4*)fmax4, box6, prefix, sorted_faces, child_a, child_b, parent,
-             leaf_l, leaf_r, num, (float4*)nlo4, (float4*)nhi4, flags);
-  return check_launch("slx_cell_build");
+  LeafSrc src{};
+  src.fmin = (const float4*)fmin4;
+  src.fmax = (const float4*)fmax4;
+  src.sorted = sorted_faces;
+  src.box = box6;
+  src.prefix = prefix;
+  src.leaf_l = leaf_l;
+  src.leaf_r = leaf_r;
+  return refit<kLeafCells>("cell_refit", kCells, sched, src, child_a, child_b, parent, num, (float4*)nlo4,
+                           (float4*)nhi4, st);
 }
 
 extern "C" size_t slx_tri_build_ws_bytes(int nf) {
@@ -483,29 +755,31 @@ extern "C" size_t slx_tri_build_ws_bytes(int nf) {
   return 2 * n * 4 + radix_sort_ws_bytes(nf) + 1024;
 }
 
-extern "C" int slx_tri_refit(int nf, const float* fmin4, const float* fmax4, const int32_t* sorted_faces,
-                             const int32_t* child_a, const int32_t* child_b, const int32_t* parent, float* nlo4,
-                             float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
+extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, float padding,
+                             const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b,
+                             const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4,
+                             slx_stream_t stream) {
   SLX_REQUIRE(nf >= 0, "slx_tri_refit: nf < 0");
   if (nf == 0) return SLX_OK;
-  SLX_REQUIRE(fmin4 && fmax4 && sorted_faces && child_a && child_b && parent && nlo4 && nhi4 && ws,
+  SLX_REQUIRE(faces && x4 && sorted_faces && child_a && child_b && parent && sched && nlo4 && nhi4,
               "slx_tri_refit: null pointer");
-  if (ws_bytes < 4 * (size_t)nf) return fail(SLX_EWS, "slx_tri_refit: workspace too small");
-  hipStream_t st = S(stream);
-  int32_t* flags = (int32_t*)ws;
-  SLX_LAUNCH("fill", k_fill_i32, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, flags, 0);
-  SLX_LAUNCH("tri_refit", k_tri_refit, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const float4*)fmin4,
-             (const float4*)fmax4, sorted_faces, child_a, child_b, parent, (float4*)nlo4, (float4*)nhi4, flags);
-  return check_launch("slx_tri_refit");
+  LeafSrc src{};
+  src.faces = faces;
+  src.x4 = (const float4*)x4;
+  src.pad = padding;
+  src.sorted = sorted_faces;
+  return refit<kLeafVertices>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
+                              (float4*)nhi4, S(stream));
 }
 
 extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6, int32_t* codes,
                              int32_t* sorted_faces, int32_t* child_a, int32_t* child_b, int32_t* parent,
-                             float* nlo4, float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
+                             int32_t* range_l, int32_t* range_r, int32_t* sched, float* nlo4, float* nhi4, void* ws,
+                             size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nf >= 0, "slx_tri_build: nf < 0");
   if (nf == 0) return SLX_OK;
-  SLX_REQUIRE(fmin4 && fmax4 && box6 && codes && sorted_faces && child_a && child_b && parent && nlo4 && nhi4 &&
-                  ws,
+  SLX_REQUIRE(fmin4 && fmax4 && box6 && codes && sorted_faces && child_a && child_b && parent && range_l && range_r &&
+                  sched && nlo4 && nhi4 && ws,
               "slx_tri_build: null pointer");
   if (ws_bytes < slx_tri_build_ws_bytes(nf)) return fail(SLX_EWS, "slx_tri_build: workspace too small");
   hipStream_t st = S(stream);
@@ -515,10 +789,17 @@ extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, con
                             ws_bytes - 8 * (size_t)nf - 512, st);
   if (rc) return rc;
   SLX_LAUNCH("karras", k_karras, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, (const uint32_t*)codes, 1,
-             child_a, child_b, parent, (int32_t*)nullptr, (int32_t*)nullptr);
+             child_a, child_b, parent, range_l, range_r);
   rc = check_launch("k_karras");
   if (rc) return rc;
-  return slx_tri_refit(nf, fmin4, fmax4, sorted_faces, child_a, child_b, parent, nlo4, nhi4, ws, ws_bytes, stream);
+  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st);
+  if (rc) return rc;
+  LeafSrc src{};
+  src.fmin = (const float4*)fmin4;
+  src.fmax = (const float4*)fmax4;
+  src.sorted = sorted_faces;
+  return refit<kLeafFaceBoxes>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
+                               (float4*)nhi4, st);
 }
 
 namespace slx {
@@ -702,13 +983,14 @@ extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, c
                              const int32_t* vert_base, const int32_t* face_base, const int32_t* face_inst,
                              float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
                              int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
-                             int32_t* range_l, int32_t* range_r, int32_t* roots, float* nlo4, float* nhi4, void* ws,
-                             size_t ws_bytes, slx_stream_t stream) {
+                             int32_t* range_l, int32_t* range_r, int32_t* roots, int32_t* sched, float* nlo4,
+                             float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nv >= 0 && nf >= 0 && n_inst >= 1 && n_inst <= (1 << kSegInstBits), "slx_seg_build: bad sizes");
   SLX_REQUIRE(nf < (1 << kSegIndexBits), "slx_seg_build: too many faces for 24-bit indices");
   if (nf == 0) return SLX_OK;
   SLX_REQUIRE(faces && x4 && vert_base && face_base && face_inst && fmin4 && fmax4 && box6 && codes && sorted_faces &&
-                  sorted_inst && child_a && child_b && parent && range_l && range_r && roots && nlo4 && nhi4 && ws,
+                  sorted_inst && child_a && child_b && parent && range_l && range_r && roots && sched && nlo4 &&
+                  nhi4 && ws,
               "slx_seg_build: null pointer");
   if (ws_bytes < slx_seg_build_ws_bytes(nf, n_inst)) return fail(SLX_EWS, "slx_seg_build: workspace too small");
   hipStream_t st = (hipStream_t)stream;
@@ -739,7 +1021,14 @@ extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, c
                range_r, sorted_inst, face_base, roots);
   rc = check_launch("slx_seg_build");
   if (rc) return rc;
-  return slx_tri_refit(nf, fmin4, fmax4, sorted_faces, child_a, child_b, parent, nlo4, nhi4, ws, ws_bytes, stream);
+  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st);
+  if (rc) return rc;
+  LeafSrc src{};
+  src.fmin = (const float4*)fmin4;
+  src.fmax = (const float4*)fmax4;
+  src.sorted = sorted_faces;
+  return refit<kLeafFaceBoxes>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
+                               (float4*)nhi4, st);
 }
 
 extern "C" int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots, float padding,

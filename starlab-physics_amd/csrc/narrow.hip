@@ -9,21 +9,13 @@
 // The reference scatters each contact's deltas with float atomics.  Here every
 // query (the three sets in the reference's loop order: dynamic vertices vs
 // static triangles, dynamic vertices vs dynamic triangles, static vertices vs
-// dynamic triangles) first counts its contact records, an exclusive scan turns
-// the counts into record offsets in that serial order, the records
-// (target vertex, delta) are written, stably sorted by target vertex, and each
-// vertex sums its records in order.  The result equals a serial scatter in the
-// reference's loop order, bit for bit, with no float atomics.
+// dynamic triangles) computes its contacts; each delta becomes a record in its
+// target vertex's bucket tagged with its position in that serial order, and each
+// vertex adds its records in that order.  The result equals a serial scatter in
+// the reference's loop order, bit for bit, with no float atomics.
 #include "slx_common.h"
 
 namespace slx {
-
-size_t radix_sort_ws_bytes(int n_max);
-int radix_sort_pairs(int n_max, const int* n_dev, uint32_t* keys, int32_t* vals, int key_bits,
-                     void* ws, size_t ws_bytes, hipStream_t st);
-size_t scan_ws_bytes(int n_max);
-int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* total_dev, void* ws,
-                   size_t ws_bytes, hipStream_t st);
 
 // ---- distance.py restated (same op order as the CPU oracle) ----
 
@@ -54,23 +46,25 @@ __device__ int d_type_pt(float3 p, float3 t0, float3 t1, float3 t2) {
   return 6;
 }
 
-// squared distance of the region's primitive pair and its gradient; g[0..3] =
+// squared distance of the region's primitive pair and its gradient g0..g3 for
 // (p, t0, t1, t2), zero for points outside the pair
-__device__ float dist_grad(int dtype, float3 p, float3 t0, float3 t1, float3 t2, float3* g) {
+__device__ __forceinline__ float dist_grad(int dtype, float3 p, float3 t0, float3 t1, float3 t2, float3& g0,
+                                           float3& g1, float3& g2, float3& g3) {
   const float3 z = make_float3(0.f, 0.f, 0.f);
-  g[0] = g[1] = g[2] = g[3] = z;
+  g0 = g1 = g2 = g3 = z;
   if (dtype <= 2) {                      // point-point: d = |p-a|^2, dd/dp = 2(p-a)
     float3 a = dtype == 0 ? t0 : (dtype == 1 ? t1 : t2);
     float3 d = sub3(p, a);
     float3 gp = make_float3(2.0f * d.x, 2.0f * d.y, 2.0f * d.z);
-    g[0] = gp;
-    g[1 + dtype] = make_float3(-gp.x, -gp.y, -gp.z);
+    float3 ga = make_float3(-gp.x, -gp.y, -gp.z);
+    g0 = gp;
+    if (dtype == 0) g1 = ga;
+    else if (dtype == 1) g2 = ga;
+    else g3 = ga;
     return dot3(d, d);
   }
-  if (dtype <= 5) {                      // point-edge (a, b)
-    int ia = dtype == 3 ? 1 : (dtype == 4 ? 2 : 1);
-    int ib = dtype == 3 ? 2 : 3;
-    float3 a = ia == 1 ? t0 : t1, b = ib == 2 ? t1 : t2;
+  if (dtype <= 5) {                      // point-edge (a, b): 3 = (t0,t1), 4 = (t1,t2), 5 = (t0,t2)
+    float3 a = dtype == 4 ? t1 : t0, b = dtype == 3 ? t1 : t2;
     float3 u = sub3(a, p), w = sub3(b, p), c = cross3(u, w), e = sub3(b, a);
     float L = dot3(e, e);
     float d = dot3(c, c) / L;
@@ -79,9 +73,10 @@ __device__ float dist_grad(int dtype, float3 p, float3 t0, float3 t1, float3 t2,
     float q = twoL * d;
     float3 ga = make_float3(twoL * wc.x + q * e.x, twoL * wc.y + q * e.y, twoL * wc.z + q * e.z);
     float3 gb = make_float3(twoL * cu.x - q * e.x, twoL * cu.y - q * e.y, twoL * cu.z - q * e.z);
-    g[ia] = ga;
-    g[ib] = gb;
-    g[0] = make_float3(-(ga.x + gb.x), -(ga.y + gb.y), -(ga.z + gb.z));
+    if (dtype == 3) { g1 = ga; g2 = gb; }
+    else if (dtype == 4) { g2 = ga; g3 = gb; }
+    else { g1 = ga; g3 = gb; }
+    g0 = make_float3(-(ga.x + gb.x), -(ga.y + gb.y), -(ga.z + gb.z));
     return d;
   }
   // point-plane: d = ((p-a).n)^2/|n|^2
@@ -94,43 +89,42 @@ __device__ float dist_grad(int dtype, float3 p, float3 t0, float3 t1, float3 t2,
   float3 gp = make_float3(q2 * n.x, q2 * n.y, q2 * n.z);
   float3 gb = make_float3(q2 * e2r.x - qq2 * e2n.x, q2 * e2r.y - qq2 * e2n.y, q2 * e2r.z - qq2 * e2n.z);
   float3 gc = make_float3(q2 * re1.x - qq2 * ne1.x, q2 * re1.y - qq2 * ne1.y, q2 * re1.z - qq2 * ne1.z);
-  g[0] = gp;
-  g[2] = gb;
-  g[3] = gc;
-  g[1] = make_float3(-((gp.x + gb.x) + gc.x), -((gp.y + gb.y) + gc.y), -((gp.z + gb.z) + gc.z));
+  g0 = gp;
+  g2 = gb;
+  g3 = gc;
+  g1 = make_float3(-((gp.x + gb.x) + gc.x), -((gp.y + gb.y) + gc.y), -((gp.z + gb.z) + gc.z));
   return d;
 }
 
 __device__ __constant__ unsigned char c_slots[7] = {0x3, 0x5, 0x9, 0x7, 0xD, 0xB, 0xF};
 
+// One candidate's deltas for slots (p, t0, t1, t2); every loop over the four slots
+// is unrolled, so the arrays stay in registers.
 struct Contact {
   int vid[4];
   float3 delta[4];
-  unsigned mask;      // slots that receive a record
+  unsigned mask;      // slots that receive a delta
   bool record;        // pair recorded for the velocity pass
 };
 
-// One candidate of query set `set` (0 vt_st, 1 vt_dy, 2 tv_st); fills the
-// records in slot order (collision_constraints_x.py:5-310).
-__device__ void contact(int set, int qv, int f, const int32_t* fdy, const int32_t* fst, const float4* y4,
-                        const float4* xs4, const float* fixed, const float* m_inv, float dhat, float k,
-                        Contact& c) {
+// One candidate of query set `set` (0 vt_st, 1 vt_dy, 2 tv_st), slots in order
+// (collision_constraints_x.py:5-310).
+__device__ __forceinline__ void contact(int set, int qv, int f, const int32_t* fdy, const int32_t* fst,
+                                        const float4* y4, const float4* xs4, const float* fixed,
+                                        const float* m_inv, float dhat, float k, Contact& c) {
   c.mask = 0;
   c.record = false;
-  float3 P[4];
-  if (set == 0) {
-    c.vid[0] = qv; c.vid[1] = fst[3 * f]; c.vid[2] = fst[3 * f + 1]; c.vid[3] = fst[3 * f + 2];
-    P[0] = xyz(y4[qv]); P[1] = xyz(xs4[c.vid[1]]); P[2] = xyz(xs4[c.vid[2]]); P[3] = xyz(xs4[c.vid[3]]);
-  } else if (set == 1) {
-    c.vid[0] = qv; c.vid[1] = fdy[3 * f]; c.vid[2] = fdy[3 * f + 1]; c.vid[3] = fdy[3 * f + 2];
-    P[0] = xyz(y4[qv]); P[1] = xyz(y4[c.vid[1]]); P[2] = xyz(y4[c.vid[2]]); P[3] = xyz(y4[c.vid[3]]);
-  } else {
-    c.vid[0] = qv; c.vid[1] = fdy[3 * f]; c.vid[2] = fdy[3 * f + 1]; c.vid[3] = fdy[3 * f + 2];
-    P[0] = xyz(xs4[qv]); P[1] = xyz(y4[c.vid[1]]); P[2] = xyz(y4[c.vid[2]]); P[3] = xyz(y4[c.vid[3]]);
-  }
-  const int dtype = d_type_pt(P[0], P[1], P[2], P[3]);
+  const int32_t* F = set == 0 ? fst : fdy;
+  c.vid[0] = qv;
+  c.vid[1] = F[3 * f];
+  c.vid[2] = F[3 * f + 1];
+  c.vid[3] = F[3 * f + 2];
+  const float4* tp = set == 0 ? xs4 : y4;
+  const float3 p = xyz(set == 2 ? xs4[qv] : y4[qv]);
+  const float3 t0 = xyz(tp[c.vid[1]]), t1 = xyz(tp[c.vid[2]]), t2 = xyz(tp[c.vid[3]]);
+  const int dtype = d_type_pt(p, t0, t1, t2);
   float3 g[4];
-  const float d = dist_grad(dtype, P[0], P[1], P[2], P[3], g);
+  const float d = dist_grad(dtype, p, t0, t1, t2, g[0], g[1], g[2], g[3]);
   const unsigned m = c_slots[dtype];
   if (set == 0) {                                        // __vt_st
     if (d < dhat) {
@@ -148,8 +142,9 @@ __device__ void contact(int set, int qv, int f, const int32_t* fdy, const int32_
   const int s0 = (set == 2) ? 1 : 0;                     // static point does not move
   float schur = 0.f;
   bool first = true;
-  for (int s = s0; s < 4; ++s) {
-    if (!(m & (1u << s))) continue;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s < s0 || !(m & (1u << s))) continue;
     int v = c.vid[s];
     float term = (fixed[v] * m_inv[v]) * dot3(g[s], g[s]);
     schur = first ? term : schur + term;
@@ -162,8 +157,9 @@ __device__ void contact(int set, int qv, int f, const int32_t* fdy, const int32_
     schur = schur + 1e-4f;
     ld = (dhat - d) / schur;
   }
-  for (int s = s0; s < 4; ++s) {
-    if (!(m & (1u << s))) continue;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (s < s0 || !(m & (1u << s))) continue;
     int v = c.vid[s];
     float a = (set == 1 && dtype == 0) ? m_inv[v] * ld : (fixed[v] * m_inv[v]) * ld;
     c.delta[s] = mul3(a, g[s]);
@@ -185,155 +181,116 @@ __device__ __forceinline__ bool query_of(const QuerySets& Q, int qg, int& set, i
   return cand != nullptr;
 }
 
-// Pass 1: number of contact records per query (and recorded pair counts).
-// Records of the first set (dynamic vertex vs static triangle) only ever move
-// the query vertex itself and come first in its serial order, so that thread
-// sums them in candidate order right here (self_acc = (sum, count)); only the
-// other two sets go through the emit / sort path.
-__global__ void k_contact_count(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
-                                const float4* __restrict__ y4, const float4* __restrict__ xs4,
-                                const float* __restrict__ fixed, const float* __restrict__ m_inv, float dhat,
-                                float k, int32_t* __restrict__ cnt, int32_t* __restrict__ pair_num,
-                                float4* __restrict__ self_acc) {
+// Every record (target vertex, delta) carries its position in the reference's
+// serial loop order, o = ((query - nv) * cap + candidate) * 4 + slot over the
+// second and third query sets (the first set only ever moves its own query
+// vertex and comes first, so its thread sums those deltas itself).  A record
+// claims a slot of its target vertex's bucket with an integer atomic; the vertex
+// then adds its records in increasing o.  The sum is therefore the serial
+// scatter's, bit for bit, whatever order the slots were claimed in.
+__global__ void __launch_bounds__(256)
+k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
+           const float4* __restrict__ y4, const float4* __restrict__ xs4, const float* __restrict__ fixed,
+           const float* __restrict__ m_inv, float dhat, float k, int bucket, int32_t* __restrict__ bcnt,
+           float4* __restrict__ brec, float4* __restrict__ self_acc, int32_t* __restrict__ pair_num,
+           int32_t* status) {
   const int nqt = 2 * Q.nv + Q.nvs;
-  int qg = blockIdx.x * blockDim.x + threadIdx.x;
+  const int qg = blockIdx.x * blockDim.x + threadIdx.x;
   if (qg >= nqt) return;
   int set, qv, ncand, nq;
   const int32_t* cand;
-  int recs = 0, pairs = 0;
+  int pairs = 0, own = 0;
+  bool over = false;
   float3 dx = make_float3(0.f, 0.f, 0.f);
   if (query_of(Q, qg, set, qv, cand, ncand, nq)) {
     for (int j = 0; j < ncand; ++j) {
-      int f = cand[(size_t)j * nq + qv];
+      const int f = cand[(size_t)j * nq + qv];
       if (set == 1 && (fdy[3 * f] == qv || fdy[3 * f + 1] == qv || fdy[3 * f + 2] == qv)) continue;
       Contact c;
       contact(set, qv, f, fdy, fst, y4, xs4, fixed, m_inv, dhat, k, c);
-      if (set == 0 && c.mask) dx = make_float3(dx.x + c.delta[0].x, dx.y + c.delta[0].y, dx.z + c.delta[0].z);
-      recs += __popc(c.mask);
       pairs += c.record ? 1 : 0;
-    }
-  }
-  if (qg < Q.nv) {
-    self_acc[qg] = make_float4(dx.x, dx.y, dx.z, (float)recs);
-    recs = 0;
-  }
-  cnt[qg] = recs;
-  if (pair_num) pair_num[qg] = pairs < Q.cap ? pairs : Q.cap;
-}
-
-// Pass 2: write the records at their serial-order offsets.
-__global__ void k_contact_emit(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
-                               const float4* __restrict__ y4, const float4* __restrict__ xs4,
-                               const float* __restrict__ fixed, const float* __restrict__ m_inv, float dhat,
-                               float k, const int32_t* __restrict__ off, int rec_cap, uint32_t* __restrict__ rkey,
-                               int32_t* __restrict__ ridx, float4* __restrict__ rval, int32_t* status) {
-  const int nqt = 2 * Q.nv + Q.nvs;
-  int qg = Q.nv + blockIdx.x * blockDim.x + threadIdx.x;     // the first set is summed in pass 1
-  if (qg >= nqt) return;
-  int set, qv, ncand, nq;
-  const int32_t* cand;
-  if (!query_of(Q, qg, set, qv, cand, ncand, nq)) return;
-  int o = off[qg];
-  for (int j = 0; j < ncand; ++j) {
-    int f = cand[(size_t)j * nq + qv];
-    if (set == 1 && (fdy[3 * f] == qv || fdy[3 * f + 1] == qv || fdy[3 * f + 2] == qv)) continue;
-    Contact c;
-    contact(set, qv, f, fdy, fst, y4, xs4, fixed, m_inv, dhat, k, c);
-    for (int s = 0; s < 4; ++s) {
-      if (!(c.mask & (1u << s))) continue;
-      if (o < rec_cap) {
-        rkey[o] = (uint32_t)c.vid[s];
-        ridx[o] = o;
-        rval[o] = make_float4(c.delta[s].x, c.delta[s].y, c.delta[s].z, 0.f);
-      } else if (status) {
-        atomicOr(status, kStatusCandOverflow);
+      if (set == 0) {
+        if (c.mask) {
+          dx = make_float3(dx.x + c.delta[0].x, dx.y + c.delta[0].y, dx.z + c.delta[0].z);
+          ++own;
+        }
+        continue;
       }
-      ++o;
+      const uint32_t o0 = ((uint32_t)(qg - Q.nv) * (uint32_t)Q.cap + (uint32_t)j) * 4u;
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) {
+        if (!(c.mask & (1u << sl))) continue;
+        const int t = c.vid[sl];
+        const int at = atomicAdd(&bcnt[t], 1);
+        if (at < bucket)
+          brec[(size_t)t * bucket + at] = make_float4(c.delta[sl].x, c.delta[sl].y, c.delta[sl].z,
+                                                      __uint_as_float(o0 + (uint32_t)sl));
+        else
+          over = true;
+      }
     }
   }
+  if (qg < Q.nv) self_acc[qg] = make_float4(dx.x, dx.y, dx.z, (float)own);
+  if (pair_num) pair_num[qg] = pairs < Q.cap ? pairs : Q.cap;
+  if (over && status) atomicOr(status, kStatusRecOverflow);
 }
 
-__global__ void k_clamp_total(int32_t* total, int rec_cap) {
-  if (threadIdx.x == 0 && *total > rec_cap) *total = rec_cap;
-}
-
-// segment bounds of each vertex in the sorted record keys
-__global__ void k_seg_bounds(int rec_cap, const int32_t* __restrict__ total, const uint32_t* __restrict__ key,
-                             int32_t* __restrict__ seg_lo, int32_t* __restrict__ seg_hi) {
-  const int n = *total;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k = key[i];
-  if (i == 0 || key[i - 1] != k) seg_lo[k] = i;
-  if (i == n - 1 || key[i + 1] != k) seg_hi[k] = i + 1;
-}
-
-// update_dx over the contact records: y += f * (dx / nc) (XPBD.py:383-387)
-__global__ void k_contact_apply(int nv, const int32_t* __restrict__ seg_lo, const int32_t* __restrict__ seg_hi,
-                                const int32_t* __restrict__ ridx, const float4* __restrict__ rval,
-                                const float4* __restrict__ self_acc, const float* __restrict__ fixed,
-                                float4* __restrict__ y4) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
+// update_dx over the contacts: y += f * (dx / nc) (XPBD.py:383-387); the bucket
+// is summed in increasing serial order.
+__global__ void __launch_bounds__(256)
+k_contact_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __restrict__ brec,
+                const float4* __restrict__ self_acc, const float* __restrict__ fixed, float4* __restrict__ y4) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
-  const float4 sa = self_acc[v];            // first-set records, already summed in order
-  int lo = seg_lo[v], hi = seg_hi[v];
-  if (hi < lo) hi = lo;
-  const float nc = sa.w + (float)(hi - lo);
+  const float4 sa = self_acc[v];            // first-set deltas, already summed in order
+  int n = bcnt[v];
+  if (n > bucket) n = bucket;
+  const float nc = sa.w + (float)n;
   if (!(nc > 0.f)) return;
   float3 dx = make_float3(sa.x, sa.y, sa.z);
-  for (int i = lo; i < hi; ++i) {
-    float4 r = rval[ridx[i]];
-    dx = make_float3(dx.x + r.x, dx.y + r.y, dx.z + r.z);
+  const float4* r = brec + (size_t)v * bucket;
+  uint32_t last = 0;
+  for (int i = 0; i < n; ++i) {             // selection by increasing order key
+    uint32_t best = 0xffffffffu;
+    int bj = 0;
+    for (int j = 0; j < n; ++j) {
+      const uint32_t o = __float_as_uint(r[j].w);
+      if ((i == 0 || o > last) && o < best) {
+        best = o;
+        bj = j;
+      }
+    }
+    last = best;
+    const float4 d = r[bj];
+    dx = make_float3(dx.x + d.x, dx.y + d.y, dx.z + d.z);
   }
-  float f = fixed[v];
-  float4 y = y4[v];
+  const float f = fixed[v];
+  const float4 y = y4[v];
   y4[v] = make_float4(y.x + f * (dx.x / nc), y.y + f * (dx.y / nc), y.z + f * (dx.z / nc), y.w);
 }
 
-__global__ void k_fill2(int n, int32_t* __restrict__ a, int32_t* __restrict__ b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) { a[i] = 0; b[i] = 0; }
-}
-
-static int vbits(int nv) {
-  int b = 1;
-  while ((1ll << b) < (long long)nv) ++b;
-  return b;
+__global__ void k_zero_i32(int n, int32_t* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 struct CollisionWs {
-  int32_t *cnt, *off, *total, *seg_lo, *seg_hi, *ridx;
-  uint32_t* rkey;
-  float4* rval;
+  int32_t* bcnt;
   float4* self_acc;
-  void* scan_ws;
-  size_t scan_bytes;
-  void* sort_ws;
-  size_t sort_bytes;
+  float4* brec;
 };
 
-static size_t layout(int nv, int nvs, int rec_cap, char* base, CollisionWs* w) {
-  size_t nq = (size_t)(2 * nv + nvs), o = 0;
+static size_t layout(int nv, int bucket, char* base, CollisionWs* w) {
+  size_t o = 0;
   auto take = [&](size_t bytes) {
     size_t at = o;
     o += (bytes + 255) & ~(size_t)255;
     return base ? base + at : nullptr;
   };
-  size_t rc = (size_t)(rec_cap > 0 ? rec_cap : 1);
   CollisionWs t;
-  t.cnt = (int32_t*)take(4 * nq);
-  t.off = (int32_t*)take(4 * nq);
-  t.total = (int32_t*)take(256);
-  t.seg_lo = (int32_t*)take(4 * (size_t)nv);
-  t.seg_hi = (int32_t*)take(4 * (size_t)nv);
-  t.rkey = (uint32_t*)take(4 * rc);
-  t.ridx = (int32_t*)take(4 * rc);
-  t.rval = (float4*)take(16 * rc);
+  t.bcnt = (int32_t*)take(4 * (size_t)nv);
   t.self_acc = (float4*)take(16 * (size_t)nv);
-  t.scan_bytes = scan_ws_bytes((int)nq);
-  t.scan_ws = take(t.scan_bytes);
-  t.sort_bytes = radix_sort_ws_bytes((int)rc);
-  t.sort_ws = take(t.sort_bytes);
+  t.brec = (float4*)take(16 * (size_t)nv * (size_t)(bucket > 0 ? bucket : 1));
   if (w) *w = t;
   return o;
 }
@@ -343,7 +300,8 @@ static size_t layout(int nv, int nvs, int rec_cap, char* base, CollisionWs* w) {
 using namespace slx;
 
 extern "C" size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap) {
-  return layout(nv, nvs, rec_cap, nullptr, nullptr);
+  (void)nvs;
+  return layout(nv, rec_cap, nullptr, nullptr);
 }
 
 extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
@@ -352,36 +310,26 @@ extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, cons
                                   const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st,
                                   int32_t* pair_num, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                                   slx_stream_t stream) {
-  SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 0, "slx_collision_pass: bad sizes");
+  SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
   if (nv == 0) return SLX_OK;
   SLX_REQUIRE(faces_dy && y4 && fixed && m_inv && ws, "slx_collision_pass: null pointer");
   SLX_REQUIRE(!(c_vt_st || c_tv_st) || (faces_st && xs4), "slx_collision_pass: static mesh missing");
   SLX_REQUIRE((!c_vt_st || n_vt_st) && (!c_vt_dy || n_vt_dy) && (!c_tv_st || n_tv_st),
               "slx_collision_pass: candidate counts missing");
+  SLX_REQUIRE((long long)(nv + nvs) * (long long)cap * 4ll < (1ll << 32),
+              "slx_collision_pass: too many queries x candidates for 32-bit order keys");
   if (ws_bytes < slx_collision_ws_bytes(nv, nvs, rec_cap))
     return fail(SLX_EWS, "slx_collision_pass: workspace too small");
   hipStream_t st = (hipStream_t)stream;
   CollisionWs w;
-  layout(nv, nvs, rec_cap, (char*)ws, &w);
+  layout(nv, rec_cap, (char*)ws, &w);
   QuerySets Q{nv, c_tv_st ? nvs : 0, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
   const int nqt = 2 * nv + Q.nvs;
-  SLX_LAUNCH("contact_count", k_contact_count, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w.cnt, pair_num, w.self_acc);
-  int rc = exclusive_scan(nqt, nullptr, w.cnt, w.off, w.total, w.scan_ws, w.scan_bytes, st);
-  if (rc) return rc;
-  SLX_LAUNCH("contact_emit", k_contact_emit, dim3(blocks_for(nqt - nv, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w.off, rec_cap, w.rkey, w.ridx,
-             w.rval, status);
-  SLX_LAUNCH("clamp_total", k_clamp_total, dim3(1), dim3(64), 0, st, w.total, rec_cap);
-  if (rec_cap > 0) {
-    rc = radix_sort_pairs(rec_cap, w.total, w.rkey, w.ridx, vbits(nv), w.sort_ws, w.sort_bytes, st);
-    if (rc) return rc;
-  }
-  SLX_LAUNCH("fill", k_fill2, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.seg_lo, w.seg_hi);
-  if (rec_cap > 0)
-    SLX_LAUNCH("seg_bounds", k_seg_bounds, dim3(blocks_for(rec_cap, 256)), dim3(256), 0, st, rec_cap, w.total,
-               w.rkey, w.seg_lo, w.seg_hi);
-  SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.seg_lo, w.seg_hi,
-             w.ridx, w.rval, w.self_acc, fixed, (float4*)y4);
+  SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
+  SLX_LAUNCH("contacts", k_contacts, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
+             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec, w.self_acc,
+             pair_num, status);
+  SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt,
+             w.brec, w.self_acc, fixed, (float4*)y4);
   return check_launch("slx_collision_pass");
 }

@@ -21,6 +21,7 @@ constexpr int kCells = kGrid * kGrid * kGrid;   // 32768
 constexpr int kStatusCandOverflow = SLX_STATUS_CAND_OVERFLOW;
 constexpr int kStatusStackOverflow = SLX_STATUS_STACK_OVERFLOW;
 constexpr int kStatusCellRange = SLX_STATUS_CELL_RANGE;
+constexpr int kStatusRecOverflow = SLX_STATUS_REC_OVERFLOW;
 
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }

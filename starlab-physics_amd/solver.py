@@ -57,7 +57,7 @@ class Solver:
         self.refit_every_substep = False
         self.cache_size = 40
         self.compliance_collision = 1e8          # XPBD.py:367
-        self.records_per_query = 8
+        self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
         self.max_colors = 64
         self._gs = None
@@ -110,7 +110,7 @@ class Solver:
     # --------------------------------------------------------------- internals
     def _collision_buffers(self):
         import torch
-        key = (self.bvh_mode, self.cache_size, self.records_per_query)
+        key = (self.bvh_mode, self.cache_size, self.records_per_vertex)
         if self._cd is not None and self._cd["key"] == key:
             return self._cd
         m, s = self.mesh_dy, self.mesh_st
@@ -124,7 +124,7 @@ class Solver:
         cd["cand"] = torch.full((cap * (2 * nv + nvs) + 1,), -1, **i32)
         cd["cand_n"] = torch.zeros(2 * nv + nvs + 1, **i32)
         cd["pair_num"] = torch.zeros(2 * nv + nvs + 1, **i32)
-        cd["rec_cap"] = int(self.records_per_query * (2 * nv + nvs) + 1024)
+        cd["rec_cap"] = int(self.records_per_vertex)
         cd["static_built"] = False
         if s is not None:
             s.xs4 = s.x4
