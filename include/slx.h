@@ -316,6 +316,8 @@ typedef struct slx_tree {
   int32_t* range_r;
   int32_t* roots;            /* [n_inst] */
   int32_t* sched;            /* refit schedule, slx_refit_sched_ints(leaves) (cell: of the static topology) */
+  const int32_t* vert_order; /* [nv] order in which this tree's mesh's vertices are processed as queries
+                                (nullable; e.g. Morton order, so that a wavefront's queries are neighbours) */
 } slx_tree;
 
 typedef struct slx_frame_desc {

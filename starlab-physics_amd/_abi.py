@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libslx.so")
+LIB_PATH = os.environ.get("SLX_LIB") or os.path.join(_HERE, "libslx.so")   # SLX_LIB: another build of the library
 _LIB = None
 
 I32, F32, PTR, STREAM = C.c_int32, C.c_float, C.c_void_p, C.c_void_p
@@ -86,7 +86,8 @@ class Tree(C.Structure):
                 ("parent", PTR), ("num", PTR), ("face_cell", PTR), ("prefix", PTR), ("leaf_l", PTR),
                 ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR), ("n_inst", I32), ("vert_base", PTR),
                 ("face_base", PTR), ("face_inst", PTR), ("vert_inst", PTR), ("sorted_inst", PTR),
-                ("range_l", PTR), ("range_r", PTR), ("roots", PTR), ("sched", PTR)]
+                ("range_l", PTR), ("range_r", PTR), ("roots", PTR), ("sched", PTR),
+                ("vert_order", PTR)]
 
 
 class FrameDesc(C.Structure):

@@ -100,6 +100,7 @@ class DeviceTree:
                       P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
                       P(self.nlo4), P(self.nhi4))
         t.range_l, t.range_r, t.sched = P(self.range_l), P(self.range_r), P(self.sched)
+        t.vert_order = P(getattr(self.mesh, "vert_order", None))
         if self.n_inst > 1:
             I = self.inst
             t.n_inst = self.n_inst

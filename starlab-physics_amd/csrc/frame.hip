@@ -53,14 +53,16 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
 }
 
 // queries of a mesh whose vertices belong to instances q_inst (batched trees only)
-int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst, float pad, int cap,
-          int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t s) {
+// q_order: processing order of the query vertices (nullable)
+int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst,
+          const int32_t* q_order, float pad, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
+          slx_stream_t s) {
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
   const bool seg = batched(t);
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4, t.fmin4, t.fmax4,
                          t.box6, cap, cand, cand_n, status, (hipStream_t)s, seg ? t.roots : nullptr,
-                         seg ? q_inst : nullptr);
+                         seg ? q_inst : nullptr, q_order);
 }
 
 }  // namespace
@@ -154,15 +156,16 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (!coll) continue;
       // broad phase (XPBD.py:186-215)
       if (nvs > 0) {
-        rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, pad, cap, c0, n0, d->status,
-                   stream);
+        rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
+                   n0, d->status, stream);
         if (rc) return rc;
       }
-      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, pad, cap, c1, n1, d->status,
-                 stream);
+      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
+                 n1, d->status, stream);
       if (rc) return rc;
       if (nvs > 0 && !tv_fresh) {
-        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, pad, cap, c2, n2, d->status, stream);
+        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
+                   d->status, stream);
         if (rc) return rc;
         tv_fresh = true;
       }

@@ -25,7 +25,13 @@ size_t scan_ws_bytes(int n_max);
 int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* total_dev, void* ws,
                    size_t ws_bytes, hipStream_t st);
 
-constexpr int kStack = 64;       // traversal stack depth (entries)
+#ifndef SLX_TRAV_WAVE
+#define SLX_TRAV_WAVE 1   // wavefront-coherent triangle traversal (0: one walk per lane)
+#endif
+#ifndef SLX_TRAV_STACK
+#define SLX_TRAV_STACK 64
+#endif
+constexpr int kStack = SLX_TRAV_STACK;   // traversal stack depth (entries), as the reference's 64
 constexpr int kTravBS = 128;     // traversal block size
 
 // ----------------------------------------------------------------- AABBs
@@ -416,8 +422,10 @@ __device__ __forceinline__ NodeBox leaf_box(int k, int n, const LeafSrc& s, int3
   return b;
 }
 
-// One workgroup per subtree: leaves into LDS, then every thread walks up from its
-// leaves; the second child to arrive at a node refits it.
+// One workgroup per subtree.  Phase 1 loads, coalesced, the leaves' boxes and
+// parents and the subtree's internal nodes' children and parents (all contiguous
+// index ranges) into LDS; phase 2 walks up from every leaf, the second child to
+// arrive at a node refitting it, touching global memory only to store boxes.
 template <int LEAF>
 __global__ void __launch_bounds__(kSubBS)
 k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t* __restrict__ ca,
@@ -427,6 +435,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
   __shared__ NodeBox s_leaf[kSubMax];
   __shared__ NodeBox s_int[kSubMax];
   __shared__ int s_flag[kSubMax];
+  __shared__ int s_ca[kSubMax], s_cb[kSubMax], s_par[kSubMax], s_lpar[kSubMax];
   const Sched S{sched, n};
   const int n_sub = S.n_sub();
   for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
@@ -434,23 +443,30 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
     for (int k = threadIdx.x; k < cnt; k += kSubBS) {
       s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
+      s_lpar[k] = parent[n - 1 + first + k];
       s_flag[k] = 0;
+      if (first + k < n - 1) {          // internal node first+k (those outside the subtree are never read)
+        s_ca[k] = ca[first + k];
+        s_cb[k] = cb[first + k];
+        s_par[k] = parent[first + k];
+      }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < cnt; k += kSubBS) {
       int node = n - 1 + first + k;
-      while (node != root) {
-        const int p = parent[node];          // inside the subtree: first <= p <= last
+      int p = s_lpar[k];
+      while (node != root) {               // p is inside the subtree: first <= p <= last
         __threadfence_block();
         if (atomicAdd(&s_flag[p - first], 1) == 0) break;
         __threadfence_block();
-        const int l = ca[p], r = cb[p];
+        const int l = s_ca[p - first], r = s_cb[p - first];
         const NodeBox A = l >= n - 1 ? s_leaf[l - (n - 1) - first] : s_int[l - first];
         const NodeBox B = r >= n - 1 ? s_leaf[r - (n - 1) - first] : s_int[r - first];
         const NodeBox P = unite<COUNTED>(A, B);
         s_int[p - first] = P;
         store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
         node = p;
+        p = s_par[p - first];
       }
     }
     __syncthreads();
@@ -469,12 +485,17 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
   __shared__ NodeBox s_cell[kTopMax][2];
   __shared__ int s_flag[kTopMax];
   __shared__ int s_up[kTopMax];
+  __shared__ int s_node[kTopMax], s_l[kTopMax], s_r[kTopMax];
   const Sched S{sched, n};
   const int n_sub = S.n_sub(), n_top = S.n_top();
   const int nl = n_top < S.top_lds() ? n_top : S.top_lds();
   for (int i = threadIdx.x; i < nl; i += kTopBS) {
     s_flag[i] = 0;
     s_up[i] = S.top_up(i);
+    const int p = S.top_node(i);
+    s_node[i] = p;
+    s_l[i] = ca[p];
+    s_r[i] = cb[p];
   }
   for (int i = threadIdx.x; i < n_sub; i += kTopBS) {
     const int4 e = S.sub(i);
@@ -494,15 +515,20 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
         prev = __hip_atomic_fetch_add(S.gflag(slot), 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       if (prev == 0) break;
-      const int p = S.top_node(slot);
-      const int l = ca[p], r = cb[p];
+      int p, l, r;
       NodeBox A, B;
       if (lds) {
         __threadfence_block();
+        p = s_node[slot];
+        l = s_l[slot];
+        r = s_r[slot];
         A = s_cell[slot][0];
         B = s_cell[slot][1];
       } else {
         __hip_atomic_store(S.gflag(slot), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p = S.top_node(slot);
+        l = ca[p];
+        r = cb[p];
         A = load_node<COUNTED>(l, num, nlo, nhi);
         B = load_node<COUNTED>(r, num, nlo, nhi);
       }
@@ -572,10 +598,11 @@ k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32
            const int32_t* __restrict__ sorted, const float4* __restrict__ nlo, const float4* __restrict__ nhi,
            const float4* __restrict__ fmin, const float4* __restrict__ fmax, const float* __restrict__ box,
            int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status,
-           const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst) {
+           const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order) {
   __shared__ int32_t s_stack[kStack * kTravBS];
-  const int q = blockIdx.x * kTravBS + threadIdx.x;
-  if (q >= nq) return;
+  const int t = blockIdx.x * kTravBS + threadIdx.x;
+  if (t >= nq) return;
+  const int q = order ? order[t] : t;
   int32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   const float3 p = xyz(q4[q]);
   const float pp = pad * 1.0f;
@@ -803,11 +830,180 @@ extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, con
 }
 
 namespace slx {
+
+// Triangle-tree traversal (traverse_cell_bvh_single's stack walk,
+// lbvh_cell.py:661-714, on the per-triangle tree): a popped node's two children
+// are tested together and only overlapping ones are pushed.  In a Karras tree the
+// children of an internal node are split and split+1 (internal, or the leaves
+// n-1+split, n-1+split+1), so their boxes are neighbours in memory and one stack
+// word names both: the left child index, with bit 31 set when the right child is
+// a leaf.  A left leaf that overlaps is pushed as an emit entry (face | bit 30) so
+// that it is reported after the right subtree, exactly as in the reference's order
+// (push left, push right, pop right first).  Candidate lists therefore match the
+// push-everything walk entry for entry; the stack only ever holds overlapping
+// nodes, so it is never deeper than that walk's.
+constexpr uint32_t kRightLeaf = 1u << 31, kEmit = 1u << 30, kLow30 = kEmit - 1;
+
+__global__ void __launch_bounds__(kTravBS)
+k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+               const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n,
+               int32_t* status, const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst,
+               const int32_t* __restrict__ order) {
+  __shared__ uint32_t s_stack[kStack * kTravBS];
+  const int t = blockIdx.x * kTravBS + threadIdx.x;
+  if (t >= nq) return;
+  const int q = order ? order[t] : t;     // spatially coherent queries share nodes within a wavefront
+  uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
+  const float3 p = xyz(q4[q]);
+  const float pp = pad * 1.0f;
+  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  int cnt = 0, flags = 0;
+  const int root = roots ? roots[q_inst[q]] : 0;
+  if (nl <= 0 || root < 0) {
+    cand_n[q] = 0;
+    return;
+  }
+  auto emit = [&](int f) {
+    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
+    else flags |= kStatusCandOverflow;
+    ++cnt;
+  };
+  const int leaf0 = nl - 1;
+  auto expand_word = [&](int l, int r) { return (uint32_t)l | (r >= leaf0 ? kRightLeaf : 0u); };
+  int sp = 0;
+  {
+    const float4 lo = nlo[root], hi = nhi[root];
+    if (overlap(a, b, xyz(lo), xyz(hi))) {
+      if (root >= leaf0) emit(__float_as_int(lo.w));
+      else stk[(sp++) * kTravBS] = expand_word(__float_as_int(lo.w), __float_as_int(hi.w));
+    }
+  }
+  while (sp > 0) {
+    const uint32_t e = stk[(--sp) * kTravBS];
+    if (e & kEmit) {
+      emit((int)(e & kLow30));
+      continue;
+    }
+    const int l = (int)(e & kLow30);
+    const int split = l >= leaf0 ? l - leaf0 : l;
+    const int r = (e & kRightLeaf) ? leaf0 + split + 1 : split + 1;
+    const float4 l0 = nlo[l], l1 = nhi[l], r0 = nlo[r], r1 = nhi[r];
+    if (overlap(a, b, xyz(l0), xyz(l1))) {
+      const uint32_t w = l >= leaf0 ? ((uint32_t)__float_as_int(l0.w) | kEmit)
+                                    : expand_word(__float_as_int(l0.w), __float_as_int(l1.w));
+      if (sp < kStack) stk[(sp++) * kTravBS] = w;
+      else flags |= kStatusStackOverflow;
+    }
+    if (overlap(a, b, xyz(r0), xyz(r1))) {
+      if (r >= leaf0) {
+        emit(__float_as_int(r0.w));
+      } else if (sp < kStack) {
+        stk[(sp++) * kTravBS] = expand_word(__float_as_int(r0.w), __float_as_int(r1.w));
+      } else {
+        flags |= kStatusStackOverflow;
+      }
+    }
+  }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (flags && status) atomicOr(status, flags);
+}
+
+// Wavefront-coherent variant of k_traverse_tri: the 64 lanes of a wavefront (64
+// queries, taken in a spatially coherent order) walk one shared depth-first
+// traversal.  A child is pushed when any lane's box overlaps it, node boxes are
+// wave-uniform scalar loads, and the stack is wave-uniform too: entry d lives in
+// lane d's register (64 entries, as the reference's stack), read back with
+// readlane, so no LDS is used.  Each lane emits a leaf only when its own box
+// overlaps it; every ancestor of such a leaf overlaps that box too, so the leaves a
+// lane emits, and their order, are exactly those of its own walk (the shared walk
+// visits a superset of its nodes in the same right-first order).
+__global__ void __launch_bounds__(256)
+k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+                    const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand,
+                    int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool active = t < nq;
+  const int q = active ? (order ? order[t] : t) : 0;
+  float3 a, b;
+  if (active) {
+    const float3 p = xyz(q4[q]);
+    const float pp = pad * 1.0f;
+    a = make_float3(p.x - pp, p.y - pp, p.z - pp);
+    b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  } else {                                   // an empty box: overlaps nothing
+    const float inf = __int_as_float(0x7f800000);
+    a = make_float3(inf, inf, inf);
+    b = make_float3(-inf, -inf, -inf);
+  }
+  int cnt = 0, flags = 0;
+  auto emit = [&](int f) {
+    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
+    else flags |= kStatusCandOverflow;
+    ++cnt;
+  };
+  const int leaf0 = nl - 1;
+  auto expand_word = [&](int l, int r) { return (uint32_t)l | (r >= leaf0 ? kRightLeaf : 0u); };
+  uint32_t sreg = 0;                         // this lane's stack slot
+  int sp = 0;                                // wave-uniform depth
+  auto push = [&](uint32_t v) {
+    if (sp < kWave) {
+      if (lane == sp) sreg = v;
+      ++sp;
+    } else {
+      flags |= kStatusStackOverflow;
+    }
+  };
+  if (nl > 0) {
+    const float4 lo = nlo[0], hi = nhi[0];
+    const bool ov = overlap(a, b, xyz(lo), xyz(hi));
+    if (__ballot(ov)) {
+      if (leaf0 == 0) {
+        if (ov) emit(__float_as_int(lo.w));
+      } else {
+        push(expand_word(__float_as_int(lo.w), __float_as_int(hi.w)));
+      }
+    }
+  }
+  while (__builtin_amdgcn_readfirstlane(sp) > 0) {
+    sp = __builtin_amdgcn_readfirstlane(sp) - 1;
+    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)sreg, sp);
+    if (e & kEmit) {                         // a left leaf, reported after its right sibling's subtree
+      const int L = (int)(e & kLow30);
+      const float4 lo = nlo[L], hi = nhi[L];
+      if (overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
+      continue;
+    }
+    const int l = (int)(e & kLow30);
+    const int split = l >= leaf0 ? l - leaf0 : l;
+    const int r = (e & kRightLeaf) ? leaf0 + split + 1 : split + 1;
+    const float4 l0 = nlo[l], l1 = nhi[l], r0 = nlo[r], r1 = nhi[r];
+    const bool ovl = overlap(a, b, xyz(l0), xyz(l1));
+    const bool ovr = overlap(a, b, xyz(r0), xyz(r1));
+    if (__ballot(ovl))
+      push(l >= leaf0 ? ((uint32_t)l | kEmit) : expand_word(__float_as_int(l0.w), __float_as_int(l1.w)));
+    if (__ballot(ovr)) {
+      if (r >= leaf0) {
+        if (ovr) emit(__float_as_int(r0.w));
+      } else {
+        push(expand_word(__float_as_int(r0.w), __float_as_int(r1.w)));
+      }
+    }
+  }
+  if (active) {
+    cand_n[q] = cnt < cap ? cnt : cap;
+    if (flags && status) atomicOr(status, flags);
+  }
+}
+
+}  // namespace slx
+
+namespace slx {
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
                     int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
-                    const int32_t* q_inst) {
+                    const int32_t* q_inst, const int32_t* order) {
   SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
@@ -815,14 +1011,22 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   SLX_REQUIRE(q4 && cand_n && (cap == 0 || cand), "slx_traverse: null pointer");
   SLX_REQUIRE(n_leaves <= 0 || (nlo4 && nhi4), "slx_traverse: null tree");
   SLX_REQUIRE(mode == 1 || (num && sorted_faces && fmin4 && fmax4 && box6), "slx_traverse: null cell tree");
+  SLX_REQUIRE(mode == 0 || n_leaves < (1 << 29), "slx_traverse: triangle tree too large for 30-bit stack words");
   if (mode == 0)
     SLX_LAUNCH(name, k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
-               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst);
+               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst, order);
+  // One walk per lane, unless the tree is coarse next to the queries (far fewer
+  // leaves than queries: the lanes of a wavefront then overlap the same few
+  // leaves and the shared walk is short).  Batched trees: lanes have different
+  // roots.  The candidate lists are the same either way.
+  else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq)
+    SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
+               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots, q_inst,
+               order);
   else
-    SLX_LAUNCH(name, k_traverse<false>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
-               (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
-               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst);
+    SLX_LAUNCH(name, k_traverse_tri_wave, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
+               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order);
   return check_launch("slx_traverse");
 }
 }  // namespace slx
@@ -833,7 +1037,7 @@ extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, in
                             int32_t* status, slx_stream_t stream) {
   return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
                          sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream), nullptr,
-                         nullptr);
+                         nullptr, nullptr);
 }
 
 // ===================== batched instances: one tree, one subtree per instance =====================
@@ -1036,5 +1240,5 @@ extern "C" int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, 
                                 int32_t* status, slx_stream_t stream) {
   SLX_REQUIRE(roots && q_inst, "slx_traverse_seg: roots and q_inst required");
   return traverse_launch("traverse_seg", 1, nq, q4, padding, nf, nullptr, nullptr, nlo4, nhi4, nullptr, nullptr,
-                         nullptr, cap, cand, cand_n, status, S(stream), roots, q_inst);
+                         nullptr, cap, cand, cand_n, status, S(stream), roots, q_inst, nullptr);
 }

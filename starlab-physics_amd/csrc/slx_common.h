@@ -70,7 +70,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
                     int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
-                    const int32_t* q_inst = nullptr);
+                    const int32_t* q_inst = nullptr, const int32_t* order = nullptr);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);

@@ -74,6 +74,7 @@ class Mesh:
         self.x4[:, :3] = self.x0
         self.y4[:, :3] = self.x0
         self._refresh_products()
+        self.vert_order = torch.from_numpy(_topo.morton_order(self.x0_np)).to(dev)
         self.face_indices = torch.from_numpy(self.fid_np.reshape(-1)).to(dev)
         self.edge_indices = torch.from_numpy(self.eid_np.reshape(-1)).to(dev)
         self.bending_indices = torch.from_numpy(self.bend_np.reshape(-1)).to(dev)

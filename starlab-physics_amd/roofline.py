@@ -9,9 +9,10 @@ candidates per query of a query set.
   predict                      40 V   (read x, v, f; write y)
   one Jacobi spring pass       12 (E+B) + 40 V   (ids + rest length; read y, x, f*w; write y)
   velocity / position update   52 V   (read x, y, f; write v, x)
-  refit                        76 F + 12 V
+  refit (face boxes fused)     96 F + 16 V   (faces + leaf order 16 F, topology 16 F, x 16 V; write 2F-1 nodes)
   traversal (nq queries, tree of Ft faces)   16 nq + 4 c nq + 32 (2 Ft - 1)
   narrow phase                 16 c V + 24 V  (per query set of V queries)
+  contact apply                56 V   (counts, first-set sums, f; read/write y)
   per-frame rebuild            Morton 20 F + 12 V, radix sort 68 F, Karras 16 F
 """
 from __future__ import annotations
@@ -32,7 +33,11 @@ def update_vx(V):
 
 
 def refit(V, F):
-    return 76 * F + 12 * V
+    return 96 * F + 16 * V
+
+
+def contact_apply(V):
+    return 56 * V
 
 
 def traversal(nq, Ft, c):
@@ -68,8 +73,8 @@ def kernel_bytes(name: str, n: dict):
     nar = narrow(V, c.get("vt_dy", 0.0))
     if "Vs" in n:
         nar += narrow(V, c.get("vt_st", 0.0)) + narrow(n["Vs"], c.get("tv_st", 0.0))
-    table["contact_count"] = nar
-    table["contact_emit"] = nar
+    table["contacts"] = nar
+    table["contact_apply"] = contact_apply(V)
     return table.get(name)
 
 
@@ -80,7 +85,7 @@ def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
     total = substeps * sub
     if collision:
         c = n.get("cbar", {})
-        per_iter = traversal(V, F, c.get("vt_dy", 0.0)) + narrow(V, c.get("vt_dy", 0.0))
+        per_iter = traversal(V, F, c.get("vt_dy", 0.0)) + narrow(V, c.get("vt_dy", 0.0)) + contact_apply(V)
         if "Vs" in n:
             per_iter += traversal(V, n["Fs"], c.get("vt_st", 0.0)) + narrow(V, c.get("vt_st", 0.0))
             per_iter += narrow(n["Vs"], c.get("tv_st", 0.0))

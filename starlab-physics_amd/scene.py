@@ -72,22 +72,27 @@ def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)) -> TriMe
     return TriMesh(x.astype(np.float32), np.array(faces, np.int32))
 
 
-def terrain(m: int = 142, size: float = 1.2, waves: int = 8, amplitude: float = 0.1,
-            base: float = 0.0, seed: int = 0) -> TriMesh:
-    """m x m height field over [-size/2, size/2]^2 with
-    y = base + sum_k (amplitude/waves) sin(kx_k x + kz_k z + phi_k); wave
-    vectors have magnitude U(2, 12) rad/m and uniform direction; phases U(0, 2 pi)."""
+def terrain_height(x, z, waves: int = 8, amplitude: float = 0.1, base: float = 0.0, seed: int = 0):
+    """y = base + sum_k (amplitude/waves) sin(kx_k x + kz_k z + phi_k) (float64);
+    wave vectors have magnitude U(2, 12) rad/m and uniform direction; phases U(0, 2 pi)."""
     rng = np.random.default_rng(seed)
     mag = rng.uniform(2.0, 12.0, waves)
     ang = rng.uniform(0.0, 2 * np.pi, waves)
     phi = rng.uniform(0.0, 2 * np.pi, waves)
+    x = np.asarray(x, np.float64)
+    z = np.asarray(z, np.float64)
+    h = np.zeros(np.broadcast(x, z).shape)
+    for k in range(waves):
+        h += (amplitude / waves) * np.sin(mag[k] * np.cos(ang[k]) * x + mag[k] * np.sin(ang[k]) * z + phi[k])
+    return base + h
+
+
+def terrain(m: int = 142, size: float = 1.2, waves: int = 8, amplitude: float = 0.1,
+            base: float = 0.0, seed: int = 0) -> TriMesh:
+    """m x m height field over [-size/2, size/2]^2 of terrain_height."""
     g = grid_cloth(m, size=size, height=0.0)
     x = g.x.astype(np.float64)
-    h = np.zeros(len(x))
-    for k in range(waves):
-        h += (amplitude / waves) * np.sin(mag[k] * np.cos(ang[k]) * x[:, 0]
-                                          + mag[k] * np.sin(ang[k]) * x[:, 2] + phi[k])
-    x[:, 1] = base + h
+    x[:, 1] = terrain_height(x[:, 0], x[:, 2], waves, amplitude, base, seed)
     return TriMesh(x.astype(np.float32), g.faces)
 
 

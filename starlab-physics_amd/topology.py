@@ -180,3 +180,28 @@ def greedy_edge_coloring(nv: int, edges: np.ndarray, max_colors: int = 64):
     graph_coloring.py:172-201)."""
     from . import _abi
     return _abi.greedy_coloring(nv, edges, max_colors)
+
+
+def _spread10(v):
+    """Insert two zero bits between each of the low 10 bits (3-D Morton)."""
+    v = v.astype(np.uint64) & np.uint64(0x3FF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x030000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x0300F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x030C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x09249249)
+    return v
+
+
+def morton_order(x):
+    """Vertex permutation in 30-bit Morton order of the positions x (V,3) within
+    their bounding box (ties by index).  Used only as the order in which
+    traversal threads take their queries: lanes of a wavefront then query
+    neighbouring points and share tree nodes; results do not depend on it."""
+    x = np.asarray(x, np.float64)
+    if len(x) == 0:
+        return np.zeros(0, np.int32)
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    ext = np.where(hi > lo, hi - lo, 1.0)
+    q = np.clip(((x - lo) / ext * 1023.0).astype(np.int64), 0, 1023)
+    code = (_spread10(q[:, 0]) << np.uint64(2)) | (_spread10(q[:, 1]) << np.uint64(1)) | _spread10(q[:, 2])
+    return np.argsort(code, kind="stable").astype(np.int32)

@@ -213,6 +213,35 @@ def test_traversal_bit_exact(slx, orc, rng, mode, pad, cap):
     assert (gt.status.item() != 0) == (fl > 0)
 
 
+@pytest.mark.parametrize("side,pad,cap", [(12, 0.01, 40), (12, 0.05, 8), (2, 0.2, 40)])
+def test_traversal_coarse_tree_bit_exact(slx, orc, rng, side, pad, cap):
+    """Many queries against a small triangle tree: the launch takes the
+    wavefront-coherent walk (leaves x 8 <= queries); same lists as the oracle."""
+    import importlib
+    import torch
+    bp = importlib.import_module("starlab-physics_amd.broadphase")
+    mesh = crumpled(slx, side)
+    gt = bp.DeviceTree(mesh, "tri")
+    gt.build(pad)
+    ot = orc.TriBVH(mesh.fid_np, mesh.positions(), pad)
+    x = mesh.positions()
+    nq = 5000
+    q = (x[rng.integers(0, len(x), nq)] + rng.normal(0, 0.03, (nq, 3))).astype(np.float32)
+    assert mesh.num_faces * 8 <= nq
+    q4 = torch.zeros((nq, 4), dtype=torch.float32, device="cuda")
+    q4[:, :3] = T(q)
+    gt.status.zero_()
+    cand, cnt = gt.query(q4, pad, cap)
+    oc, on, fl = ot.query(q - np.float32(pad), q + np.float32(pad), cap)
+    cnt = cnt.cpu().numpy()
+    cand = cand.cpu().numpy()
+    np.testing.assert_array_equal(cnt, on)
+    for i in range(nq):
+        np.testing.assert_array_equal(cand[:cnt[i], i], oc[i, :on[i]])
+    assert (gt.status.item() != 0) == (fl > 0)
+    assert on.sum() > 0
+
+
 def test_collision_pass_bit_exact(slx, orc, rng):
     """Contact projection on the same y and the same candidate lists."""
     import importlib

@@ -173,3 +173,16 @@ def test_config4_instance_sizes(slx):
     batch = importlib.import_module("starlab-physics_amd.batch")
     sides = batch.instance_sides(256, 16, 256, seed=0)
     assert int((sides.astype(np.int64) ** 2).sum()) == 6416788
+
+
+def test_morton_order_is_a_permutation_with_locality(slx):
+    topo = __import__("importlib").import_module("starlab-physics_amd.topology")
+    g = slx.scene.grid_cloth(64, size=1.0, height=0.2, jitter=1e-3, seed=0)
+    order = topo.morton_order(g.x)
+    assert sorted(order.tolist()) == list(range(len(g.x)))
+    # 64 consecutive entries (one wavefront) stay within a small patch of the cloth
+    p = g.x[order]
+    spans = [np.ptp(p[i:i + 64, [0, 2]], axis=0).max() for i in range(0, len(p) - 63, 64)]
+    row_spans = [np.ptp(g.x[i:i + 64, [0, 2]], axis=0).max() for i in range(0, len(p) - 63, 64)]
+    assert np.median(spans) < 0.5 * np.median(row_spans)
+    assert topo.morton_order(np.zeros((0, 3))).shape == (0,)
