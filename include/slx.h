@@ -285,6 +285,26 @@ int slx_mlp_to_bf16(const float* w_f32, void* w_bf16, slx_stream_t stream);
  * Jacobi update residual. */
 int slx_max_update(int nv, const float* a4, const float* b4, uint32_t* out, slx_stream_t stream);
 
+/* Residual solve (no reference counterpart: the reference runs one pass per
+ * substep, XPBD.py:443-469, and repeating it does not converge because its bending
+ * term acts on the unchanging x).  Anchored at the reference pass's result y1, the
+ * substep's implicit-Euler stretch problem
+ *   min sum_v |y_v - y1_v|^2/(2 w_v) + sum_e kappa C_e(y)^2/2
+ * (w = fixed*m_inv, kappa = stiffness*dt^2, C_e the stretch constraints of
+ * XPBD.py:146-167) is solved by Jacobi with the diagonal Gauss-Newton step
+ * y_v -= f_v (y_v - y1_v + w_v sum kappa C n) / (1 + w_v sum kappa); the residual of
+ * an iterate is its largest step component.  Pass k reads y_k (ya for even k, yb
+ * for odd) and writes y_{k+1} into the other buffer, and max-reduces the residual
+ * of y_k into res[k] (float bits).  A pass whose predecessor's residual is <= tol,
+ * or that follows such a pass, only sets *conv = min(*conv, k-1) and returns, so
+ * the first k with res[k] <= tol names the answer y_k.  slx_solve_reset zeroes
+ * res[0..n_res) and sets *conv = INT_MAX (no memset; graph-safe). */
+int slx_solve_reset(int n_res, uint32_t* res, int32_t* conv, slx_stream_t stream);
+int slx_implicit_passes(int nv, const float* fixed, const float* y_anchor, float* ya, float* yb,
+                        const int32_t* slice_off, const int32_t* slice_width, const int32_t* entries,
+                        float kappa, int k0, int count, float tol, uint32_t* res, int32_t* conv,
+                        slx_stream_t stream);
+
 /* ---------------- one frame (Solver.forward, XPBD.py:443-469) ---------------- */
 
 typedef struct slx_tree {

@@ -12,6 +12,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+CLIP = 8.0
+
 LAYOUT = [("W1", (64, 32)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 64)), ("b3", (64,)),
           ("W4", (32, 64)), ("b4", (32,))]
 
@@ -68,6 +70,7 @@ def forward(P, feat, fmean, fistd, leaf_weights=False):
     requires_grad (for gradients w.r.t. the values the GPU multiplies)."""
     f = torch.as_tensor(np.asarray(feat, np.float64))
     a = (f - torch.as_tensor(np.asarray(fmean, np.float64))) * torch.as_tensor(np.asarray(fistd, np.float64))
+    a = a.clamp(-CLIP, CLIP)          # normalised inputs clamped to +-8 (csrc/mlp.hip norm_feature)
     a = torch.cat([a, torch.zeros(a.shape[0], 8, dtype=torch.float64)], dim=1)
     a = bf16(a)
     leaves = {}

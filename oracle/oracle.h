@@ -37,6 +37,12 @@ extern "C" {
                        const R* fixed, const R* m_inv, R k_b, const int* gs_order,          \
                        const int* edges, const R* l0, const int* color_off, int n_colors,   \
                        R k_s, R* dx, R* nc);                                                \
+  /* residual solve (this package's definition; no reference counterpart): one Jacobi    \
+     pass of the implicit-Euler stretch problem anchored at ya (stretch constraints of     \
+     XPBD.py:146-167 on y); *res = largest step component of the input y */               \
+  void orc_implicit_pass_##S(int nv, int ne, const int* edges, const R* l0, const R* ya,   \
+                             R* y, const R* fixed, const R* m_inv, R kappa, R* gs, R* ks,  \
+                             R* res);                                                       \
   /* XPBD.py:340-343 compute_velocity then XPBD.py:332-337 update_x */                       \
   void orc_update_vx_##S(int nv, R* x, R* v, const R* y, const R* fixed, R damping, R dt);  \
   /* distance.py:4-47 d_type_PT */                                                           \

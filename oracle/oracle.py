@@ -73,6 +73,20 @@ def spring_pass(mesh, x, y, fixed, m_inv, k_s, k_b):
     return y
 
 
+def implicit_pass(mesh, ya, y, fixed, m_inv, kappa):
+    """One residual-solve pass (implicit-Euler stretch problem anchored at ya);
+    updates y in place and returns the residual (largest step) of the input y."""
+    ct, s = _real(y.dtype)
+    gs = np.zeros_like(y)
+    ks = np.zeros(len(fixed), y.dtype)
+    res = np.zeros(1, y.dtype)
+    l0 = mesh["l0"].astype(y.dtype)
+    getattr(lib(), "orc_implicit_pass_" + s)(len(fixed), len(l0), P(mesh["edges"]), P(l0),
+                                              P(np.ascontiguousarray(ya, y.dtype)), P(y), P(fixed), P(m_inv),
+                                              ct(kappa), P(gs), P(ks), P(res))
+    return float(res[0])
+
+
 def gs_pass(mesh, x, y, fixed, m_inv, k_s, k_b, order, offsets):
     ct, s = _real(x.dtype)
     dt = x.dtype

@@ -78,6 +78,46 @@ void FN(spring_pass)(int nv, int ne, int nb, const int* edges, const R* l0, cons
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
 }
 
+/* Residual solve pass (implicit-Euler stretch problem anchored at ya, Jacobi with
+   the diagonal Gauss-Newton step): per vertex g = sum over stretch edges of
+   (kappa C) n with n from the neighbour to the vertex, ks = sum of kappa,
+   step = f ((y - ya) + w g) / (1 + w ks), y -= step; *res = max |step| over all
+   components of the input y's steps.  Scattered in edge order, which is each
+   vertex's constraint order. */
+void FN(implicit_pass)(int nv, int ne, const int* edges, const R* l0, const R* ya, R* y, const R* fixed,
+                       const R* m_inv, R kappa, R* gs, R* ks, R* res) {
+  for (int i = 0; i < 3 * nv; ++i) gs[i] = 0;
+  for (int i = 0; i < nv; ++i) ks[i] = 0;
+  for (int i = 0; i < ne; ++i) {
+    const int a = edges[2 * i], b = edges[2 * i + 1];
+    R d[3];
+    SUB(y + 3 * a, y + 3 * b, d);
+    R len = SQRT(DOT(d, d));
+    R C = len - l0[i];
+    R inv = (R)1 / len;
+    R kc = kappa * C;
+    for (int c = 0; c < 3; ++c) {
+      R t = kc * (inv * d[c]);
+      gs[3 * a + c] = gs[3 * a + c] + t;
+      gs[3 * b + c] = gs[3 * b + c] - t;
+    }
+    ks[a] = ks[a] + kappa;
+    ks[b] = ks[b] + kappa;
+  }
+  R r = 0;
+  for (int v = 0; v < nv; ++v) {
+    R w = fixed[v] * m_inv[v];
+    R den = (R)1 + w * ks[v];
+    for (int c = 0; c < 3; ++c) {
+      R st = fixed[v] * (((y[3 * v + c] - ya[3 * v + c]) + w * gs[3 * v + c]) / den);
+      R as = st < (R)0 ? -st : st;
+      if (as > r) r = as;
+      y[3 * v + c] = y[3 * v + c] - st;
+    }
+  }
+  *res = r;
+}
+
 /* Graph-coloured Gauss-Seidel iteration (the reference colours stretch edges,
    graph_coloring.py:131-201, but has no sweep): bending constraints on x as one
    Jacobi-averaged pass, then for each colour, each edge of the colour in order,

@@ -68,6 +68,8 @@ SIGNATURES = {
     "slx_mlp_adam": [I32, PTR, PTR, PTR, PTR, PTR, F32, F32, F32, F32, I32, STREAM],
     "slx_mlp_to_bf16": [PTR, PTR, STREAM],
     "slx_max_update": [I32, PTR, PTR, PTR, STREAM],
+    "slx_solve_reset": [I32, PTR, PTR, STREAM],
+    "slx_implicit_passes": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, I32, I32, F32, PTR, PTR, STREAM],
     "slx_greedy_edge_coloring": [I32, I32, PTR, I32, PTR, PTR, PTR],
     "slx_gs_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, F32, F32, STREAM],
 }

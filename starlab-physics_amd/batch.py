@@ -31,9 +31,9 @@ def instance_sides(n_inst=256, lo=16, hi=256, seed=0):
 
 
 def make_instances(n_inst=256, side_lo=16, side_hi=256, seed=0, size=1.0, sphere_subdiv=3,
-                   r_lo=0.2, r_hi=0.5):
+                   r_lo=0.2, r_hi=0.5, gap=0.05):
     """Cloth instance i: side_i x side_i vertex grid (side ~ U{side_lo..side_hi}),
-    1 m square at height r_i + 0.05 above its own sphere of radius r_i ~ U(r_lo, r_hi)
+    1 m square at height r_i + gap above its own sphere of radius r_i ~ U(r_lo, r_hi)
     at the origin.  Instances share the space but never interact."""
     rng = np.random.default_rng(seed)
     sides = rng.integers(side_lo, side_hi + 1, n_inst)
@@ -42,7 +42,7 @@ def make_instances(n_inst=256, side_lo=16, side_hi=256, seed=0, size=1.0, sphere
     cloths, spheres = [], []
     for i in range(n_inst):
         r = float(radii[i])
-        cloths.append(_scene.grid_cloth(int(sides[i]), size=size, height=r + 0.05, jitter=1e-4, seed=seed + i))
+        cloths.append(_scene.grid_cloth(int(sides[i]), size=size, height=r + gap, jitter=1e-4, seed=seed + i))
         spheres.append(_scene.TriMesh((base.x.astype(np.float64) * r).astype(np.float32), base.faces))
     return cloths, spheres, sides, radii
 

@@ -11,7 +11,7 @@
 //   12-17 bending strains (|x_v-x_u| - l0)/l0 of the first 6 bending entries
 //   18-20 sum over stretch entries of strain * unit(y_v - y_u)
 //   21-23 sum over bending entries of strain * unit(x_v - x_u)
-// normalised by per-feature mean / std of the training set.
+// normalised by per-feature mean / std of the training set and clamped to +-8.
 //
 // Parameters (one FP32 master array, one BF16 working copy), K and N padded to
 // the MFMA tile: W1[64][32] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
@@ -114,6 +114,13 @@ __device__ __forceinline__ void load_weights(__bf16* sW, float* sB, const __bf16
   for (int i = threadIdx.x; i < kOutP; i += blockDim.x) sB[3 * kH + i] = Wf[ob4 + i];
 }
 
+// normalised input, clamped to +-kClip: the strain features are heavy-tailed (a
+// few vertices near pins sit tens of standard deviations out)
+constexpr float kClip = 8.0f;
+__device__ __forceinline__ float norm_feature(float f, float mean, float istd) {
+  return fminf(fmaxf((f - mean) * istd, -kClip), kClip);
+}
+
 // ---- features ----
 __device__ void vertex_features(int v, const float4* x4, const float4* v4, const float4* y4, const int32_t* soff,
                                 const int32_t* swid, const int2* ent, float dt, float inv_s, float* f) {
@@ -181,7 +188,7 @@ k_warm_start(int nv, const float4* __restrict__ x4, const float4* __restrict__ v
       if (v < nv) vertex_features(v, x4, v4, y4, soff, swid, ent, dt, inv_s, f);
       else for (int i = 0; i < kFeat; ++i) f[i] = 0.f;
       for (int i = 0; i < kInP; ++i)
-        a0[l * kInP + i] = (__bf16)(i < kFeat && v < nv ? (f[i] - fmean[i]) * fistd[i] : 0.f);
+        a0[l * kInP + i] = (__bf16)(i < kFeat && v < nv ? norm_feature(f[i], fmean[i], fistd[i]) : 0.f);
     }
     __syncthreads();
     FwdLds L{a0, a1, a2, a3, nullptr, nullptr, nullptr};
@@ -227,7 +234,8 @@ k_train_grad(int n, int tiles_per_block, const float* __restrict__ feat, const f
     if (l < 32) {
       const int m = m0 + l;
       for (int i = 0; i < kInP; ++i)
-        a0[l * kInP + i] = (__bf16)(i < kFeat && m < n ? (feat[(size_t)m * kFeat + i] - fmean[i]) * fistd[i] : 0.f);
+        a0[l * kInP + i] =
+            (__bf16)(i < kFeat && m < n ? norm_feature(feat[(size_t)m * kFeat + i], fmean[i], fistd[i]) : 0.f);
     }
     __syncthreads();
     FwdLds L{a0, a1, a2, a3, z[0], z[1], z[2]};

@@ -1,9 +1,11 @@
-"""GPU checks of the learned warm start: features and MLP forward against the
-float64 reference (BF16 operands emulated; tolerance 2e-2 relative on outputs,
-1e-5 on features), gradients of the MFMA backward against autograd (relative
-error of the whole gradient <= 3e-2), training reduces the loss, and the trained
-network reaches the residual in fewer Jacobi iterations than plain Jacobi, with
-the plain solver as fallback."""
+"""GPU checks of the learned warm start and the residual solve it seeds: the
+residual-solve passes bit for bit against the oracle (iterates and residuals)
+with their self-terminating chain; features and MLP forward against the float64
+reference (BF16 operands emulated; tolerance 2e-2 relative on outputs, 1e-5 on
+features); gradients of the MFMA backward against autograd (relative error of
+the whole gradient <= 3e-2); training reduces the loss; the trained network
+reaches the residual in fewer Jacobi iterations than plain Jacobi on a held-out
+seed; a broken network falls back to the plain solve with identical results."""
 import importlib
 
 import numpy as np
@@ -17,16 +19,68 @@ def L():
 
 
 def cloth(slx, n=48, seed=0):
+    """Cloth pinned at two corners with stiff stretch: the reference's k_s slot
+    (stretch, on y) gets stiffness_bending * dt^2 (XPBD.py:354-355), so both
+    stiffnesses are 1e6 here."""
     import torch
     g = slx.scene.grid_cloth(n, size=1.0, height=0.5, jitter=1e-3, seed=seed)
     mesh = slx.Mesh(x=g.x, faces=g.faces)
-    sim = slx.Solver(mesh, None, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3,
+    sim = slx.Solver(mesh, None, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e6,
                      g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
     sel = torch.zeros(mesh.num_verts, device="cuda")
     sel[0] = 1
     sel[n - 1] = 1
     sim.set_fixed_vertices(sel)
     return mesh, sim
+
+
+def test_implicit_passes_bit_exact_and_self_terminating(slx):
+    import torch
+    from oracle import oracle as orc
+    learned = L()
+    mesh, sim = cloth(slx, 24)
+    rs = learned.ResidualSolver(sim)
+    for _ in range(12):                       # a few plain substeps: the cloth swings
+        rs.substep(10, tol=1e-5, max_iters=400)
+    s = rs._scalars(10)
+    rs.reference_pass(s)
+    y1 = rs.y1[:, :3].cpu().numpy().copy()
+    fixed = mesh.fixed.cpu().numpy()
+    hd = mesh.host_dict()
+    npass = 6
+    oy = y1.copy()
+    ores = [orc.implicit_pass(hd, y1, oy, fixed, hd["m_inv"], np.float32(rs.kappa)) for _ in range(npass)]
+    A = importlib.import_module("starlab-physics_amd._abi")
+    res = torch.zeros(npass + 1, dtype=torch.int32, device="cuda")
+    conv = torch.zeros(1, dtype=torch.int32, device="cuda")
+    rs.ya.copy_(rs.y1)
+    A.call("slx_solve_reset", npass + 1, A.ptr(res), A.ptr(conv), A.stream_handle())
+    A.call("slx_implicit_passes", mesh.num_verts, A.ptr(mesh.fixed), A.ptr(rs.y1), A.ptr(rs.ya), A.ptr(rs.yb),
+           A.ptr(mesh.adj_slice_off), A.ptr(mesh.adj_slice_width), A.ptr(mesh.adj_entries), rs.kappa, 0, npass, 0.0,
+           A.ptr(res), A.ptr(conv), A.stream_handle())
+    gres = res.cpu().numpy().view(np.float32)[:npass]
+    np.testing.assert_array_equal(gres, np.array(ores, np.float32))
+    out = rs.ya if npass % 2 == 0 else rs.yb
+    np.testing.assert_array_equal(out[:, :3].cpu().numpy(), oy)
+    assert ores[0] > 1e-7 and ores[-1] < ores[0]
+    # a chain launched past convergence stops by itself: tol between r_1 and r_2
+    tol = float(np.float32(ores[2]))
+    first = next(k for k, r in enumerate(ores) if r <= tol)
+    rs.ya.copy_(rs.y1)
+    A.call("slx_solve_reset", npass + 1, A.ptr(res), A.ptr(conv), A.stream_handle())
+    A.call("slx_implicit_passes", mesh.num_verts, A.ptr(mesh.fixed), A.ptr(rs.y1), A.ptr(rs.ya), A.ptr(rs.yb),
+           A.ptr(mesh.adj_slice_off), A.ptr(mesh.adj_slice_width), A.ptr(mesh.adj_entries), rs.kappa, 0, npass, tol,
+           A.ptr(res), A.ptr(conv), A.stream_handle())
+    gres = res.cpu().numpy().view(np.float32)
+    assert conv.item() == first
+    np.testing.assert_array_equal(gres[:first + 1], np.array(ores[:first + 1], np.float32))
+    assert np.all(gres[first + 1:] == 0)
+    # y_first is intact in its buffer
+    oy = y1.copy()
+    for _ in range(first):
+        orc.implicit_pass(hd, y1, oy, fixed, hd["m_inv"], np.float32(rs.kappa))
+    got = rs.yb if first & 1 else rs.ya
+    np.testing.assert_array_equal(got[:, :3].cpu().numpy(), oy)
 
 
 def test_mfma_probe_layout():
@@ -51,10 +105,10 @@ def test_features_match_reference(slx):
     for _ in range(3):
         sim.forward(10, 1)
     rs = L().ResidualSolver(sim)
-    s = sim.scalars(10)
-    rs.predict(s)
+    s = rs._scalars(10)
+    rs.reference_pass(s)
     F = rs.features(s).cpu().numpy()
-    ref = mlp_ref.features(mesh.positions(), mesh.v4[:, :3].cpu().numpy(), rs.ypred[:, :3].cpu().numpy(),
+    ref = mlp_ref.features(mesh.positions(), mesh.v4[:, :3].cpu().numpy(), rs.y1[:, :3].cpu().numpy(),
                            mesh.adj, s["dt"], rs.inv_s)
     np.testing.assert_allclose(F, ref, rtol=1e-4, atol=1e-5)
     assert np.abs(ref).max() > 1e-3
@@ -94,50 +148,61 @@ def test_forward_matches_reference_through_warm_start(slx):
     import torch
     from oracle import mlp_ref
     mesh, sim = cloth(slx, 20)
+    for _ in range(3):
+        sim.forward(10, 1)
     learned = L()
     md = learned.WarmStart(seed=3)
-    md.out_scale = 1e-3
     rs = learned.ResidualSolver(sim, md)
-    s = sim.scalars(10)
-    rs.predict(s)
+    s = rs._scalars(10)
+    rs.reference_pass(s)
     F = rs.features(s).cpu().numpy()
+    # normalised inputs of order one, and an output scale whose corrections are
+    # far above float32 resolution of the positions
+    md.fmean.copy_(torch.from_numpy(F.mean(0).astype(np.float32)))
+    sd = F.std(0)
+    md.fistd.copy_(torch.from_numpy(np.where(sd > 0, 1.0 / np.maximum(sd, 1e-30), 0.0).astype(np.float32)))
+    md.out_scale = 1e-2
     rs.warm(s)
-    got = (rs.ya[:, :3] - rs.ypred[:, :3]).cpu().numpy() / md.out_scale
+    got = (rs.ya[:, :3] - rs.y1[:, :3]).cpu().numpy() / md.out_scale
     out, _ = mlp_ref.forward(mlp_ref.unpack(md.w.cpu().numpy()), F, md.fmean.cpu().numpy(), md.fistd.cpu().numpy())
     ref = out[:, :3].detach().numpy() * mesh.fixed.cpu().numpy()[:, None]
+    assert np.abs(ref).max() > 1e-2
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err < 2e-2, err
 
 
 def test_training_and_warm_start_gain(slx):
-    """Train on converged plain-Jacobi substeps of seeded variants; on a held-out
-    seed the warm start needs fewer Jacobi iterations to reach the residual."""
+    """Train on plain solves (residual 1e-6) of seeded variants; on a held-out
+    seed the warm start reaches the residual 1e-5 in fewer Jacobi iterations."""
     import torch
     learned = L()
     rng = np.random.default_rng(0)
     feats, tgts = [], []
-    for seed in (1, 2):
+    for seed in (1, 2, 3):
         mesh, sim = cloth(slx, 32, seed)
         rs = learned.ResidualSolver(sim)
         for _ in range(40):
-            rs.collect(10, 1e-6, 3000, 800, rng, feats, tgts)
+            rs.collect(10, 1e-6, 2000, 1000, rng, feats, tgts)
     feat = torch.cat(feats).contiguous()
     tgt = torch.cat(tgts).contiguous()
+    assert torch.isfinite(feat).all() and torch.isfinite(tgt).all()
     md = learned.WarmStart(seed=0)
     md.set_normalisation(feat, tgt)
     losses = md.train(feat, tgt, epochs=30, batch=4096, lr=2e-3)
-    assert losses[-1] < 0.5 * losses[0]
+    print("training loss first / last epoch:", losses[0], losses[-1])
+    assert np.isfinite(losses).all() and losses[-1] < 0.3 * losses[0]
     res = {}
     for learned_on in (False, True):
         mesh, sim = cloth(slx, 32, seed=7)
         rs = learned.ResidualSolver(sim, md)
-        it = 0
+        it, falls = 0, 0
         for _ in range(30):
-            k, _ = rs.substep(10, tol=1e-5, max_iters=3000, learned=learned_on)
+            k, fell = rs.substep(10, tol=1e-5, max_iters=2000, learned=learned_on)
             it += k
-        res[learned_on] = it
-    print("iterations plain / learned:", res[False], res[True])
-    assert res[True] < res[False]
+            falls += fell
+        res[learned_on] = (it, falls)
+    print("iterations plain / learned (fallbacks):", res[False], res[True])
+    assert res[True][0] < 0.5 * res[False][0]
 
 
 def test_fallback_when_the_network_is_wrong(slx):
@@ -146,11 +211,11 @@ def test_fallback_when_the_network_is_wrong(slx):
     md = learned.WarmStart(seed=5)
     md.out_scale = float("nan")   # a broken network: the residual is not finite
     rs = learned.ResidualSolver(sim, md)
-    k, fell = rs.substep(10, tol=1e-5, max_iters=3000, learned=True)
+    k, fell = rs.substep(10, tol=1e-5, max_iters=2000, learned=True)
     assert fell
     assert np.isfinite(mesh.positions()).all()
     # the fallback substep equals a plain substep from the same state
     mesh2, sim2 = cloth(slx, 24)
     rs2 = learned.ResidualSolver(sim2)
-    rs2.substep(10, tol=1e-5, max_iters=3000, learned=False)
+    rs2.substep(10, tol=1e-5, max_iters=2000, learned=False)
     np.testing.assert_array_equal(mesh.positions(), mesh2.positions())

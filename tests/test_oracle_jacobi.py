@@ -125,3 +125,69 @@ def test_planted_sign_error_is_caught(orc):
     orc.spring_pass(mesh, x, ya, f, w, 50.0, 1.0)
     orc.spring_pass(mesh, x, yb, f, w, 1.0, 50.0)
     assert abs(ya[0, 0] - yb[0, 0]) > 1e-3
+
+
+# ---- residual solve (implicit-Euler stretch problem anchored at ya) ----
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_implicit_pass_single_edge_closed_form(orc, dtype):
+    """Vertex 0 pinned, vertex 1 free (w) stretched by delta along x, anchored
+    where it is: one step moves it by w kappa delta / (1 + w kappa) towards 0."""
+    mesh = dict(edges=np.array([[0, 1]], np.int32), l0=np.array([1.0]))
+    y = np.array([[0, 0, 0], [1.25, 0, 0]], dtype)
+    ya = y.copy()
+    fixed = np.array([0.0, 1.0], dtype)
+    w = np.array([3.0, 2.0], dtype)
+    kappa = 0.7
+    r = orc.implicit_pass(mesh, ya, y, fixed, w, dtype(kappa))
+    step = 2.0 * kappa * 0.25 / (1 + 2.0 * kappa)
+    tol = 1e-6 if dtype == np.float32 else 1e-14
+    assert abs(y[1, 0] - (1.25 - step)) < tol and abs(r - step) < tol
+    assert np.all(y[0] == 0) and y[1, 1] == 0 and y[1, 2] == 0
+
+
+def test_implicit_pass_zero_stiffness_returns_to_anchor(orc):
+    mesh = dict(edges=np.array([[0, 1], [1, 2]], np.int32), l0=np.array([1.0, 1.0]))
+    rng = np.random.default_rng(0)
+    ya = rng.normal(size=(3, 3))
+    y = ya + rng.normal(size=(3, 3)) * 0.1
+    orc.implicit_pass(mesh, ya, y, np.ones(3), np.ones(3), 0.0)
+    np.testing.assert_allclose(y, ya, atol=1e-15)
+
+
+def test_implicit_solve_reaches_a_stationary_point_of_the_energy(slx, orc):
+    """Iterated to a tiny residual, the iterate zeroes the gradient of
+    E(y) = sum_v |y_v - ya_v|^2 / (2 w_v) + sum_e kappa C_e(y)^2 / 2 over free
+    vertices (checked by central differences of E, independent of the step)."""
+    m, d = grid_scene(slx, n=6, jitter=2e-2, seed=3)
+    rng = np.random.default_rng(1)
+    x0 = d["x"].astype(np.float64)
+    ya = x0 + rng.normal(0, 0.02, x0.shape)
+    y = ya.copy()
+    fixed = np.ones(len(x0))
+    fixed[0] = 0
+    w = d["m_inv"].astype(np.float64)
+    kappa = 40.0
+    mesh = dict(edges=d["edges"], l0=d["l0"].astype(np.float64))
+    for _ in range(20000):
+        r = orc.implicit_pass(mesh, ya, y, fixed, w, kappa)
+        if r < 1e-13:
+            break
+    assert r < 1e-13
+    E, l0 = d["edges"], d["l0"].astype(np.float64)
+
+    def energy(z):
+        free = fixed > 0
+        e = np.sum(np.sum((z - ya) ** 2, axis=1)[free] / (2 * w[free]))
+        c = np.linalg.norm(z[E[:, 0]] - z[E[:, 1]], axis=1) - l0
+        return e + 0.5 * kappa * np.sum(c ** 2)
+    h = 1e-6
+    g = []
+    for v in range(1, len(y)):
+        for c in range(3):
+            zp, zm = y.copy(), y.copy()
+            zp[v, c] += h
+            zm[v, c] -= h
+            g.append((energy(zp) - energy(zm)) / (2 * h))
+    scale = np.abs(ya - x0).max() / w.max()
+    assert np.abs(g).max() < 1e-6 * max(scale, 1.0) + 1e-7
