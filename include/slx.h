@@ -175,7 +175,13 @@ size_t slx_tri_build_ws_bytes(int nf);
 int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6,
                   int32_t* codes, int32_t* sorted_faces, int32_t* child_a, int32_t* child_b,
                   int32_t* parent, int32_t* range_l, int32_t* range_r, int32_t* sched,
-                  float* nlo4, float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream);
+                  float* nlo4, float* nhi4, float* quad4, void* ws, size_t ws_bytes,
+                  slx_stream_t stream);
+/* quad4 (nullable): the triangle tree's 4-wide node array, float4 [8 (nf-1)], kept
+ * by every build and refit: internal node p holds its four grandchildren's boxes
+ * (left-left, left-right, right-left, right-right; a leaf child in the first slot
+ * of its pair with the other slot empty) as (lo, node) (hi, face or -1).  The
+ * traversal then tests two levels per load; candidate lists are unchanged. */
 /* Refit of the triangle LBVH (topology and schedule kept): leaf k gets the padded
  * box of face sorted_faces[k] computed from x4 (the face boxes of
  * computeAABB_faces, meshTaichiWrapper.py:350-361, fused in; fmin4/fmax4 are not
@@ -183,7 +189,7 @@ int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* b
 int slx_tri_refit(int nf, const int32_t* faces, const float* x4, float padding,
                   const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b,
                   const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4,
-                  slx_stream_t stream);
+                  float* quad4, slx_stream_t stream);
 
 /* traverse_cell_bvh_single, lbvh_cell.py:655-714, for nq query points p -+ padding
  * (p = xyz of q4).  mode 0 = cell tree (queries whose centre cell id exceeds 32767
@@ -192,8 +198,9 @@ int slx_tri_refit(int nf, const int32_t* faces, const float* x4, float padding,
  * status bit (the reference writes past its 40-entry cache). */
 int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves,
                  const int32_t* num, const int32_t* sorted_faces, const float* nlo4,
-                 const float* nhi4, const float* fmin4, const float* fmax4, const float* box6,
-                 int cap, int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
+                 const float* nhi4, const float* quad4, const float* fmin4, const float* fmax4,
+                 const float* box6, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
+                 slx_stream_t stream);
 
 /* Batched instances (no reference counterpart beyond the mesh concatenation of
  * framework/meshio/concat.py:11-58): instances packed CSR-style (vert_base,
@@ -208,11 +215,11 @@ int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float*
                   float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
                   int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
                   int32_t* range_l, int32_t* range_r, int32_t* roots, int32_t* sched, float* nlo4,
-                  float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream);
+                  float* nhi4, float* quad4, void* ws, size_t ws_bytes, slx_stream_t stream);
 /* Traversal of a batched tree: query q starts at roots[q_inst[q]]. */
 int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots,
-                     float padding, int nf, const float* nlo4, const float* nhi4, int cap,
-                     int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
+                     float padding, int nf, const float* nlo4, const float* nhi4, const float* quad4,
+                     int cap, int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream);
 
 /* ---------------- narrow phase (collision_constraints_x.py, distance.py) ---------------- */
 
@@ -338,6 +345,7 @@ typedef struct slx_tree {
   int32_t* sched;            /* refit schedule, slx_refit_sched_ints(leaves) (cell: of the static topology) */
   const int32_t* vert_order; /* [nv] order in which this tree's mesh's vertices are processed as queries
                                 (nullable; e.g. Morton order, so that a wavefront's queries are neighbours) */
+  float* quad4;              /* triangle: 4-wide node array, float4 [8 (nf-1)] (nullable, see below) */
 } slx_tree;
 
 typedef struct slx_frame_desc {

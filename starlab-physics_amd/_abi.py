@@ -48,10 +48,10 @@ SIGNATURES = {
     "slx_cell_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
                        PTR, PTR, C.c_size_t, STREAM],
     "slx_tri_build_ws_bytes": [I32],
-    "slx_tri_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t,
-                      STREAM],
-    "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
-    "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_tri_build": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
+                      C.c_size_t, STREAM],
+    "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
+    "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
                            PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
@@ -59,8 +59,8 @@ SIGNATURES = {
     "slx_frame": [PTR, STREAM],
     "slx_seg_build_ws_bytes": [I32, I32],
     "slx_seg_build": [I32, I32, I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
-                      PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
-    "slx_traverse_seg": [I32, PTR, PTR, PTR, F32, I32, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+                      PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
+    "slx_traverse_seg": [I32, PTR, PTR, PTR, F32, I32, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_mlp_num_params": [],
     "slx_features": [I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, PTR, STREAM],
     "slx_warm_start": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, PTR, PTR, PTR, PTR, F32, STREAM],
@@ -89,7 +89,7 @@ class Tree(C.Structure):
                 ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR), ("n_inst", I32), ("vert_base", PTR),
                 ("face_base", PTR), ("face_inst", PTR), ("vert_inst", PTR), ("sorted_inst", PTR),
                 ("range_l", PTR), ("range_r", PTR), ("roots", PTR), ("sched", PTR),
-                ("vert_order", PTR)]
+                ("vert_order", PTR), ("quad4", PTR)]
 
 
 class FrameDesc(C.Structure):

@@ -77,6 +77,8 @@ class DeviceTree:
             ws = _abi.lib().slx_tri_build_ws_bytes(nf)
         self.nlo4 = torch.zeros((nn, 4), **f32)
         self.nhi4 = torch.zeros((nn, 4), **f32)
+        # triangle trees: 4-wide node array (grandchild boxes of every internal node)
+        self.quad4 = torch.zeros((max(nf - 1, 1) * 8, 4), **f32) if mode == "tri" else None
         # batched instances: one subtree per instance (slx_seg_build)
         self.inst = getattr(mesh, "inst", None)
         self.n_inst = 0
@@ -100,6 +102,7 @@ class DeviceTree:
                       P(self.num), P(self.face_cell), P(self.prefix), P(self.leaf_l), P(self.leaf_r),
                       P(self.nlo4), P(self.nhi4))
         t.range_l, t.range_r, t.sched = P(self.range_l), P(self.range_r), P(self.sched)
+        t.quad4 = P(self.quad4)
         t.vert_order = P(getattr(self.mesh, "vert_order", None))
         if self.n_inst > 1:
             I = self.inst
@@ -131,7 +134,8 @@ class DeviceTree:
                       float(padding), P(I["vert_base"]), P(I["face_base"]), P(I["face_inst"]), P(self.fmin4),
                       P(self.fmax4), P(self.box6), P(self.codes), P(self.sorted_faces), P(self.sorted_inst),
                       P(self.child_a), P(self.child_b), P(self.parent), P(self.range_l), P(self.range_r),
-                      P(self.roots), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
+                      P(self.roots), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.quad4), P(self.ws),
+                      self.ws.numel(), s)
             return
         self.face_aabbs(padding, x4)
         self.mesh_aabb(padding, x4)
@@ -143,7 +147,8 @@ class DeviceTree:
         else:
             _abi.call("slx_tri_build", self.nf, P(self.fmin4), P(self.fmax4), P(self.box6), P(self.codes),
                       P(self.sorted_faces), P(self.child_a), P(self.child_b), P(self.parent), P(self.range_l),
-                      P(self.range_r), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.ws), self.ws.numel(), s)
+                      P(self.range_r), P(self.sched), P(self.nlo4), P(self.nhi4), P(self.quad4), P(self.ws),
+                      self.ws.numel(), s)
 
     def refit(self, padding, x4=None):
         """Triangle tree: node boxes from new positions, same topology (the face
@@ -154,7 +159,7 @@ class DeviceTree:
         x4 = self.mesh.x4 if x4 is None else x4
         _abi.call("slx_tri_refit", self.nf, P(self.faces), P(x4), float(padding), P(self.sorted_faces),
                   P(self.child_a), P(self.child_b), P(self.parent), P(self.sched), P(self.nlo4), P(self.nhi4),
-                  _abi.stream_handle())
+                  P(self.quad4), _abi.stream_handle())
 
     def query(self, q4, padding, cap, cand=None, cand_n=None, status=None, q_inst=None):
         """Candidate faces of boxes q4.xyz -+ padding, column-major (cap, nq)."""
@@ -170,10 +175,11 @@ class DeviceTree:
             if q_inst is None:
                 raise ValueError("batched tree queries need q_inst (instance of each query)")
             _abi.call("slx_traverse_seg", nq, P(q4), P(q_inst), P(self.roots), float(padding), self.nf,
-                      P(self.nlo4), P(self.nhi4), cap, P(cand), P(cand_n), P(st), _abi.stream_handle())
+                      P(self.nlo4), P(self.nhi4), P(self.quad4), cap, P(cand), P(cand_n), P(st),
+                      _abi.stream_handle())
             return cand, cand_n
         _abi.call("slx_traverse", 0 if self.mode == "cell" else 1, nq, P(q4), float(padding),
                   self.n_leaves if self.nf > 0 else 0, P(self.num), P(self.sorted_faces), P(self.nlo4),
-                  P(self.nhi4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n), P(st),
-                  _abi.stream_handle())
+                  P(self.nhi4), P(self.quad4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n),
+                  P(st), _abi.stream_handle())
         return cand, cand_n

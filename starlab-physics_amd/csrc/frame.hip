@@ -2,7 +2,8 @@
 // the sequence of Solver.forward (framework/physics/XPBD.py:443-469).
 //
 //  if collision: rebuild the dynamic tree from x (and the static tree when asked)
-//  per substep:  [triangle tree: refit from x]
+//  per substep:  [triangle tree: refit from x, on a side stream, joined before
+//                 the first query of the dynamic tree]
 //                spring pass with compute_y fused in
 //                per iteration: [spring pass] broad phase (3 query sets),
 //                               collision pass
@@ -28,6 +29,29 @@ using namespace slx;
 
 bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
 
+// A second stream per device for the per-substep refit, which depends only on x
+// and so runs beside the spring pass and the static-tree queries (fork/join with
+// events, also under graph capture).  Default priority.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+SideStream g_side[64];
+
+int side_stream(SideStream** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(SLX_EHIP, "slx_frame: no device");
+  SideStream& ss = g_side[dev];
+  if (!ss.s) {
+    if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess)
+      return fail(SLX_EHIP, "slx_frame: side stream creation failed");
+  }
+  *out = &ss;
+  return SLX_OK;
+}
+
 size_t tree_ws(const slx_tree& t) {
   if (t.nf <= 0) return 0;
   if (batched(t)) return slx_seg_build_ws_bytes(t.nf, t.n_inst);
@@ -40,7 +64,7 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
   if (batched(t))
     return slx_seg_build(nv, t.nf, t.n_inst, faces, x4, pad, t.vert_base, t.face_base, t.face_inst, t.fmin4, t.fmax4,
                          t.box6, t.codes, t.sorted_faces, t.sorted_inst, t.child_a, t.child_b, t.parent, t.range_l,
-                         t.range_r, t.roots, t.sched, t.nlo4, t.nhi4, ws, ws_bytes, s);
+                         t.range_r, t.roots, t.sched, t.nlo4, t.nhi4, t.quad4, ws, ws_bytes, s);
   int rc = slx_face_aabbs(t.nf, faces, x4, pad, t.fmin4, t.fmax4, s);
   if (rc) return rc;
   rc = slx_mesh_aabb(nv, x4, pad, t.box6, ws, ws_bytes, s);
@@ -49,7 +73,7 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
     return slx_cell_build(t.nf, t.fmin4, t.fmax4, t.box6, t.child_a, t.child_b, t.parent, t.sched, t.face_cell,
                           t.prefix, t.sorted_faces, t.num, t.leaf_l, t.leaf_r, t.nlo4, t.nhi4, status, ws, ws_bytes, s);
   return slx_tri_build(t.nf, t.fmin4, t.fmax4, t.box6, t.codes, t.sorted_faces, t.child_a, t.child_b, t.parent,
-                       t.range_l, t.range_r, t.sched, t.nlo4, t.nhi4, ws, ws_bytes, s);
+                       t.range_l, t.range_r, t.sched, t.nlo4, t.nhi4, t.quad4, ws, ws_bytes, s);
 }
 
 // queries of a mesh whose vertices belong to instances q_inst (batched trees only)
@@ -60,9 +84,9 @@ int query(const char* name, const slx_tree& t, int nq, const float* q4, const in
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
   const bool seg = batched(t);
-  return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4, t.fmin4, t.fmax4,
-                         t.box6, cap, cand, cand_n, status, (hipStream_t)s, seg ? t.roots : nullptr,
-                         seg ? q_inst : nullptr, q_order);
+  return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
+                         t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,
+                         (hipStream_t)s, seg ? t.roots : nullptr, seg ? q_inst : nullptr, q_order);
 }
 
 }  // namespace
@@ -129,13 +153,31 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   float4* ycur = (float4*)d->y4;
   float4* yalt = (float4*)d->y4_alt;
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
+  SideStream* side = nullptr;
+  const bool refits = coll && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0 && d->n_substeps > 1;
+  if (refits) {
+    rc = side_stream(&side);
+    if (rc) return rc;
+  }
   for (int s = 0; s < d->n_substeps; ++s) {
-    if (coll && s > 0 && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0) {
+    bool refit_pending = false;
+    if (refits && s > 0) {
+      // fork: the refit reads x as the previous substep left it
+      if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
+        return fail(SLX_EHIP, "slx_frame: fork failed");
       rc = slx_tri_refit(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
-                         d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, stream);
+                         d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4,
+                         (slx_stream_t)side->s);
       if (rc) return rc;
+      if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
+      refit_pending = true;
       tv_fresh = false;
     }
+    auto join_refit = [&]() -> int {
+      if (!refit_pending) return SLX_OK;
+      refit_pending = false;
+      return hipStreamWaitEvent(st, side->join, 0) == hipSuccess ? SLX_OK : fail(SLX_EHIP, "slx_frame: join failed");
+    };
     if (d->method == 1) {
       rc = slx_compute_y(nv, d->x4, d->v4, (float*)ycur, d->dt, d->gx, d->gy, d->gz, stream);
       if (rc) return rc;
@@ -160,6 +202,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                    n0, d->status, stream);
         if (rc) return rc;
       }
+      rc = join_refit();                     // the dynamic tree is needed from here on
+      if (rc) return rc;
       rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
                  n1, d->status, stream);
       if (rc) return rc;
@@ -175,6 +219,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                               d->rec_cap, d->status, d->ws, d->ws_bytes, stream);
       if (rc) return rc;
     }
+    rc = join_refit();                       // (n_iters == 0: nothing queried the tree)
+    if (rc) return rc;
     rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
     if (rc) return rc;
   }

@@ -28,6 +28,9 @@ int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* to
 #ifndef SLX_TRAV_WAVE
 #define SLX_TRAV_WAVE 1   // wavefront-coherent triangle traversal (0: one walk per lane)
 #endif
+#ifndef SLX_TRAV_QUAD
+#define SLX_TRAV_QUAD 1   // 4-wide node walk when the tree has its quad array (0: binary walk)
+#endif
 #ifndef SLX_TRAV_STACK
 #define SLX_TRAV_STACK 64
 #endif
@@ -365,6 +368,25 @@ __device__ __forceinline__ void store_node(int j, const NodeBox& b, int wl, int 
   if (COUNTED) num[j] = b.num;
 }
 
+// 4-wide node array of a triangle tree (float4 quad[8 * internal node], one
+// 128-byte line per node): slot 2*side(child) + side(grandchild) of internal node p
+// holds the box of p's grandchild as (lo, node index) (hi, face id or -1); when
+// p's child is a leaf, its pair of slots holds (that leaf, empty).  Slots 0..3 are
+// (left-left, left-right, right-left, right-right): pushed in order and popped
+// last-first they visit leaves right-first, the reference's order.  Each internal
+// node writes its own line when it is refit.
+__device__ __forceinline__ float4 slot_lo(const NodeBox& b, int node) {
+  return make_float4(b.lo.x, b.lo.y, b.lo.z, __int_as_float(node));
+}
+__device__ __forceinline__ float4 slot_hi(const NodeBox& b, int face) {
+  return make_float4(b.hi.x, b.hi.y, b.hi.z, __int_as_float(face));
+}
+__device__ __forceinline__ void empty_slot(float4* d) {
+  const float inf = __int_as_float(0x7f800000);
+  d[0] = make_float4(inf, inf, inf, __int_as_float(-1));      // overlaps nothing
+  d[1] = make_float4(-inf, -inf, -inf, __int_as_float(-1));
+}
+
 // where leaf boxes come from
 enum LeafKind { kLeafFaceBoxes = 0, kLeafVertices = 1, kLeafCells = 2 };
 struct LeafSrc {
@@ -378,6 +400,7 @@ struct LeafSrc {
   const int32_t* prefix;
   int32_t* leaf_l;
   int32_t* leaf_r;
+  float4* quad;             // triangle trees: 4-wide node array (nullable)
 };
 
 // Leaf k: its box, written to the node arrays as well.  Cell leaves follow
@@ -435,7 +458,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
   __shared__ NodeBox s_leaf[kSubMax];
   __shared__ NodeBox s_int[kSubMax];
   __shared__ int s_flag[kSubMax];
-  __shared__ int s_ca[kSubMax], s_cb[kSubMax], s_par[kSubMax], s_lpar[kSubMax];
+  __shared__ int s_ca[kSubMax], s_cb[kSubMax], s_par[kSubMax], s_lface[kSubMax];
   const Sched S{sched, n};
   const int n_sub = S.n_sub();
   for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
@@ -443,7 +466,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
     for (int k = threadIdx.x; k < cnt; k += kSubBS) {
       s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
-      s_lpar[k] = parent[n - 1 + first + k];
+      if constexpr (!COUNTED) s_lface[k] = src.sorted[first + k];
       s_flag[k] = 0;
       if (first + k < n - 1) {          // internal node first+k (those outside the subtree are never read)
         s_ca[k] = ca[first + k];
@@ -454,7 +477,8 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     __syncthreads();
     for (int k = threadIdx.x; k < cnt; k += kSubBS) {
       int node = n - 1 + first + k;
-      int p = s_lpar[k];
+      if (node == root) continue;
+      int p = parent[node];
       while (node != root) {               // p is inside the subtree: first <= p <= last
         __threadfence_block();
         if (atomicAdd(&s_flag[p - first], 1) == 0) break;
@@ -462,11 +486,37 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
         const int l = s_ca[p - first], r = s_cb[p - first];
         const NodeBox A = l >= n - 1 ? s_leaf[l - (n - 1) - first] : s_int[l - first];
         const NodeBox B = r >= n - 1 ? s_leaf[r - (n - 1) - first] : s_int[r - first];
-        const NodeBox P = unite<COUNTED>(A, B);
-        s_int[p - first] = P;
-        store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
+        s_int[p - first] = unite<COUNTED>(A, B);
         node = p;
         p = s_par[p - first];
+      }
+    }
+    __syncthreads();
+    // the subtree's internal nodes (both children arrived: flag 2) to global memory,
+    // coalesced: node boxes one node per thread, quad lines 8 threads per line
+    for (int k = threadIdx.x; k < cnt; k += kSubBS)
+      if (s_flag[k] == 2) store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
+    if constexpr (!COUNTED) {
+      if (src.quad) {
+        auto box_of = [&](int g) -> const NodeBox& {
+          return g >= n - 1 ? s_leaf[g - (n - 1) - first] : s_int[g - first];
+        };
+        for (int t = threadIdx.x; t < 8 * cnt; t += kSubBS) {
+          const int k = t >> 3, j = t & 7;
+          if (s_flag[k] != 2) continue;
+          const int slot = j >> 1, half = j & 1;
+          const int c = slot < 2 ? s_ca[k] : s_cb[k];
+          float4 v;
+          if (c >= n - 1 && (slot & 1)) {            // a leaf child's companion slot: empty
+            const float inf = __int_as_float(0x7f800000);
+            v = half ? make_float4(-inf, -inf, -inf, __int_as_float(-1)) : make_float4(inf, inf, inf, __int_as_float(-1));
+          } else {
+            const int g = c >= n - 1 ? c : ((slot & 1) ? s_cb[c - first] : s_ca[c - first]);
+            const NodeBox& B = box_of(g);
+            v = half ? slot_hi(B, g >= n - 1 ? s_lface[g - (n - 1) - first] : -1) : slot_lo(B, g);
+          }
+          src.quad[8 * (size_t)(first + k) + j] = v;
+        }
       }
     }
     __syncthreads();
@@ -481,7 +531,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
 template <bool COUNTED>
 __global__ void __launch_bounds__(kTopBS)
 k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
-            const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi) {
+            const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
   __shared__ NodeBox s_cell[kTopMax][2];
   __shared__ int s_flag[kTopMax];
   __shared__ int s_up[kTopMax];
@@ -538,6 +588,31 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
       if (up >= 0 && (up >> 1) < nl) s_cell[up >> 1][up & 1] = P;
     }
   }
+  if constexpr (!COUNTED) {
+    if (quad) {                            // quad lines of the top nodes, once every box is final
+      __syncthreads();
+      auto grand = [&](float4* d, int g) {
+        const float4 lo = nlo[g], hi = nhi[g];
+        d[0] = make_float4(lo.x, lo.y, lo.z, __int_as_float(g));
+        d[1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(g >= n - 1 ? __float_as_int(lo.w) : -1));
+      };
+      for (int i = threadIdx.x; i < n_top; i += kTopBS) {
+        const int p = S.top_node(i);
+        float4* qp = quad + 8 * (size_t)p;
+        const int c2[2] = {ca[p], cb[p]};
+        for (int h = 0; h < 2; ++h) {
+          const int c = c2[h];
+          if (c >= n - 1) {
+            grand(qp + 4 * h, c);
+            empty_slot(qp + 4 * h + 2);
+          } else {
+            grand(qp + 4 * h, __float_as_int(nlo[c].w));
+            grand(qp + 4 * h + 2, __float_as_int(nhi[c].w));
+          }
+        }
+      }
+    }
+  }
 }
 
 size_t refit_sched_ints(int n) { return 8 + 8 * (size_t)(n > 0 ? n : 1); }
@@ -556,7 +631,8 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
   constexpr bool COUNTED = LEAF == kLeafCells;
   const int grid = n < kSubGrid ? n : kSubGrid;
   SLX_LAUNCH(name, k_refit_sub<LEAF>, dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb, parent, num, nlo, nhi);
-  SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi);
+  SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
+             src.quad);
   return check_launch(name);
 }
 
@@ -784,7 +860,7 @@ extern "C" size_t slx_tri_build_ws_bytes(int nf) {
 
 extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, float padding,
                              const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b,
-                             const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4,
+                             const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4, float* quad4,
                              slx_stream_t stream) {
   SLX_REQUIRE(nf >= 0, "slx_tri_refit: nf < 0");
   if (nf == 0) return SLX_OK;
@@ -795,14 +871,15 @@ extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, floa
   src.x4 = (const float4*)x4;
   src.pad = padding;
   src.sorted = sorted_faces;
+  src.quad = (float4*)quad4;
   return refit<kLeafVertices>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
                               (float4*)nhi4, S(stream));
 }
 
 extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6, int32_t* codes,
                              int32_t* sorted_faces, int32_t* child_a, int32_t* child_b, int32_t* parent,
-                             int32_t* range_l, int32_t* range_r, int32_t* sched, float* nlo4, float* nhi4, void* ws,
-                             size_t ws_bytes, slx_stream_t stream) {
+                             int32_t* range_l, int32_t* range_r, int32_t* sched, float* nlo4, float* nhi4,
+                             float* quad4, void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nf >= 0, "slx_tri_build: nf < 0");
   if (nf == 0) return SLX_OK;
   SLX_REQUIRE(fmin4 && fmax4 && box6 && codes && sorted_faces && child_a && child_b && parent && range_l && range_r &&
@@ -825,6 +902,7 @@ extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, con
   src.fmin = (const float4*)fmin4;
   src.fmax = (const float4*)fmax4;
   src.sorted = sorted_faces;
+  src.quad = (float4*)quad4;
   return refit<kLeafFaceBoxes>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
                                (float4*)nhi4, st);
 }
@@ -999,10 +1077,79 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, co
 }  // namespace slx
 
 namespace slx {
+
+// Triangle-tree walk over the 4-wide node array: a popped node's four grandchild
+// slots (one 128-byte line) are tested together, so one dependent load covers two
+// tree levels.  Overlapping slots are pushed in slot order (a leaf as an emit
+// entry), so they pop right-first: the leaves come out in the reference's order
+// (the right subtree before the left, i.e. in decreasing leaf position), and the
+// set of emitted leaves is the same, since a grandchild overlaps the query box only
+// if its parent does (a parent's box is the union of its children's).
+__global__ void __launch_bounds__(kTravBS)
+k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+                const float4* __restrict__ nhi, const float4* __restrict__ quad, int cap, int32_t* __restrict__ cand,
+                int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ roots,
+                const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order) {
+  __shared__ uint32_t s_stack[kStack * kTravBS];
+  const int t = blockIdx.x * kTravBS + threadIdx.x;
+  if (t >= nq) return;
+  const int q = order ? order[t] : t;
+  uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
+  const float3 p = xyz(q4[q]);
+  const float pp = pad * 1.0f;
+  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  int cnt = 0, flags = 0;
+  const int root = roots ? roots[q_inst[q]] : 0;
+  if (nl <= 0 || root < 0) {
+    cand_n[q] = 0;
+    return;
+  }
+  auto emit = [&](int f) {
+    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
+    else flags |= kStatusCandOverflow;
+    ++cnt;
+  };
+  int sp = 0;
+  {
+    const float4 lo = nlo[root], hi = nhi[root];
+    if (overlap(a, b, xyz(lo), xyz(hi))) {
+      if (root >= nl - 1) emit(__float_as_int(lo.w));
+      else stk[(sp++) * kTravBS] = (uint32_t)root;
+    }
+  }
+  while (sp > 0) {
+    const uint32_t e = stk[(--sp) * kTravBS];
+    if (e & kEmit) {
+      emit((int)(e & kLow30));
+      continue;
+    }
+    const float4* qn = quad + 8 * (size_t)e;
+    float4 lo[4], hi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = qn[2 * k];
+      hi[k] = qn[2 * k + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!overlap(a, b, xyz(lo[k]), xyz(hi[k]))) continue;
+      const int face = __float_as_int(hi[k].w);
+      const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)__float_as_int(lo[k].w);
+      if (sp < kStack) stk[(sp++) * kTravBS] = w;
+      else flags |= kStatusStackOverflow;
+    }
+  }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (flags && status) atomicOr(status, flags);
+}
+
+}  // namespace slx
+
+namespace slx {
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
-                    const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
-                    int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
+                    const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
+                    int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
                     const int32_t* q_inst, const int32_t* order) {
   SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
@@ -1020,10 +1167,16 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   // leaves than queries: the lanes of a wavefront then overlap the same few
   // leaves and the shared walk is short).  Batched trees: lanes have different
   // roots.  The candidate lists are the same either way.
-  else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq)
-    SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
-               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots, q_inst,
-               order);
+  else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq) {
+    if (quad4 && SLX_TRAV_QUAD)
+      SLX_LAUNCH(name, k_traverse_quad, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
+                 padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n,
+                 status, roots, q_inst, order);
+    else
+      SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
+                 padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots, q_inst,
+                 order);
+  }
   else
     SLX_LAUNCH(name, k_traverse_tri_wave, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
                padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order);
@@ -1032,12 +1185,12 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
 }  // namespace slx
 
 extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
-                            const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* fmin4,
-                            const float* fmax4, const float* box6, int cap, int32_t* cand, int32_t* cand_n,
-                            int32_t* status, slx_stream_t stream) {
+                            const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* quad4,
+                            const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
+                            int32_t* cand_n, int32_t* status, slx_stream_t stream) {
   return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
-                         sorted_faces, nlo4, nhi4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream), nullptr,
-                         nullptr, nullptr);
+                         sorted_faces, nlo4, nhi4, quad4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream),
+                         nullptr, nullptr, nullptr);
 }
 
 // ===================== batched instances: one tree, one subtree per instance =====================
@@ -1188,7 +1341,7 @@ extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, c
                              float* fmin4, float* fmax4, float* box6, int32_t* codes, int32_t* sorted_faces,
                              int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
                              int32_t* range_l, int32_t* range_r, int32_t* roots, int32_t* sched, float* nlo4,
-                             float* nhi4, void* ws, size_t ws_bytes, slx_stream_t stream) {
+                             float* nhi4, float* quad4, void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nv >= 0 && nf >= 0 && n_inst >= 1 && n_inst <= (1 << kSegInstBits), "slx_seg_build: bad sizes");
   SLX_REQUIRE(nf < (1 << kSegIndexBits), "slx_seg_build: too many faces for 24-bit indices");
   if (nf == 0) return SLX_OK;
@@ -1231,14 +1384,15 @@ extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, c
   src.fmin = (const float4*)fmin4;
   src.fmax = (const float4*)fmax4;
   src.sorted = sorted_faces;
+  src.quad = (float4*)quad4;
   return refit<kLeafFaceBoxes>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
                                (float4*)nhi4, st);
 }
 
 extern "C" int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots, float padding,
-                                int nf, const float* nlo4, const float* nhi4, int cap, int32_t* cand, int32_t* cand_n,
-                                int32_t* status, slx_stream_t stream) {
+                                int nf, const float* nlo4, const float* nhi4, const float* quad4, int cap,
+                                int32_t* cand, int32_t* cand_n, int32_t* status, slx_stream_t stream) {
   SLX_REQUIRE(roots && q_inst, "slx_traverse_seg: roots and q_inst required");
-  return traverse_launch("traverse_seg", 1, nq, q4, padding, nf, nullptr, nullptr, nlo4, nhi4, nullptr, nullptr,
-                         nullptr, cap, cand, cand_n, status, S(stream), roots, q_inst, nullptr);
+  return traverse_launch("traverse_seg", 1, nq, q4, padding, nf, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
+                         nullptr, nullptr, cap, cand, cand_n, status, S(stream), roots, q_inst, nullptr);
 }

@@ -109,16 +109,14 @@ struct Contact {
 
 // One candidate of query set `set` (0 vt_st, 1 vt_dy, 2 tv_st), slots in order
 // (collision_constraints_x.py:5-310).
-__device__ __forceinline__ void contact(int set, int qv, int f, const int32_t* fdy, const int32_t* fst,
-                                        const float4* y4, const float4* xs4, const float* fixed,
-                                        const float* m_inv, float dhat, float k, Contact& c) {
+__device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* y4, const float4* xs4,
+                                        const float* fixed, const float* m_inv, float dhat, float k, Contact& c) {
   c.mask = 0;
   c.record = false;
-  const int32_t* F = set == 0 ? fst : fdy;
   c.vid[0] = qv;
-  c.vid[1] = F[3 * f];
-  c.vid[2] = F[3 * f + 1];
-  c.vid[3] = F[3 * f + 2];
+  c.vid[1] = fv.x;
+  c.vid[2] = fv.y;
+  c.vid[3] = fv.z;
   const float4* tp = set == 0 ? xs4 : y4;
   const float3 p = xyz(set == 2 ? xs4[qv] : y4[qv]);
   const float3 t0 = xyz(tp[c.vid[1]]), t1 = xyz(tp[c.vid[2]]), t2 = xyz(tp[c.vid[3]]);
@@ -203,30 +201,42 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
   bool over = false;
   float3 dx = make_float3(0.f, 0.f, 0.f);
   if (query_of(Q, qg, set, qv, cand, ncand, nq)) {
-    for (int j = 0; j < ncand; ++j) {
-      const int f = cand[(size_t)j * nq + qv];
-      if (set == 1 && (fdy[3 * f] == qv || fdy[3 * f + 1] == qv || fdy[3 * f + 2] == qv)) continue;
-      Contact c;
-      contact(set, qv, f, fdy, fst, y4, xs4, fixed, m_inv, dhat, k, c);
-      pairs += c.record ? 1 : 0;
-      if (set == 0) {
-        if (c.mask) {
-          dx = make_float3(dx.x + c.delta[0].x, dx.y + c.delta[0].y, dx.z + c.delta[0].z);
-          ++own;
-        }
-        continue;
-      }
-      const uint32_t o0 = ((uint32_t)(qg - Q.nv) * (uint32_t)Q.cap + (uint32_t)j) * 4u;
+    const int32_t* F = set == 0 ? fst : fdy;
+    // candidates in groups of four: all ids and corner ids are loaded before the
+    // group is processed, in candidate order
+    for (int j0 = 0; j0 < ncand; j0 += 4) {
+      int3 fv[4];
 #pragma unroll
-      for (int sl = 0; sl < 4; ++sl) {
-        if (!(c.mask & (1u << sl))) continue;
-        const int t = c.vid[sl];
-        const int at = atomicAdd(&bcnt[t], 1);
-        if (at < bucket)
-          brec[(size_t)t * bucket + at] = make_float4(c.delta[sl].x, c.delta[sl].y, c.delta[sl].z,
-                                                      __uint_as_float(o0 + (uint32_t)sl));
-        else
-          over = true;
+      for (int g = 0; g < 4; ++g) {
+        const int f = j0 + g < ncand ? cand[(size_t)(j0 + g) * nq + qv] : -1;
+        fv[g] = f >= 0 ? make_int3(F[3 * f], F[3 * f + 1], F[3 * f + 2]) : make_int3(-1, -1, -1);
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (j0 + g >= ncand) break;
+        if (set == 1 && (fv[g].x == qv || fv[g].y == qv || fv[g].z == qv)) continue;
+        Contact c;
+        contact(set, qv, fv[g], y4, xs4, fixed, m_inv, dhat, k, c);
+        pairs += c.record ? 1 : 0;
+        if (set == 0) {
+          if (c.mask) {
+            dx = make_float3(dx.x + c.delta[0].x, dx.y + c.delta[0].y, dx.z + c.delta[0].z);
+            ++own;
+          }
+          continue;
+        }
+        const uint32_t o0 = ((uint32_t)(qg - Q.nv) * (uint32_t)Q.cap + (uint32_t)(j0 + g)) * 4u;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+          if (!(c.mask & (1u << sl))) continue;
+          const int t = c.vid[sl];
+          const int at = atomicAdd(&bcnt[t], 1);
+          if (at < bucket)
+            brec[(size_t)t * bucket + at] = make_float4(c.delta[sl].x, c.delta[sl].y, c.delta[sl].z,
+                                                        __uint_as_float(o0 + (uint32_t)sl));
+          else
+            over = true;
+        }
       }
     }
   }

@@ -68,8 +68,8 @@ int gs_pass(int nv, const float4* x4, const float* fixed, float4* y4, const int3
 // stack traversal launcher (lbvh.hip); name is the kernel timer label
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
-                    const float* fmin4, const float* fmax4, const float* box6, int cap, int32_t* cand,
-                    int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
+                    const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
+                    int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
                     const int32_t* q_inst = nullptr, const int32_t* order = nullptr);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled

@@ -41,7 +41,7 @@ def test_version_and_host_argument_checks(lib):
     assert "nv < 0" in lib.last_error()
     assert L.slx_spring_pass(4, None, None, None, None, None, None, None, 0.0, 0.0, None) == -1
     assert "null pointer" in lib.last_error()
-    assert L.slx_traverse(2, 1, None, 0.0, 0, None, None, None, None, None, None, None, 1, None, None,
+    assert L.slx_traverse(2, 1, None, 0.0, 0, None, None, None, None, None, None, None, None, 1, None, None,
                           None, None) == -1
     with pytest.raises(lib.SlxError):
         lib.call("slx_update_vx", 3, None, None, None, None, 0.0, 0.0, None)
