@@ -158,6 +158,14 @@ def main():
     avg_s = tot_ms / cnt / 1e3
     nbytes = rl.kernel_bytes(dom, counts)
     achieved = nbytes / avg_s / 1e9 if nbytes else None
+    traffic = None
+    # measured HBM bytes per launch of the dominant kernel (two rocprofv3 counter passes,
+    # calibrated on a copy kernel), valid for the default workload
+    tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
+    if collision and a.n == 1024 and a.substeps == 20 and a.iters == 2 and os.path.exists(tpath):
+        tk = json.load(open(tpath))["kernels"].get(dom)
+        if tk:
+            traffic = tk["fetch_bytes"] + tk["write_bytes"]
     cfg = (f"{a.n}x{a.n} cloth ({V} vertices, {F} triangles, {E} stretch + {B} bending constraints), "
            f"{a.substeps} substeps x {a.iters} Jacobi iterations per frame, ")
     cfg += ("terrain (142x142 height field, 39762 triangles) + self vertex-triangle collision, triangle LBVH "
@@ -178,7 +186,7 @@ def main():
         "config": {"workload": cfg, "substeps_per_step": a.substeps, "iters": a.iters, "collision": collision,
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": (achieved / 8000.0) if achieved else None, "traffic": None, "kernel": dom,
+                     "frac": (achieved / 8000.0) if achieved else None, "traffic": traffic, "kernel": dom,
                      "algorithmic_bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6},
         "step_roofline_frac": (rl.step_bytes(counts, a.iters, a.substeps, collision) * a.steps / el)
                               / rl.HBM_PEAK_BYTES_PER_S,

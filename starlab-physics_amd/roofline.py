@@ -9,8 +9,10 @@ candidates per query of a query set.
   predict                      40 V   (read x, v, f; write y)
   one Jacobi spring pass       12 (E+B) + 40 V   (ids + rest length; read y, x, f*w; write y)
   velocity / position update   52 V   (read x, y, f; write v, x)
-  refit (face boxes fused)     96 F + 16 V   (faces + leaf order 16 F, topology 16 F, x 16 V; write 2F-1 nodes)
-  traversal (nq queries, tree of Ft faces)   16 nq + 4 c nq + 32 (2 Ft - 1)
+  refit (face boxes fused)     224 F + 16 V  (faces + leaf order 16 F, topology 16 F, x 16 V; write 2F-1
+                                              nodes 64 F and the 4-wide node array 128 F)
+  traversal (nq queries, tree of Ft faces)   16 nq + 4 nq + 4 c nq + tree, tree = 128 (Ft - 1) + 32 for the
+                               4-wide node array, 32 (2 Ft - 1) for the binary node arrays
   narrow phase                 16 c V + 24 V  (per query set of V queries)
   contact apply                56 V   (counts, first-set sums, f; read/write y)
   per-frame rebuild            Morton 20 F + 12 V, radix sort 68 F, Karras 16 F
@@ -33,15 +35,16 @@ def update_vx(V):
 
 
 def refit(V, F):
-    return 96 * F + 16 * V
+    return 224 * F + 16 * V
 
 
 def contact_apply(V):
     return 56 * V
 
 
-def traversal(nq, Ft, c):
-    return 16 * nq + 4 * c * nq + 32 * (2 * Ft - 1)
+def traversal(nq, Ft, c, quad=True):
+    tree = 128 * max(Ft - 1, 0) + 32 if quad else 32 * (2 * Ft - 1)
+    return 16 * nq + 4 * nq + 4 * c * nq + tree
 
 
 def narrow(nq, c):
@@ -67,7 +70,8 @@ def kernel_bytes(name: str, n: dict):
         "tri_refit": refit(V, F),
     }
     if "Vs" in n:
-        table["traverse_vt_st"] = traversal(V, n["Fs"], c.get("vt_st", 0.0))
+        # the static (terrain) tree is coarse next to the queries: the wavefront walk over binary nodes
+        table["traverse_vt_st"] = traversal(V, n["Fs"], c.get("vt_st", 0.0), quad=False)
         table["traverse_tv_st"] = traversal(n["Vs"], F, c.get("tv_st", 0.0))
     table["traverse_vt_dy"] = traversal(V, F, c.get("vt_dy", 0.0))
     nar = narrow(V, c.get("vt_dy", 0.0))
@@ -87,7 +91,7 @@ def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
         c = n.get("cbar", {})
         per_iter = traversal(V, F, c.get("vt_dy", 0.0)) + narrow(V, c.get("vt_dy", 0.0)) + contact_apply(V)
         if "Vs" in n:
-            per_iter += traversal(V, n["Fs"], c.get("vt_st", 0.0)) + narrow(V, c.get("vt_st", 0.0))
+            per_iter += traversal(V, n["Fs"], c.get("vt_st", 0.0), quad=False) + narrow(V, c.get("vt_st", 0.0))
             per_iter += narrow(n["Vs"], c.get("tv_st", 0.0))
             total += substeps * traversal(n["Vs"], F, c.get("tv_st", 0.0))
         total += substeps * (refit(V, F) + n_iters * per_iter) + rebuild(V, F)
