@@ -3,7 +3,8 @@
 //
 //  if collision: rebuild the dynamic tree from x (and the static tree when asked)
 //  per substep:  [triangle tree: refit from x, on a side stream, joined before
-//                 the first query of the dynamic tree]
+//                 the first query of the dynamic tree; the static-tree queries
+//                 run on another side stream beside the dynamic-tree queries]
 //                spring pass with compute_y fused in
 //                per iteration: [spring pass] broad phase (3 query sets),
 //                               collision pass
@@ -29,12 +30,13 @@ using namespace slx;
 
 bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
 
-// A second stream per device for the per-substep refit, which depends only on x
-// and so runs beside the spring pass and the static-tree queries (fork/join with
-// events, also under graph capture).  Default priority.
+// Two extra streams per device (fork/join with events, also under graph capture;
+// default priority): s for the per-substep refit, which depends only on x and so
+// runs beside the spring pass; q for the static-tree queries, which run beside
+// the dynamic-tree queries.
 struct SideStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t s = nullptr, q = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr;
 };
 SideStream g_side[64];
 
@@ -44,8 +46,11 @@ int side_stream(SideStream** out) {
   SideStream& ss = g_side[dev];
   if (!ss.s) {
     if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ss.q, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.qfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess)
       return fail(SLX_EHIP, "slx_frame: side stream creation failed");
   }
   *out = &ss;
@@ -155,7 +160,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
   SideStream* side = nullptr;
   const bool refits = coll && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0 && d->n_substeps > 1;
-  if (refits) {
+  if (coll) {
     rc = side_stream(&side);
     if (rc) return rc;
   }
@@ -196,11 +201,16 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         if (rc) return rc;
       }
       if (!coll) continue;
-      // broad phase (XPBD.py:186-215)
+      // broad phase (XPBD.py:186-215); the static-tree queries on their own stream
+      bool static_pending = false;
       if (nvs > 0) {
+        if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
+          return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
-                   n0, d->status, stream);
+                   n0, d->status, (slx_stream_t)side->q);
         if (rc) return rc;
+        if (hipEventRecord(side->qjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
+        static_pending = true;
       }
       rc = join_refit();                     // the dynamic tree is needed from here on
       if (rc) return rc;
@@ -213,6 +223,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         if (rc) return rc;
         tv_fresh = true;
       }
+      if (static_pending && hipStreamWaitEvent(st, side->qjoin, 0) != hipSuccess)
+        return fail(SLX_EHIP, "slx_frame: join failed");
       // narrow phase + projection + update_dx (XPBD.py:236-264, 365-369)
       rc = slx_collision_pass(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                               d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
