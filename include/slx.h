@@ -45,6 +45,7 @@ extern "C" {
 #define SLX_STATUS_STACK_OVERFLOW 2  /* a traversal stack exceeded its depth */
 #define SLX_STATUS_CELL_RANGE 4      /* a face centre fell outside the 32^3 cell grid */
 #define SLX_STATUS_REC_OVERFLOW 8    /* a vertex received more than rec_cap contact deltas (dropped) */
+#define SLX_STATUS_PAIR_OVERFLOW 16  /* a query held more than pair_cap contacting pairs (dropped) */
 
 typedef void* slx_stream_t; /* hipStream_t */
 
@@ -229,15 +230,52 @@ int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32
  * accumulated per vertex in that serial order without float atomics; y4 updated
  * in place.  pair_num [2nv+nvs] (nullable) receives the recorded pair counts.
  * rec_cap = contact deltas kept per dynamic vertex and pass from the second and
- * third sets (more -> dropped, status bit); (nv+nvs)*cap*4 must stay below 2^32. */
+ * third sets (more -> dropped, status bit); (nv+nvs)*cap*4 must stay below 2^32.
+ *
+ * Pair records for the velocity pass (the reference's vt_st_pair / vt_dy_pair /
+ * tv_st_pair fields with their g and schur, XPBD.py:51-73): with pair_rec and
+ * pair_cnt non-null, each query q of the 2nv+nvs keeps the pairs among its first
+ * cap records that are in contact (d < dhat; the others have zero gradients and
+ * no effect on velocities), at most pair_cap (more -> dropped, status bit
+ * SLX_STATUS_PAIR_OVERFLOW; pair_cap = cap never drops).  Layout: float4
+ * pair_rec[(j*4 + w) * (2nv+nvs) + q] for pair j < pair_cnt[q], w = 0..3:
+ * (g0, schur), (g1, face bits), (g2, type bits), (g3, 0). */
 size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap);
 int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st,
                        float* y4, const float* xs4, const float* fixed, const float* m_inv,
                        float dhat, float k_col, int cap, const int32_t* c_vt_st,
                        const int32_t* n_vt_st, const int32_t* c_vt_dy, const int32_t* n_vt_dy,
                        const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
+                       int pair_cap, int32_t* pair_cnt, float* pair_rec,
                        int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                        slx_stream_t stream);
+
+/* ---------------- velocity pass (collision_constraints_v.py) ---------------- */
+
+/* compute_velocity, XPBD.py:340-343: v = (1-damping)*f*(y-x)/dt; v4.w = f. */
+int slx_compute_velocity(int nv, const float* x4, float* v4, const float* y4, const float* fixed,
+                         float damping, float dt, slx_stream_t stream);
+/* update_x, XPBD.py:332-337: x += dt*v. */
+int slx_update_x(int nv, float* x4, const float* v4, float dt, slx_stream_t stream);
+/* solve_constraints_v, XPBD.py:374-381: zero dv/nc; with pair records (see
+ * slx_collision_pass) the velocity-level contact and friction constraints of
+ * each query set (solve_collision_constraints_v, XPBD.py:283-311, with __vt_st /
+ * __vt_dy / __tv_st of collision_constraints_v.py:5-570) accumulated per vertex in
+ * the reference's serial order; then update_dv (XPBD.py:397-404): dv /= nc where
+ * nc > 0, v += f*dv where m_inv > 0.  v4 in place.  pair_cnt null: update_dv
+ * alone (the reference without collision handling).
+ *   vs4      static vertex velocities (float4, nullable = at rest, as the
+ *            reference's static meshes);
+ *   m_inv_st static inverse masses, read by __vt_st type 3 at the dynamic vertex's
+ *            index (collision_constraints_v.py:64), 0 past nvs or when null;
+ *   x4       nullable: when given, update_x (x += dt*v) is fused in.
+ * Workspace: slx_collision_ws_bytes(nv, nvs, rec_cap); (nv+nvs)*pair_cap*8 must
+ * stay below 2^32. */
+int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                      const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
+                      float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec, float* x4,
+                      float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
+                      slx_stream_t stream);
 
 /* ---------------- graph-coloured Gauss-Seidel ---------------- */
 
@@ -384,6 +422,14 @@ typedef struct slx_frame_desc {
   int32_t* status;                       /* device status word */
   void* ws;
   size_t ws_bytes;
+  /* velocity pass (enable_velocity_update, XPBD.py:29, 467-468) */
+  int32_t velocity;
+  float mu;
+  const float* vs4;                      /* static vertex velocities, nullable */
+  const float* m_inv_st;                 /* static inverse masses, nullable */
+  int32_t pair_cap;                      /* contacting pairs kept per query */
+  int32_t* pair_cnt;                     /* [2nv + nvs] */
+  float* pair_rec;                       /* float4 [4 pair_cap (2nv + nvs)] */
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

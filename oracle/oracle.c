@@ -11,6 +11,8 @@
 
 static inline float sqrt_f(float a) { return sqrtf(a); }
 static inline double sqrt_d(double a) { return sqrt(a); }
+static inline float fabs_f(float a) { return fabsf(a); }
+static inline double fabs_d(double a) { return fabs(a); }
 
 #define R float
 #define S f

@@ -58,7 +58,32 @@ extern "C" {
                               R k_col, int cap, const int* vt_st_c, const int* vt_st_n,     \
                               const int* vt_dy_c, const int* vt_dy_n, const int* tv_st_c,   \
                               const int* tv_st_n, R* dx, R* nc, int* pair_num_vt_st,        \
-                              int* pair_num_vt_dy, int* pair_num_tv_st);
+                              int* pair_num_vt_dy, int* pair_num_tv_st);                    \
+  /* the same pass, also storing the pair records of each set k (0 vt_st, 1 vt_dy,           \
+     2 tv_st) in the reference's layout [query][cap]: pi_k int[2] (face, type),              \
+     pg_k R[12] (g0..g3), ps_k R[1] (schur); nullable */                                      \
+  void orc_collision_pass_rec_##S(int nv, int nvs, const int* faces_dy, const int* faces_st,\
+                                  R* y, const R* xs, const R* fixed, const R* m_inv, R dhat, \
+                                  R k_col, int cap, const int* vt_st_c, const int* vt_st_n, \
+                                  const int* vt_dy_c, const int* vt_dy_n,                   \
+                                  const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,     \
+                                  int* pair_num_vt_st, int* pair_num_vt_dy,                 \
+                                  int* pair_num_tv_st, int* pi0, R* pg0, R* ps0, int* pi1,  \
+                                  R* pg1, R* ps1, int* pi2, R* pg2, R* ps2);                \
+  /* XPBD.py:340-343 compute_velocity */                                                     \
+  void orc_compute_velocity_##S(int nv, const R* x, R* v, const R* y, const R* fixed,       \
+                                R damping, R dt);                                           \
+  /* XPBD.py:332-337 update_x */                                                             \
+  void orc_update_x_##S(int nv, R* x, const R* v, R dt);                                    \
+  /* XPBD.py:374-381 solve_constraints_v with 283-311 and                                   \
+     collision_constraints_v.py:5-570 over the recorded pairs, then update_dv               \
+     (XPBD.py:397-404); v updated in place, dv/nc are scratch outputs */                     \
+  void orc_velocity_pass_##S(int nv, int nvs, const int* faces_dy, const int* faces_st,     \
+                             R* v, const R* vs, const R* fixed, const R* m_inv,             \
+                             const R* m_inv_st, R mu, int cap, const int* pn0,              \
+                             const int* pi0, const R* pg0, const R* ps0, const int* pn1,    \
+                             const int* pi1, const R* pg1, const R* ps1, const int* pn2,    \
+                             const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc);
 
 ORC_DECL_REAL(float, f)
 ORC_DECL_REAL(double, d)

@@ -106,6 +106,16 @@ def update_vx(x, v, y, fixed, damping, dt):
     getattr(lib(), "orc_update_vx_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
 
 
+def compute_velocity(x, v, y, fixed, damping, dt):
+    ct, s = _real(x.dtype)
+    getattr(lib(), "orc_compute_velocity_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
+
+
+def update_x(x, v, dt):
+    ct, s = _real(x.dtype)
+    getattr(lib(), "orc_update_x_" + s)(len(x), P(x), P(v), ct(dt))
+
+
 def d_type_pt(p, t0, t1, t2, dtype=np.float32):
     s = "f" if np.dtype(dtype) == np.float32 else "d"
     a = [np.ascontiguousarray(q, dtype) for q in (p, t0, t1, t2)]
@@ -287,23 +297,73 @@ def greedy_coloring(nv, edges, max_colors=64):
     return color, order, offsets[:nc + 1]
 
 
-def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st):
+def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st, record=False):
+    """Position-level collision pass; with record=True the result also holds the
+    pair records of the three query sets ("pairs": per set a dict of num [n],
+    fid/dtype [n, cap], g [n, cap, 4, 3], schur [n, cap]), the reference's
+    vt_st_pair/vt_dy_pair/tv_st_pair fields (XPBD.py:51-73)."""
     ct, s = _real(y.dtype)
     nv = len(fixed)
     nvs = 0 if xs is None else len(xs)
     dx = np.zeros_like(y)
     nc = np.zeros(nv, y.dtype)
     pn = [np.zeros(max(nv, 1), np.int32), np.zeros(max(nv, 1), np.int32), np.zeros(max(nvs, 1), np.int32)]
+    rec = []
+    for n in (nv, nv, nvs):
+        n = max(n, 1)
+        rec.append((np.zeros((n, cap, 2), np.int32), np.zeros((n, cap, 4, 3), y.dtype),
+                    np.zeros((n, cap), y.dtype)) if record else (None, None, None))
 
     def cn(t):
         return (None, None) if t is None else (np.ascontiguousarray(t[0], np.int32), np.ascontiguousarray(t[1], np.int32))
     (a, an), (b, bn), (c, cnn) = cn(vt_st), cn(vt_dy), cn(tv_st)
     xs_ = np.zeros((1, 3), y.dtype) if xs is None else np.ascontiguousarray(xs, y.dtype)
     fst = np.zeros((1, 3), np.int32) if faces_st is None else np.ascontiguousarray(faces_st, np.int32)
-    getattr(lib(), "orc_collision_pass_" + s)(
+    getattr(lib(), "orc_collision_pass_rec_" + s)(
         nv, nvs, P(mesh_dy["faces"]), P(fst), P(y), P(xs_), P(fixed), P(m_inv), ct(dhat), ct(k_col), cap,
-        P(a), P(an), P(b), P(bn), P(c), P(cnn), P(dx), P(nc), P(pn[0]), P(pn[1]), P(pn[2]))
-    return dict(dx=dx, nc=nc, pair_num=pn)
+        P(a), P(an), P(b), P(bn), P(c), P(cnn), P(dx), P(nc), P(pn[0]), P(pn[1]), P(pn[2]),
+        *[P(arr) for r in rec for arr in r])
+    out = dict(dx=dx, nc=nc, pair_num=pn)
+    if record:
+        present = (vt_st is not None, vt_dy is not None, tv_st is not None)
+        out["pairs"] = [dict(num=pn[k], fid=rec[k][0][..., 0], dtype=rec[k][0][..., 1], g=rec[k][1],
+                             schur=rec[k][2], pi=rec[k][0]) if present[k] else None for k in range(3)]
+        out["cap"] = cap
+    return out
+
+
+def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, cap, nvs=None):
+    """solve_constraints_v (XPBD.py:374-381, 283-311; collision_constraints_v.py
+    5-570; update_dv 397-404) over pair records from collision_pass(record=True);
+    v is updated in place.  pairs None: no collision, update_dv alone.
+    vs None: static vertices at rest.  nvs: static vertex count (default: from
+    vs, m_inv_st, the static-vertex pair records or the static faces)."""
+    ct, s = _real(v.dtype)
+    nv = len(fixed)
+    if nvs is None:
+        if vs is not None:
+            nvs = len(vs)
+        elif m_inv_st is not None:
+            nvs = len(m_inv_st)
+        elif pairs is not None and pairs[2] is not None:
+            nvs = len(pairs[2]["num"])
+        elif faces_st is not None:
+            nvs = int(np.max(faces_st)) + 1
+        else:
+            nvs = 0
+    dv = np.zeros_like(v)
+    nc = np.zeros(nv, v.dtype)
+    vs_ = np.zeros((max(nvs, 1), 3), v.dtype) if vs is None else np.ascontiguousarray(vs, v.dtype)
+    fst = np.zeros((1, 3), np.int32) if faces_st is None else np.ascontiguousarray(faces_st, np.int32)
+    mst = None if m_inv_st is None else np.ascontiguousarray(m_inv_st, v.dtype)
+    args = []
+    for k in range(3):
+        p = None if pairs is None else pairs[k]
+        args += [None, None, None, None] if p is None else [P(p["num"]), P(p["pi"]), P(p["g"]), P(p["schur"])]
+    getattr(lib(), "orc_velocity_pass_" + s)(
+        nv, nvs, P(mesh_dy["faces"]), P(fst), P(v), P(vs_), P(fixed), P(m_inv), P(mst), ct(mu), cap, *args,
+        P(dv), P(nc))
+    return dict(dv=dv, nc=nc)
 
 
 # ---------------------------------------------------------------- full driver
@@ -318,8 +378,9 @@ class OracleCloth:
 
     def __init__(self, mesh_dy, mesh_st=None, *, dhat, stiffness_stretch, stiffness_bending,
                  g, dt, dtype=np.float32, damping=0.001, padding=0.05, collision=False,
-                 bvh="cell", cap=40, refit_every_substep=False, method="jacobi"):
+                 bvh="cell", cap=40, refit_every_substep=False, method="jacobi", velocity=False, mu=0.8):
         self.method = method
+        self.velocity, self.mu = velocity, mu
         self.dt_ = np.dtype(dtype)
         self.m = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh_dy.items()}
         self.m["edges"] = np.ascontiguousarray(self.m["edges"], np.int32)
@@ -392,8 +453,20 @@ class OracleCloth:
                 if self.collision:
                     cands = self._broadphase()
                     xs = None if self.st is None else np.asarray(self.st["x"], self.dt_)
-                    collision_pass(self.m, None if self.st is None else self.st["faces"], self.y, xs,
-                                   self.fixed, self.m_inv, R(self.dhat), R(1e8), self.cap,
-                                   cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"))
-            update_vx(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
+                    res = collision_pass(self.m, None if self.st is None else self.st["faces"], self.y, xs,
+                                         self.fixed, self.m_inv, R(self.dhat), R(1e8), self.cap,
+                                         cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"),
+                                         record=self.velocity)
+                    self.last_pairs = res.get("pairs")
+            if not self.velocity:
+                update_vx(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
+                continue
+            # compute_velocity, solve_constraints_v, update_x (XPBD.py:465-469)
+            compute_velocity(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
+            st = self.st
+            vs = None if st is None else np.ascontiguousarray(st.get("v", np.zeros_like(st["x"])), self.dt_)
+            mst = None if st is None or "m_inv" not in st else np.asarray(st["m_inv"], self.dt_)
+            velocity_pass(self.m, None if st is None else st["faces"], self.v, vs, self.fixed, self.m_inv, mst,
+                          R(np.float32(self.mu)), self.last_pairs if self.collision else None, self.cap)
+            update_x(self.x, self.v, dts)
         return self.x

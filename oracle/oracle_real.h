@@ -289,8 +289,27 @@ R FN(dist)(int dtype, const R* p, const R* t0, const R* t1, const R* t2, R* g) {
 static const unsigned char CAT(slots_, S)[7] = {0x3, 0x5, 0x9, 0x7, 0xD, 0xB, 0xF};
 
 /* collision_constraints_x.py:5-69 __vt_st: dynamic vertex vs static triangle. */
+/* Pair record of query q (collision_constraints_x.py:60-68, 164-172, 301-309):
+   while the query holds fewer than cap records, store (fid, dtype), g0..g3 and
+   schur at its next slot.  pi/pg/ps null: count only.  Layout [query][cap] as
+   the reference's pair fields: pi int[2], pg R[12], ps R[1] per slot. */
+static void CAT(record_, S)(int q, int cap, int* pnum, int* pi, R* pg, R* ps, int fid, int dtype,
+                            const R* g, R schur) {
+  int j = pnum[q];
+  if (j >= cap) return;
+  if (pi) {
+    size_t at = (size_t)q * (size_t)cap + (size_t)j;
+    pi[2 * at] = fid;
+    pi[2 * at + 1] = dtype;
+    for (int c = 0; c < 12; ++c) pg[12 * at + c] = g[c];
+    ps[at] = schur;
+  }
+  pnum[q] = j + 1;
+}
+
 static void CAT(vt_st_, S)(int vd, int fs, const int* faces_st, const R* y, const R* xs,
-                           const R* m_inv, R dhat, R k, R* dx, R* nc, int cap, int* pnum) {
+                           const R* m_inv, R dhat, R k, R* dx, R* nc, int cap, int* pnum,
+                           int* pi, R* pg, R* ps) {
   const R* p = y + 3 * vd;
   const R *t0 = xs + 3 * faces_st[3 * fs], *t1 = xs + 3 * faces_st[3 * fs + 1],
           *t2 = xs + 3 * faces_st[3 * fs + 2];
@@ -303,7 +322,7 @@ static void CAT(vt_st_, S)(int vd, int fs, const int* faces_st, const R* y, cons
     R ld = (k * (dhat - d)) / ((k * schur) + (R)1);
     for (int c = 0; c < 3; ++c) dx[3 * vd + c] = dx[3 * vd + c] + (w * ld) * g[c];
     nc[vd] = nc[vd] + (R)1;
-    if (pnum[vd] < cap) pnum[vd] += 1;
+    CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fs, dtype, g, schur);
   }
 }
 
@@ -311,7 +330,7 @@ static void CAT(vt_st_, S)(int vd, int fs, const int* faces_st, const R* y, cons
    Only the triangle's vertices move; the pair is recorded whatever d is. */
 static void CAT(tv_st_, S)(int fd, int vs, const int* faces_dy, R* y, const R* xs,
                            const R* fixed, const R* m_inv, R dhat, R k, R* dx, R* nc, int cap,
-                           int* pnum) {
+                           int* pnum, int* pi, R* pg, R* ps) {
   const R* p = xs + 3 * vs;
   int vid[4] = {-1, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
   const R *t0 = y + 3 * vid[1], *t1 = y + 3 * vid[2], *t2 = y + 3 * vid[3];
@@ -338,15 +357,20 @@ static void CAT(tv_st_, S)(int fd, int vs, const int* faces_dy, R* y, const R* x
     }
     for (int s = 1; s < 4; ++s)
       if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+    CAT(record_, S)(vs, cap, pnum, pi, pg, ps, fd, dtype, g, schur);
+    return;
   }
-  if (pnum[vs] < cap) pnum[vs] += 1;
+  /* out of range: recorded with zero gradients and schur (collision_constraints_x.py:86-87) */
+  R z[12] = {0};
+  CAT(record_, S)(vs, cap, pnum, pi, pg, ps, fd, dtype, z, (R)0);
 }
 
 /* collision_constraints_x.py:175-310 __vt_dy: dynamic vertex vs dynamic triangle
    (self contact), hard constraint with ld = (dhat - d) / (sum f w |g|^2 + 1e-4).
    Type 0 omits f on both points' deltas (collision_constraints_x.py:199-200). */
 static void CAT(vt_dy_, S)(int vd, int fd, const int* faces_dy, R* y, const R* fixed,
-                           const R* m_inv, R dhat, R* dx, R* nc, int cap, int* pnum) {
+                           const R* m_inv, R dhat, R* dx, R* nc, int cap, int* pnum, int* pi, R* pg,
+                           R* ps) {
   int vid[4] = {vd, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
   const R *p = y + 3 * vid[0], *t0 = y + 3 * vid[1], *t1 = y + 3 * vid[2], *t2 = y + 3 * vid[3];
   int dtype = FN(d_type_pt)(p, t0, t1, t2);
@@ -373,15 +397,21 @@ static void CAT(vt_dy_, S)(int vd, int fd, const int* faces_dy, R* y, const R* f
     }
     for (int s = 0; s < 4; ++s)
       if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+    CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fd, dtype, g, schur);
+    return;
   }
-  if (pnum[vd] < cap) pnum[vd] += 1;
+  R z[12] = {0};
+  CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fd, dtype, z, (R)0);
 }
 
-void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
-                        const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
-                        const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
-                        const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
-                        int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st) {
+/* XPBD.py:236-264 + update_dx.  pi_k/pg_k/ps_k (nullable) receive the pair
+   records of query set k (0 vt_st, 1 vt_dy, 2 tv_st) for the velocity pass. */
+void FN(collision_pass_rec)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
+                            const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
+                            const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
+                            const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
+                            int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0, R* pg0, R* ps0,
+                            int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2) {
   for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
   for (int i = 0; i < nv; ++i) { nc[i] = 0; pn_vt_st[i] = 0; pn_vt_dy[i] = 0; }
   for (int i = 0; i < nvs; ++i) pn_tv_st[i] = 0;
@@ -389,7 +419,8 @@ void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_s
     for (int v = 0; v < nv; ++v) {
       int n = vt_st_n[v] < cap ? vt_st_n[v] : cap;
       for (int j = 0; j < n; ++j)
-        CAT(vt_st_, S)(v, vt_st_c[v * cap + j], faces_st, y, xs, m_inv, dhat, k_col, dx, nc, cap, pn_vt_st);
+        CAT(vt_st_, S)(v, vt_st_c[v * cap + j], faces_st, y, xs, m_inv, dhat, k_col, dx, nc, cap, pn_vt_st,
+                       pi0, pg0, ps0);
     }
   if (vt_dy_c)
     for (int v = 0; v < nv; ++v) {
@@ -397,16 +428,274 @@ void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_s
       for (int j = 0; j < n; ++j) {
         int f = vt_dy_c[v * cap + j];
         if (faces_dy[3 * f] == v || faces_dy[3 * f + 1] == v || faces_dy[3 * f + 2] == v) continue;
-        CAT(vt_dy_, S)(v, f, faces_dy, y, fixed, m_inv, dhat, dx, nc, cap, pn_vt_dy);
+        CAT(vt_dy_, S)(v, f, faces_dy, y, fixed, m_inv, dhat, dx, nc, cap, pn_vt_dy, pi1, pg1, ps1);
       }
     }
   if (tv_st_c)
     for (int v = 0; v < nvs; ++v) {
       int n = tv_st_n[v] < cap ? tv_st_n[v] : cap;
       for (int j = 0; j < n; ++j)
-        CAT(tv_st_, S)(tv_st_c[v * cap + j], v, faces_dy, y, xs, fixed, m_inv, dhat, k_col, dx, nc, cap, pn_tv_st);
+        CAT(tv_st_, S)(tv_st_c[v * cap + j], v, faces_dy, y, xs, fixed, m_inv, dhat, k_col, dx, nc, cap, pn_tv_st,
+                       pi2, pg2, ps2);
     }
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+}
+
+void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
+                        const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
+                        const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
+                        const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
+                        int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st) {
+  FN(collision_pass_rec)(nv, nvs, faces_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st_c,
+                         vt_st_n, vt_dy_c, vt_dy_n, tv_st_c, tv_st_n, dx, nc, pn_vt_st, pn_vt_dy,
+                         pn_tv_st, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
+}
+
+/* ---------------- velocity pass (collision_constraints_v.py, XPBD.py:283-311, 374-404) ---- */
+
+/* XPBD.py:340-343: v = (1 - damping) * f * (y - x) / dt. */
+void FN(compute_velocity)(int nv, const R* x, R* v, const R* y, const R* fixed, R damping, R dt) {
+  R a = (R)1 - damping;
+  for (int i = 0; i < nv; ++i)
+    for (int c = 0; c < 3; ++c) v[3 * i + c] = ((a * fixed[i]) * (y[3 * i + c] - x[3 * i + c])) / dt;
+}
+
+/* XPBD.py:332-337: x += dt * v. */
+void FN(update_x)(int nv, R* x, const R* v, R dt) {
+  for (int i = 0; i < 3 * nv; ++i) x[i] = x[i] + dt * v[i];
+}
+
+static inline R CAT(norm3_, S)(const R* a) { return SQRT(DOT(a, a)); }
+
+static inline void CAT(vadd_, S)(R* dv, R* nc, int v, const R* d) {
+  for (int c = 0; c < 3; ++c) dv[3 * v + c] = dv[3 * v + c] + d[c];
+  nc[v] = nc[v] + (R)1;
+}
+
+/* collision_constraints_v.py:5-134 __vt_st: dynamic vertex v0 against static
+   triangle fs.  Normal part: ld = -Cv / schur, dv0 += w ld g0 (type 3 takes w
+   from the static mesh's inverse masses at index v0, :64 -- 0 past the static
+   vertex count, where the reference reads outside that field).  Friction: g0Tan =
+   p - vTan0 with p the static contact point's velocity (weighted by gradient
+   norms on edges and the face; type 5 uses vTan0 - p, :109), full correction if
+   mu |Cv| > cTan, else mu times it. */
+static void CAT(vel_vt_st_, S)(int v0, int fs, int dtype, const R* g, R schur, const int* faces_st,
+                               const R* v, const R* vs, const R* m_inv, const R* m_inv_st, int nvs, R mu,
+                               R* dv, R* nc) {
+  int vid[4] = {v0, faces_st[3 * fs], faces_st[3 * fs + 1], faces_st[3 * fs + 2]};
+  const R *u0 = v + 3 * v0, *u1 = vs + 3 * vid[1], *u2 = vs + 3 * vid[2], *u3 = vs + 3 * vid[3];
+  R Cv = ((DOT(g, u0) + DOT(g + 3, u1)) + DOT(g + 6, u2)) + DOT(g + 9, u3);
+  if (!(Cv < (R)0)) return;
+  R ld = -Cv / schur;
+  R m0 = m_inv[v0];
+  R mn = m0;
+  if (dtype == 3) mn = (m_inv_st && v0 < nvs) ? m_inv_st[v0] : (R)0;
+  R d[3], vt[3], p[3], gt[3];
+  for (int c = 0; c < 3; ++c) d[c] = (mn * ld) * g[c];
+  CAT(vadd_, S)(dv, nc, v0, d);
+  for (int c = 0; c < 3; ++c) vt[c] = u0[c] + (m0 * ld) * g[c];
+  if (dtype <= 2) {
+    const R* q = dtype == 0 ? u1 : (dtype == 1 ? u2 : u3);
+    for (int c = 0; c < 3; ++c) p[c] = q[c];
+  } else if (dtype <= 5) {
+    const R *ga = dtype == 4 ? g + 6 : g + 3, *gb = dtype == 3 ? g + 6 : g + 9;
+    const R *qa = dtype == 4 ? u2 : u1, *qb = dtype == 3 ? u2 : u3;
+    R a = CAT(norm3_, S)(ga), b = CAT(norm3_, S)(gb);
+    for (int c = 0; c < 3; ++c) p[c] = (a * qa[c] + b * qb[c]) / (a + b);
+  } else {
+    R a = CAT(norm3_, S)(g + 3), b = CAT(norm3_, S)(g + 6), cc = CAT(norm3_, S)(g + 9);
+    for (int c = 0; c < 3; ++c) p[c] = ((a * u1[c] + b * u2[c]) + cc * u3[c]) / ((a + b) + cc);
+  }
+  for (int c = 0; c < 3; ++c) gt[c] = dtype == 5 ? vt[c] - p[c] : p[c] - vt[c];
+  R dd = DOT(gt, gt);
+  R cT = (R)0.5f * dd;
+  R sT = m0 * dd + (R)1e-4f;
+  R lT = cT / sT;
+  R mu_l = (mu * CAT(fabs_, S)(Cv) > cT) ? (R)1 : mu;
+  for (int c = 0; c < 3; ++c) d[c] = mu_l * ((m0 * lT) * gt[c]);
+  CAT(vadd_, S)(dv, nc, v0, d);
+}
+
+/* Moving slots among (p, t0, t1, t2) per type, without the point. */
+static const unsigned char CAT(tri_slots_, S)[7] = {0x2, 0x4, 0x8, 0x6, 0xC, 0xA, 0xE};
+
+/* collision_constraints_v.py:136-326 __tv_st: static vertex vs0 against dynamic
+   triangle fd.  Cv uses g3 for both t1 and t2 (:144).  Normal: dv_k += f w ld g_k;
+   friction on vTan_k = v_k + f w g_k ld, against the static vertex's velocity. */
+static void CAT(vel_tv_st_, S)(int vs0, int fd, int dtype, const R* g, R schur, const int* faces_dy,
+                               const R* v, const R* vs, const R* fixed, const R* m_inv, R mu, R* dv,
+                               R* nc) {
+  int vid[4] = {vs0, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
+  const R* u[4] = {vs + 3 * vs0, v + 3 * vid[1], v + 3 * vid[2], v + 3 * vid[3]};
+  R Cv = ((DOT(g, u[0]) + DOT(g + 3, u[1])) + DOT(g + 9, u[2])) + DOT(g + 9, u[3]);
+  if (!(Cv < (R)0)) return;
+  R ld = -Cv / schur;
+  unsigned m = CAT(tri_slots_, S)[dtype];
+  R vt[4][3], d[3];
+  for (int s = 1; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    int w = vid[s];
+    for (int c = 0; c < 3; ++c) d[c] = ((fixed[w] * m_inv[w]) * ld) * g[3 * s + c];
+    CAT(vadd_, S)(dv, nc, w, d);
+  }
+  for (int s = 1; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    int w = vid[s];
+    for (int c = 0; c < 3; ++c) vt[s][c] = u[s][c] + ((fixed[w] * m_inv[w]) * g[3 * s + c]) * ld;
+  }
+  R gt[4][3], q[3];
+  R cT, sT;
+  if (dtype <= 2) {
+    int s = dtype + 1, w = vid[s];
+    for (int c = 0; c < 3; ++c) gt[s][c] = u[0][c] - vt[s][c];
+    R dd = DOT(gt[s], gt[s]);
+    cT = (R)0.5f * dd;
+    sT = m_inv[w] * dd + (R)1e-4f;
+  } else if (dtype <= 5) {
+    int sa = dtype == 4 ? 2 : 1, sb = dtype == 3 ? 2 : 3;
+    R a = CAT(norm3_, S)(g + 3 * sa), b = CAT(norm3_, S)(g + 3 * sb);
+    R ab = a + b;
+    for (int c = 0; c < 3; ++c) q[c] = u[0][c] - (a * vt[sa][c] + b * vt[sb][c]) / ab;
+    for (int c = 0; c < 3; ++c) {
+      gt[sa][c] = (a / ab) * q[c];
+      gt[sb][c] = (b / ab) * q[c];
+    }
+    cT = (R)0.5f * DOT(q, q);
+    sT = (m_inv[vid[sa]] * DOT(gt[sa], gt[sa]) + m_inv[vid[sb]] * DOT(gt[sb], gt[sb])) + (R)1e-4f;
+  } else {
+    R a = CAT(norm3_, S)(g + 3), b = CAT(norm3_, S)(g + 6), cc = CAT(norm3_, S)(g + 9);
+    R abc = (a + b) + cc;
+    for (int c = 0; c < 3; ++c) q[c] = u[0][c] - ((a * vt[1][c] + b * vt[2][c]) + cc * vt[3][c]) / abc;
+    for (int c = 0; c < 3; ++c) {
+      gt[1][c] = (a / abc) * q[c];
+      gt[2][c] = (b / abc) * q[c];
+      gt[3][c] = (cc / abc) * q[c];
+    }
+    cT = (R)0.5f * DOT(q, q);
+    sT = ((m_inv[vid[1]] * DOT(gt[1], gt[1]) + m_inv[vid[2]] * DOT(gt[2], gt[2])) +
+          m_inv[vid[3]] * DOT(gt[3], gt[3])) + (R)1e-4f;
+  }
+  R lT = cT / sT;
+  R mu_l = (mu * CAT(fabs_, S)(Cv) > cT) ? (R)1 : mu;
+  for (int s = 1; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    int w = vid[s];
+    for (int c = 0; c < 3; ++c) d[c] = mu_l * ((m_inv[w] * lT) * gt[s][c]);
+    CAT(vadd_, S)(dv, nc, w, d);
+  }
+}
+
+/* collision_constraints_v.py:329-570 __vt_dy: dynamic vertex v0 against
+   dynamic triangle fd.  Normal: dv_k += f w g_k ld; friction with the relative
+   velocity of the two points (types 0-2) or of v0 and the weighted edge/face
+   point (types 3-6).  Type 1 scales the face point's friction delta by the
+   inverse mass of t0 instead of t1 (:388). */
+static void CAT(vel_vt_dy_, S)(int v0, int fd, int dtype, const R* g, R schur, const int* faces_dy,
+                               const R* v, const R* fixed, const R* m_inv, R mu, R* dv, R* nc) {
+  int vid[4] = {v0, faces_dy[3 * fd], faces_dy[3 * fd + 1], faces_dy[3 * fd + 2]};
+  const R* u[4] = {v + 3 * v0, v + 3 * vid[1], v + 3 * vid[2], v + 3 * vid[3]};
+  R Cv = ((DOT(g, u[0]) + DOT(g + 3, u[1])) + DOT(g + 6, u[2])) + DOT(g + 9, u[3]);
+  if (!(Cv < (R)0)) return;
+  R ld = -Cv / schur;
+  unsigned m = CAT(slots_, S)[dtype];
+  R vt[4][3], d[3], gt[4][3], mt[4];
+  for (int s = 0; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    int w = vid[s];
+    for (int c = 0; c < 3; ++c) d[c] = ((fixed[w] * m_inv[w]) * g[3 * s + c]) * ld;
+    CAT(vadd_, S)(dv, nc, w, d);
+  }
+  for (int s = 0; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    int w = vid[s];
+    for (int c = 0; c < 3; ++c) vt[s][c] = u[s][c] + ((fixed[w] * m_inv[w]) * g[3 * s + c]) * ld;
+    mt[s] = m_inv[w];
+  }
+  R cT, sT;
+  if (dtype <= 2) {
+    int s = dtype + 1;
+    for (int c = 0; c < 3; ++c) {
+      gt[0][c] = -(vt[0][c] - vt[s][c]);
+      gt[s][c] = vt[0][c] - vt[s][c];
+    }
+    R d0 = DOT(gt[0], gt[0]), d1 = DOT(gt[s], gt[s]);
+    cT = (R)0.5f * (d1 + d0);
+    sT = (mt[0] * d0 + mt[s] * d1) + (R)1e-4f;
+    if (dtype == 1) mt[s] = m_inv[vid[1]];
+  } else if (dtype <= 5) {
+    int sa = dtype == 4 ? 2 : 1, sb = dtype == 3 ? 2 : 3;
+    R a = CAT(norm3_, S)(g + 3 * sa), b = CAT(norm3_, S)(g + 3 * sb);
+    R ab = a + b;
+    for (int c = 0; c < 3; ++c) gt[0][c] = (a * vt[sa][c] + b * vt[sb][c]) / ab - vt[0][c];
+    for (int c = 0; c < 3; ++c) {
+      gt[sa][c] = (-(a / ab)) * gt[0][c];
+      gt[sb][c] = (-(b / ab)) * gt[0][c];
+    }
+    R d0 = DOT(gt[0], gt[0]), da = DOT(gt[sa], gt[sa]), db = DOT(gt[sb], gt[sb]);
+    cT = (R)0.5f * ((d0 + da) + db);
+    sT = ((mt[0] * d0 + mt[sa] * da) + mt[sb] * db) + (R)1e-4f;
+  } else {
+    R a = CAT(norm3_, S)(g + 3), b = CAT(norm3_, S)(g + 6), cc = CAT(norm3_, S)(g + 9);
+    R abc = (a + b) + cc;
+    for (int c = 0; c < 3; ++c)
+      gt[0][c] = ((a * vt[1][c] + b * vt[2][c]) + cc * vt[3][c]) / abc - vt[0][c];
+    for (int c = 0; c < 3; ++c) {
+      gt[1][c] = (-(a / abc)) * gt[0][c];
+      gt[2][c] = (-(b / abc)) * gt[0][c];
+      gt[3][c] = (-(cc / abc)) * gt[0][c];
+    }
+    R d0 = DOT(gt[0], gt[0]), d1 = DOT(gt[1], gt[1]), d2 = DOT(gt[2], gt[2]), d3 = DOT(gt[3], gt[3]);
+    cT = (R)0.5f * (((d0 + d1) + d2) + d3);
+    sT = (((mt[0] * d0 + mt[1] * d1) + mt[2] * d2) + mt[3] * d3) + (R)1e-4f;
+  }
+  R lT = cT / sT;
+  R mu_l = (mu * CAT(fabs_, S)(Cv) > cT) ? (R)1 : mu;
+  for (int s = 0; s < 4; ++s) {
+    if (!(m & (1u << s))) continue;
+    for (int c = 0; c < 3; ++c) d[c] = mu_l * ((mt[s] * lT) * gt[s][c]);
+    CAT(vadd_, S)(dv, nc, vid[s], d);
+  }
+}
+
+/* solve_constraints_v (XPBD.py:374-381): dv = nc = 0; the recorded pairs of the
+   three query sets in the reference's loop order (XPBD.py:283-311); update_dv
+   (XPBD.py:397-404): dv /= nc where nc > 0, v += f dv where m_inv > 0.
+   pn_k null: set k absent.  vs: static vertex velocities (zeros in the
+   reference). */
+void FN(velocity_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* v, const R* vs,
+                       const R* fixed, const R* m_inv, const R* m_inv_st, R mu, int cap,
+                       const int* pn0, const int* pi0, const R* pg0, const R* ps0,
+                       const int* pn1, const int* pi1, const R* pg1, const R* ps1,
+                       const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc) {
+  for (int i = 0; i < 3 * nv; ++i) dv[i] = 0;
+  for (int i = 0; i < nv; ++i) nc[i] = 0;
+  if (pn0)
+    for (int q = 0; q < nv; ++q)
+      for (int j = 0; j < pn0[q]; ++j) {
+        size_t at = (size_t)q * (size_t)cap + (size_t)j;
+        CAT(vel_vt_st_, S)(q, pi0[2 * at], pi0[2 * at + 1], pg0 + 12 * at, ps0[at], faces_st, v, vs, m_inv,
+                           m_inv_st, nvs, mu, dv, nc);
+      }
+  if (pn1)
+    for (int q = 0; q < nv; ++q)
+      for (int j = 0; j < pn1[q]; ++j) {
+        size_t at = (size_t)q * (size_t)cap + (size_t)j;
+        CAT(vel_vt_dy_, S)(q, pi1[2 * at], pi1[2 * at + 1], pg1 + 12 * at, ps1[at], faces_dy, v, fixed, m_inv,
+                           mu, dv, nc);
+      }
+  if (pn2)
+    for (int q = 0; q < nvs; ++q)
+      for (int j = 0; j < pn2[q]; ++j) {
+        size_t at = (size_t)q * (size_t)cap + (size_t)j;
+        CAT(vel_tv_st_, S)(q, pi2[2 * at], pi2[2 * at + 1], pg2 + 12 * at, ps2[at], faces_dy, v, vs, fixed,
+                           m_inv, mu, dv, nc);
+      }
+  for (int i = 0; i < nv; ++i) {
+    if (nc[i] > (R)0)
+      for (int c = 0; c < 3; ++c) dv[3 * i + c] = dv[3 * i + c] / nc[i];
+    if (m_inv[i] > (R)0)
+      for (int c = 0; c < 3; ++c) v[3 * i + c] = v[3 * i + c] + fixed[i] * dv[3 * i + c];
+  }
 }
 
 #undef FN

@@ -54,7 +54,11 @@ SIGNATURES = {
     "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
-                           PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
+                           PTR, PTR, I32, PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
+    "slx_compute_velocity": [I32, PTR, PTR, PTR, PTR, F32, F32, STREAM],
+    "slx_update_x": [I32, PTR, PTR, F32, STREAM],
+    "slx_velocity_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, I32, PTR, PTR, PTR, F32, I32, PTR, PTR,
+                          C.c_size_t, STREAM],
     "slx_frame_ws_bytes": [PTR],
     "slx_frame": [PTR, STREAM],
     "slx_seg_build_ws_bytes": [I32, I32],
@@ -103,7 +107,8 @@ class FrameDesc(C.Structure):
                 ("rebuild_static", I32), ("cap", I32), ("rec_cap", I32), ("method", I32), ("gs_edges", PTR),
                 ("gs_color_off", PTR), ("n_colors", I32), ("dyn", Tree), ("st", Tree),
                 ("cand", PTR), ("cand_n", PTR), ("pair_num", PTR), ("status", PTR), ("ws", PTR),
-                ("ws_bytes", C.c_size_t)]
+                ("ws_bytes", C.c_size_t), ("velocity", I32), ("mu", F32), ("vs4", PTR), ("m_inv_st", PTR),
+                ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR)]
 
 
 class SlxError(RuntimeError):

@@ -7,10 +7,12 @@
 //                 run on another side stream beside the dynamic-tree queries]
 //                spring pass with compute_y fused in
 //                per iteration: [spring pass] broad phase (3 query sets),
-//                               collision pass
-//                compute_velocity + update_x
-//  without collision the substeps run as two fused kernels per substep
-//  (slx_substeps_plain).
+//                               collision pass (the last one records the
+//                               contacting pairs when the velocity pass is on)
+//                compute_velocity + update_x, or with enable_velocity_update
+//                compute_velocity, velocity pass, update_x (XPBD.py:465-469)
+//  without collision (and without the velocity pass) the substeps run as two
+//  fused kernels per substep (slx_substeps_plain).
 //
 // Every size the kernels need is either static or device-resident, so the
 // frame enqueues work without any host synchronisation and can be captured in
@@ -121,7 +123,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(d->method == 0 || d->method == 1, "slx_frame: method must be 0 (Jacobi) or 1 (Gauss-Seidel)");
   SLX_REQUIRE(d->method == 0 || d->n_colors == 0 || (d->gs_edges && d->gs_color_off),
               "slx_frame: Gauss-Seidel colouring missing");
-  if (!d->collision && d->method == 0) {
+  const bool vel = d->velocity != 0;
+  if (!d->collision && d->method == 0 && !vel) {
     // x4_alt / v4_alt: the caller's y4_alt doubles as one ping-pong buffer is not
     // possible (four buffers are needed); use the workspace for the alternates.
     size_t need = 2 * sizeof(float4) * (size_t)d->nv;
@@ -140,6 +143,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                                                                                                d->st.vert_inst))),
               "slx_frame: batched trees need instance ids on both meshes");
   SLX_REQUIRE(d->ws_bytes >= slx_frame_ws_bytes(d), "slx_frame: workspace too small");
+  SLX_REQUIRE(!vel || !coll || (d->pair_cnt && d->pair_rec && d->pair_cap >= 0),
+              "slx_frame: the velocity pass needs pair records");
   const int nv = d->nv, nvs = coll ? d->nvs : 0, cap = d->cap;
   const float pad = d->padding;
   int rc;
@@ -226,14 +231,24 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (static_pending && hipStreamWaitEvent(st, side->qjoin, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: join failed");
       // narrow phase + projection + update_dx (XPBD.py:236-264, 365-369)
+      const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
       rc = slx_collision_pass(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                               d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
+                              rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
                               d->rec_cap, d->status, d->ws, d->ws_bytes, stream);
       if (rc) return rc;
     }
     rc = join_refit();                       // (n_iters == 0: nothing queried the tree)
     if (rc) return rc;
-    rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
+    if (vel) {
+      rc = slx_compute_velocity(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
+      if (rc) return rc;
+      rc = slx_velocity_pass(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st, d->mu,
+                             d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec, d->x4, d->dt,
+                             d->rec_cap > 0 ? d->rec_cap : 1, d->status, d->ws, d->ws_bytes, stream);
+    } else {
+      rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
+    }
     if (rc) return rc;
   }
   if (ycur != (float4*)d->y4) {

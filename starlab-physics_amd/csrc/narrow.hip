@@ -105,6 +105,10 @@ struct Contact {
   float3 delta[4];
   unsigned mask;      // slots that receive a delta
   bool record;        // pair recorded for the velocity pass
+  bool hit;           // d < dhat (a recorded pair that can act on velocities)
+  int dtype;
+  float3 g[4];        // gradients as recorded (zero outside the pair's points)
+  float schur;        // as recorded
 };
 
 // One candidate of query set `set` (0 vt_st, 1 vt_dy, 2 tv_st), slots in order
@@ -113,6 +117,8 @@ __device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* 
                                         const float* fixed, const float* m_inv, float dhat, float k, Contact& c) {
   c.mask = 0;
   c.record = false;
+  c.hit = false;
+  c.schur = 0.f;
   c.vid[0] = qv;
   c.vid[1] = fv.x;
   c.vid[2] = fv.y;
@@ -124,6 +130,9 @@ __device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* 
   float3 g[4];
   const float d = dist_grad(dtype, p, t0, t1, t2, g[0], g[1], g[2], g[3]);
   const unsigned m = c_slots[dtype];
+  c.dtype = dtype;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) c.g[s] = g[s];
   if (set == 0) {                                        // __vt_st
     if (d < dhat) {
       float w = m_inv[qv];
@@ -132,11 +141,14 @@ __device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* 
       c.delta[0] = mul3(w * ld, g[0]);
       c.mask = 1u;
       c.record = true;
+      c.hit = true;
+      c.schur = schur;
     }
     return;
   }
   c.record = true;                                       // __tv_st / __vt_dy record every pair
   if (!(d < dhat)) return;
+  c.hit = true;
   const int s0 = (set == 2) ? 1 : 0;                     // static point does not move
   float schur = 0.f;
   bool first = true;
@@ -155,6 +167,7 @@ __device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* 
     schur = schur + 1e-4f;
     ld = (dhat - d) / schur;
   }
+  c.schur = schur;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (s < s0 || !(m & (1u << s))) continue;
@@ -186,19 +199,23 @@ __device__ __forceinline__ bool query_of(const QuerySets& Q, int qg, int& set, i
 // claims a slot of its target vertex's bucket with an integer atomic; the vertex
 // then adds its records in increasing o.  The sum is therefore the serial
 // scatter's, bit for bit, whatever order the slots were claimed in.
+//
+// REC: the query also stores its contacting pair records for the velocity pass
+// (see slx_collision_pass for the layout).
+template <bool REC>
 __global__ void __launch_bounds__(256)
 k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
            const float4* __restrict__ y4, const float4* __restrict__ xs4, const float* __restrict__ fixed,
            const float* __restrict__ m_inv, float dhat, float k, int bucket, int32_t* __restrict__ bcnt,
            float4* __restrict__ brec, float4* __restrict__ self_acc, int32_t* __restrict__ pair_num,
-           int32_t* status) {
+           int pcap, int32_t* __restrict__ pair_cnt, float4* __restrict__ pair_rec, int32_t* status) {
   const int nqt = 2 * Q.nv + Q.nvs;
   const int qg = blockIdx.x * blockDim.x + threadIdx.x;
   if (qg >= nqt) return;
   int set, qv, ncand, nq;
   const int32_t* cand;
-  int pairs = 0, own = 0;
-  bool over = false;
+  int pairs = 0, own = 0, stored = 0;
+  bool over = false, pover = false;
   float3 dx = make_float3(0.f, 0.f, 0.f);
   if (query_of(Q, qg, set, qv, cand, ncand, nq)) {
     const int32_t* F = set == 0 ? fst : fdy;
@@ -206,9 +223,11 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
     // group is processed, in candidate order
     for (int j0 = 0; j0 < ncand; j0 += 4) {
       int3 fv[4];
+      int fid[4];
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int f = j0 + g < ncand ? cand[(size_t)(j0 + g) * nq + qv] : -1;
+        fid[g] = f;
         fv[g] = f >= 0 ? make_int3(F[3 * f], F[3 * f + 1], F[3 * f + 2]) : make_int3(-1, -1, -1);
       }
 #pragma unroll
@@ -217,6 +236,18 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
         if (set == 1 && (fv[g].x == qv || fv[g].y == qv || fv[g].z == qv)) continue;
         Contact c;
         contact(set, qv, fv[g], y4, xs4, fixed, m_inv, dhat, k, c);
+        if (REC && c.record && c.hit && pairs < Q.cap) {
+          if (stored < pcap) {
+            const size_t col = (size_t)stored * 4u * (size_t)nqt + (size_t)qg;
+            pair_rec[col] = make_float4(c.g[0].x, c.g[0].y, c.g[0].z, c.schur);
+            pair_rec[col + (size_t)nqt] = make_float4(c.g[1].x, c.g[1].y, c.g[1].z, __int_as_float(fid[g]));
+            pair_rec[col + 2 * (size_t)nqt] = make_float4(c.g[2].x, c.g[2].y, c.g[2].z, __int_as_float(c.dtype));
+            pair_rec[col + 3 * (size_t)nqt] = make_float4(c.g[3].x, c.g[3].y, c.g[3].z, 0.f);
+            ++stored;
+          } else {
+            pover = true;
+          }
+        }
         pairs += c.record ? 1 : 0;
         if (set == 0) {
           if (c.mask) {
@@ -242,7 +273,9 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
   }
   if (qg < Q.nv) self_acc[qg] = make_float4(dx.x, dx.y, dx.z, (float)own);
   if (pair_num) pair_num[qg] = pairs < Q.cap ? pairs : Q.cap;
+  if (REC) pair_cnt[qg] = stored;
   if (over && status) atomicOr(status, kStatusRecOverflow);
+  if (REC && pover && status) atomicOr(status, kStatusPairOverflow);
 }
 
 // update_dx over the contacts: y += f * (dx / nc) (XPBD.py:383-387); the bucket
@@ -257,23 +290,7 @@ k_contact_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __
   if (n > bucket) n = bucket;
   const float nc = sa.w + (float)n;
   if (!(nc > 0.f)) return;
-  float3 dx = make_float3(sa.x, sa.y, sa.z);
-  const float4* r = brec + (size_t)v * bucket;
-  uint32_t last = 0;
-  for (int i = 0; i < n; ++i) {             // selection by increasing order key
-    uint32_t best = 0xffffffffu;
-    int bj = 0;
-    for (int j = 0; j < n; ++j) {
-      const uint32_t o = __float_as_uint(r[j].w);
-      if ((i == 0 || o > last) && o < best) {
-        best = o;
-        bj = j;
-      }
-    }
-    last = best;
-    const float4 d = r[bj];
-    dx = make_float3(dx.x + d.x, dx.y + d.y, dx.z + d.z);
-  }
+  const float3 dx = bucket_sum(make_float3(sa.x, sa.y, sa.z), brec + (size_t)v * bucket, n);
   const float f = fixed[v];
   const float4 y = y4[v];
   y4[v] = make_float4(y.x + f * (dx.x / nc), y.y + f * (dx.y / nc), y.z + f * (dx.z / nc), y.w);
@@ -284,42 +301,21 @@ __global__ void k_zero_i32(int n, int32_t* __restrict__ a) {
   if (i < n) a[i] = 0;
 }
 
-struct CollisionWs {
-  int32_t* bcnt;
-  float4* self_acc;
-  float4* brec;
-};
-
-static size_t layout(int nv, int bucket, char* base, CollisionWs* w) {
-  size_t o = 0;
-  auto take = [&](size_t bytes) {
-    size_t at = o;
-    o += (bytes + 255) & ~(size_t)255;
-    return base ? base + at : nullptr;
-  };
-  CollisionWs t;
-  t.bcnt = (int32_t*)take(4 * (size_t)nv);
-  t.self_acc = (float4*)take(16 * (size_t)nv);
-  t.brec = (float4*)take(16 * (size_t)nv * (size_t)(bucket > 0 ? bucket : 1));
-  if (w) *w = t;
-  return o;
-}
-
 }  // namespace slx
 
 using namespace slx;
 
 extern "C" size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap) {
   (void)nvs;
-  return layout(nv, rec_cap, nullptr, nullptr);
+  return contact_ws_layout(nv, rec_cap, nullptr, nullptr);
 }
 
 extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
                                   const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col,
                                   int cap, const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
                                   const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st,
-                                  int32_t* pair_num, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
-                                  slx_stream_t stream) {
+                                  int32_t* pair_num, int pair_cap, int32_t* pair_cnt, float* pair_rec,
+                                  int rec_cap, int32_t* status, void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
   if (nv == 0) return SLX_OK;
   SLX_REQUIRE(faces_dy && y4 && fixed && m_inv && ws, "slx_collision_pass: null pointer");
@@ -328,17 +324,24 @@ extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, cons
               "slx_collision_pass: candidate counts missing");
   SLX_REQUIRE((long long)(nv + nvs) * (long long)cap * 4ll < (1ll << 32),
               "slx_collision_pass: too many queries x candidates for 32-bit order keys");
+  const bool rec = pair_rec != nullptr;
+  SLX_REQUIRE(!rec || (pair_cnt && pair_cap >= 0), "slx_collision_pass: pair records need pair_cnt and pair_cap");
   if (ws_bytes < slx_collision_ws_bytes(nv, nvs, rec_cap))
     return fail(SLX_EWS, "slx_collision_pass: workspace too small");
   hipStream_t st = (hipStream_t)stream;
-  CollisionWs w;
-  layout(nv, rec_cap, (char*)ws, &w);
-  QuerySets Q{nv, c_tv_st ? nvs : 0, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
+  ContactWs w;
+  contact_ws_layout(nv, rec_cap, (char*)ws, &w);
+  QuerySets Q{nv, nvs, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
   const int nqt = 2 * nv + Q.nvs;
   SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
-  SLX_LAUNCH("contacts", k_contacts, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-             (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec, w.self_acc,
-             pair_num, status);
+  if (rec)
+    SLX_LAUNCH("contacts", k_contacts<true>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
+               (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
+               w.self_acc, pair_num, pair_cap, pair_cnt, (float4*)pair_rec, status);
+  else
+    SLX_LAUNCH("contacts", k_contacts<false>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
+               (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
+               w.self_acc, pair_num, 0, nullptr, nullptr, status);
   SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt,
              w.brec, w.self_acc, fixed, (float4*)y4);
   return check_launch("slx_collision_pass");

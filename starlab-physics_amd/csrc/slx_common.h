@@ -22,6 +22,7 @@ constexpr int kStatusCandOverflow = SLX_STATUS_CAND_OVERFLOW;
 constexpr int kStatusStackOverflow = SLX_STATUS_STACK_OVERFLOW;
 constexpr int kStatusCellRange = SLX_STATUS_CELL_RANGE;
 constexpr int kStatusRecOverflow = SLX_STATUS_REC_OVERFLOW;
+constexpr int kStatusPairOverflow = SLX_STATUS_PAIR_OVERFLOW;
 
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -48,6 +49,51 @@ __device__ __forceinline__ bool overlap(float3 amin, float3 amax, float3 bmin, f
 }
 
 inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Per-vertex contact-delta buckets (position and velocity passes share them):
+// bcnt[v] claimed slots, brec[v * bucket + i] = (delta, order key bits),
+// self_acc[v] = (deltas the vertex's own first-set query summed in order, count).
+struct ContactWs {
+  int32_t* bcnt;
+  float4* self_acc;
+  float4* brec;
+};
+
+inline size_t contact_ws_layout(int nv, int bucket, char* base, ContactWs* w) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return base ? base + at : nullptr;
+  };
+  ContactWs t;
+  t.bcnt = (int32_t*)take(4 * (size_t)nv);
+  t.self_acc = (float4*)take(16 * (size_t)nv);
+  t.brec = (float4*)take(16 * (size_t)nv * (size_t)(bucket > 0 ? bucket : 1));
+  if (w) *w = t;
+  return o;
+}
+
+// acc + the n bucket records r[0..n) added in increasing order key (.w bits);
+// keys are distinct, so the result is that of a serial scatter in key order
+__device__ __forceinline__ float3 bucket_sum(float3 acc, const float4* __restrict__ r, int n) {
+  uint32_t last = 0;
+  for (int i = 0; i < n; ++i) {
+    uint32_t best = 0xffffffffu;
+    int bj = 0;
+    for (int j = 0; j < n; ++j) {
+      const uint32_t o = __float_as_uint(r[j].w);
+      if ((i == 0 || o > last) && o < best) {
+        best = o;
+        bj = j;
+      }
+    }
+    last = best;
+    const float4 d = r[bj];
+    acc = make_float3(acc.x + d.x, acc.y + d.y, acc.z + d.z);
+  }
+  return acc;
+}
 
 // error reporting (thread-local message; see abi.hip)
 int fail(int code, const char* fmt, ...);

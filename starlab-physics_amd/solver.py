@@ -10,6 +10,9 @@ Extensions (all default to the reference's behaviour):
                                    per triangle (needed for large cloths)
   refit_every_substep              triangle tree: refit from x at each substep
   cache_size (40)                  candidate capacity per query (XPBD.py:51)
+  pair_cache_size (None)           contacting pairs kept per query for the
+                                   velocity pass (None = cache_size, which never
+                                   drops one; a smaller value saves memory)
   status()                         device status word (overflow flags)
 `step` is an alias of `forward`.
 """
@@ -56,6 +59,7 @@ class Solver:
         self.bvh_mode = "cell"
         self.refit_every_substep = False
         self.cache_size = 40
+        self.pair_cache_size = None
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
@@ -110,8 +114,10 @@ class Solver:
     # --------------------------------------------------------------- internals
     def _collision_buffers(self):
         import torch
+        pcap = self.cache_size if self.pair_cache_size is None else int(self.pair_cache_size)
         key = (self.bvh_mode, self.cache_size, self.records_per_vertex)
         if self._cd is not None and self._cd["key"] == key:
+            self._pair_buffers(self._cd, pcap)
             return self._cd
         m, s = self.mesh_dy, self.mesh_st
         nv, nvs = m.num_verts, (s.num_verts if s is not None else 0)
@@ -129,7 +135,19 @@ class Solver:
         if s is not None:
             s.xs4 = s.x4
         self._cd = cd
+        self._pair_buffers(cd, pcap)
         return cd
+
+    def _pair_buffers(self, cd, pcap):
+        """Pair records for the velocity pass (allocated once it is enabled)."""
+        import torch
+        if not self.enable_velocity_update or cd.get("pair_cap") == pcap:
+            return
+        m, s = self.mesh_dy, self.mesh_st
+        nq = 2 * m.num_verts + (s.num_verts if s is not None else 0)
+        cd["pair_cap"] = pcap
+        cd["pair_cnt"] = torch.zeros(nq + 1, dtype=torch.int32, device=self.device)
+        cd["pair_rec"] = torch.zeros(max(16 * pcap * nq, 16), dtype=torch.float32, device=self.device)
 
     def _desc(self, n_substeps, n_iters):
         import torch
@@ -168,6 +186,13 @@ class Solver:
             d.dyn = cd["dyn"].struct()
             d.cap, d.rec_cap = self.cache_size, cd["rec_cap"]
             d.cand, d.cand_n, d.pair_num = P(cd["cand"]), P(cd["cand_n"]), P(cd["pair_num"])
+            if self.enable_velocity_update:
+                d.pair_cap, d.pair_cnt, d.pair_rec = cd["pair_cap"], P(cd["pair_cnt"]), P(cd["pair_rec"])
+                if s is not None:
+                    d.vs4, d.m_inv_st = P(s.v4), P(s.m_inv)
+        # velocity pass (XPBD.py:467-468)
+        d.velocity = 1 if self.enable_velocity_update else 0
+        d.mu = _f32(self.mu)
         need = _abi.lib().slx_frame_ws_bytes(_abi.C.byref(d))
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
@@ -209,6 +234,20 @@ class Solver:
         m = self.mesh_dy
         _abi.call("slx_update_vx", m.num_verts, _abi.ptr(m.x4), _abi.ptr(m.v4), _abi.ptr(m.y4),
                   _abi.ptr(m.fixed), _f32(self.damping), _f32(dt), _abi.stream_handle())
+
+    def pairs(self):
+        """Last recorded contacting pairs (velocity pass on): dict with cnt [nq],
+        and per pair j < cnt[q] of query q: g [nq, pair_cap, 4, 3], schur, face,
+        type (queries in the order vt_st, vt_dy, tv_st)."""
+        import torch
+        cd = self._cd
+        pc = cd["pair_cap"]
+        nq = cd["pair_cnt"].numel() - 1
+        r = cd["pair_rec"][:16 * pc * nq].view(pc, 4, nq, 4).permute(2, 0, 1, 3)
+        g = r[..., :3]
+        return dict(cnt=cd["pair_cnt"][:nq], g=g, schur=r[:, :, 0, 3],
+                    face=r[:, :, 1, 3].contiguous().view(torch.int32),
+                    type=r[:, :, 2, 3].contiguous().view(torch.int32))
 
     def candidates(self):
         """Last broad-phase candidate lists: dict set -> (cand (cap, nq), counts)."""

@@ -45,8 +45,16 @@ def test_version_and_host_argument_checks(lib):
                           None, None) == -1
     with pytest.raises(lib.SlxError):
         lib.call("slx_update_vx", 3, None, None, None, None, 0.0, 0.0, None)
+    assert L.slx_velocity_pass(-1, 0, None, None, None, None, None, None, None, 0.8, 40, None, None, None, 0.0, 1,
+                               None, None, 0, None) == -1
+    assert "bad sizes" in lib.last_error()
+    assert L.slx_velocity_pass(4, 0, None, None, None, None, None, None, None, 0.8, 40, None, None, None, 0.0, 1,
+                               None, None, 0, None) == -1
+    assert "null pointer" in lib.last_error()
     # zero-sized calls are no-ops
     assert L.slx_compute_y(0, None, None, None, 0.0, 0.0, 0.0, 0.0, None) == 0
+    assert L.slx_velocity_pass(0, 0, None, None, None, None, None, None, None, 0.8, 40, None, None, None, 0.0, 1,
+                               None, None, 0, None) == 0
 
 
 def test_workspace_queries_are_host_only(lib):

@@ -272,7 +272,7 @@ def test_collision_pass_bit_exact(slx, orc, rng):
     P = A.ptr
     A.call("slx_collision_pass", nv, nvs, P(cloth.face_indices), P(obst.face_indices), P(cloth.y4), P(obst.x4),
            P(cloth.fixed), P(cloth.m_inv), float(np.float32(dhat)), float(np.float32(1e8)), cap, P(c0), P(n0),
-           P(c1), P(n1), P(c2), P(n2), P(pn), rec_cap, P(st), P(ws), ws.numel(), A.stream_handle())
+           P(c1), P(n1), P(c2), P(n2), P(pn), 0, None, None, rec_cap, P(st), P(ws), ws.numel(), A.stream_handle())
     yo = y.copy()
     to_host = lambda c, n: (np.ascontiguousarray(c.cpu().numpy().T), n.cpu().numpy())
     out = orc.collision_pass(dict(faces=cloth.fid_np), obst.fid_np, yo, obst.positions(), fixed,
@@ -313,10 +313,12 @@ def collision_scene(slx, n=32, mode="cell", refit=False, layers=1, seed=0):
 
 def orc_cloth(cloth, obst, sim):
     from oracle import oracle as orc
-    return orc.OracleCloth(cloth.host_dict(), dict(x=obst.positions(), faces=obst.fid_np), dhat=sim.dHat,
-                           stiffness_stretch=sim.stiffness_stretch, stiffness_bending=sim.stiffness_bending,
-                           g=sim.g, dt=sim.dt, padding=sim.padding, collision=True, bvh=sim.bvh_mode,
-                           cap=sim.cache_size, refit_every_substep=sim.refit_every_substep)
+    return orc.OracleCloth(cloth.host_dict(), dict(x=obst.positions(), faces=obst.fid_np, m_inv=obst.m_inv_np),
+                           dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
+                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
+                           collision=True, bvh=sim.bvh_mode, cap=sim.cache_size,
+                           refit_every_substep=sim.refit_every_substep, velocity=sim.enable_velocity_update,
+                           mu=sim.mu)
 
 
 @pytest.mark.parametrize("mode,refit,layers,iters", [("cell", False, 1, 1), ("tri", False, 1, 1),
