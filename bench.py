@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--substeps", type=int, default=20)
     p.add_argument("--iters", type=int, default=2)
     p.add_argument("--no-collision", action="store_true")
+    p.add_argument("--velocity", action="store_true",
+                   help="enable the velocity-level contact and friction pass (enable_velocity_update)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--copy-probe", action="store_true",
@@ -94,12 +96,14 @@ def build_scene(slx, n, collision):
 def cpu_baseline(cloth, terr, sim, iters, seconds):
     """The serial FP32 oracle (1 core) on the same scene: whole substeps
     (predict, `iters` x [spring pass, broad phase, collision pass],
-    velocity/position update), repeated for about `seconds`."""
+    velocity/position update, the velocity pass when enabled), repeated for
+    about `seconds`."""
     from oracle import oracle as orc
-    st = None if terr is None else dict(x=terr.positions(), faces=terr.fid_np)
+    st = None if terr is None else dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
     ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt / 20, padding=sim.padding,
-                          collision=terr is not None, bvh="tri", cap=sim.cache_size, refit_every_substep=True)
+                          collision=terr is not None, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
+                          velocity=sim.enable_velocity_update, mu=sim.mu)
     t0 = time.perf_counter()
     reps = 0
     while True:
@@ -109,9 +113,30 @@ def cpu_baseline(cloth, terr, sim, iters, seconds):
         if el >= seconds:
             break
     what = "with terrain + self collision" if terr is not None else "collision off"
+    if sim.enable_velocity_update:
+        what += ", velocity pass"
     return dict(value=cloth.num_verts * reps / el, unit="vertex-substeps/s", cores=1, kind="port",
                 sample=f"{reps} substeps x {iters} Jacobi iterations of the {cloth.num_verts}-vertex cloth ({what}) "
                        f"on the serial FP32 oracle, {el:.1f} s")
+
+
+def velocity_counts(sim, V):
+    """Contacting pairs of the last substep and the bucket deltas they give
+    (two per moving point of a second- or third-set pair; every pair counted,
+    whatever the sign of its Cv), for the velocity pass's byte model."""
+    pr = sim.pairs()
+    cnt = pr["cnt"].cpu().numpy()
+    P = int(cnt.sum())
+    types = pr["type"].cpu().numpy()
+    moving = np.array([2, 2, 2, 3, 3, 3, 4])          # points of a vertex-triangle pair per type
+    R = 0
+    for q0, q1, pts in ((V, 2 * V, moving), (2 * V, len(cnt), moving - 1)):
+        c = cnt[q0:q1]
+        if c.size == 0 or c.max() == 0:
+            continue
+        j = np.arange(types.shape[1])[None, :] < c[:, None]
+        R += int(2 * pts[np.clip(types[q0:q1], 0, 6)][j].sum())
+    return dict(pairs=P, vel_records=R)
 
 
 def main():
@@ -130,6 +155,7 @@ def main():
     abi = importlib.import_module("starlab-physics_amd._abi")
     collision = not a.no_collision
     cloth, terr, sim = build_scene(slx, a.n, collision)
+    sim.enable_velocity_update = a.velocity
     V, E, B, F = cloth.num_verts, cloth.num_edges, cloth.bending_constraint_count, cloth.num_faces
     for _ in range(a.warmup):
         sim.forward(a.substeps, a.iters)
@@ -152,6 +178,8 @@ def main():
         for k, (c, n) in sim.candidates().items():
             cbar[k] = float(n.float().mean().item())
         counts.update(Vs=terr.num_verts, Fs=terr.num_faces, cbar=cbar)
+    if a.velocity and collision:
+        counts.update(velocity_counts(sim, V))
     value = V * a.substeps * a.steps / el
     dom = max(kern, key=lambda k: kern[k][0])
     tot_ms, cnt = kern[dom]
@@ -162,7 +190,7 @@ def main():
     # measured HBM bytes per launch of the dominant kernel (two rocprofv3 counter passes,
     # calibrated on a copy kernel), valid for the default workload
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
-    if collision and a.n == 1024 and a.substeps == 20 and a.iters == 2 and os.path.exists(tpath):
+    if collision and not a.velocity and a.n == 1024 and a.substeps == 20 and a.iters == 2 and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
         if tk:
             traffic = tk["fetch_bytes"] + tk["write_bytes"]
@@ -170,6 +198,8 @@ def main():
            f"{a.substeps} substeps x {a.iters} Jacobi iterations per frame, ")
     cfg += ("terrain (142x142 height field, 39762 triangles) + self vertex-triangle collision, triangle LBVH "
             "rebuilt per frame and refit per substep" if collision else "collision off")
+    if a.velocity:
+        cfg += ", velocity-level contact and friction pass (mu 0.8)"
     out = {
         "metric": "vertex-substeps/s",
         "value": value,
@@ -184,6 +214,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded grid cloth and sinusoid terrain)",
         "config": {"workload": cfg, "substeps_per_step": a.substeps, "iters": a.iters, "collision": collision,
+                   "velocity_pass": a.velocity,
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": (achieved / 8000.0) if achieved else None, "traffic": traffic, "kernel": dom,
@@ -192,6 +223,7 @@ def main():
                               / rl.HBM_PEAK_BYTES_PER_S,
         "status": status,
         "mean_candidates": cbar,
+        "velocity_pairs": {k: counts[k] for k in ("pairs", "vel_records") if k in counts},
         "kernels_ms": {k: {"total_ms": round(v[0], 4), "launches": v[1], "avg_us": round(v[0] / max(v[1], 1) * 1e3, 3)}
                        for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
     }

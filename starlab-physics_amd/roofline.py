@@ -16,6 +16,17 @@ candidates per query of a query set.
   narrow phase                 16 c V + 24 V  (per query set of V queries)
   contact apply                56 V   (counts, first-set sums, f; read/write y)
   per-frame rebuild            Morton 20 F + 12 V, radix sort 68 F, Karras 16 F
+
+Velocity pass (enable_velocity_update), with P contacting pairs recorded by the
+last collision pass of a substep and R bucket deltas (two per moving point of
+each second- and third-set pair whose Cv < 0; the benchmark counts every such
+pair, an upper bound):
+  pair records (contacts)      64 P + 4 nq   (g0..g3, schur, face, type; counts)
+  compute_velocity             52 V   (read x, y, f; write v)
+  velocity contacts            4 nq + 76 P + 16 (V + Vs) + 8 V + 16 V + 20 R
+                               (counts, records + face ids, velocities, f and w,
+                                first-set sums, bucket deltas + counters)
+  velocity apply               92 V + 16 R   (counts, sums, f, w; read/write v and x; buckets)
 """
 from __future__ import annotations
 
@@ -55,6 +66,22 @@ def rebuild(V, F):
     return (20 * F + 12 * V) + 68 * F + 16 * F
 
 
+def pair_records(P, nq):
+    return 64 * P + 4 * nq
+
+
+def compute_velocity(V):
+    return 52 * V
+
+
+def vel_contacts(nq, P, V, Vs, R):
+    return 4 * nq + 76 * P + 16 * (V + Vs) + 8 * V + 16 * V + 20 * R
+
+
+def vel_apply(V, R):
+    return 92 * V + 16 * R
+
+
 def kernel_bytes(name: str, n: dict):
     """Algorithmic bytes of one launch of kernel `name` (None if not modelled)."""
     V, E, B, F = n["V"], n["E"], n["B"], n["F"]
@@ -79,6 +106,12 @@ def kernel_bytes(name: str, n: dict):
         nar += narrow(V, c.get("vt_st", 0.0)) + narrow(n["Vs"], c.get("tv_st", 0.0))
     table["contacts"] = nar
     table["contact_apply"] = contact_apply(V)
+    if "pairs" in n:
+        Vs = n.get("Vs", 0)
+        nq = 2 * V + Vs
+        table["compute_velocity"] = compute_velocity(V)
+        table["vel_contacts"] = vel_contacts(nq, n["pairs"], V, Vs, n["vel_records"])
+        table["vel_apply"] = vel_apply(V, n["vel_records"])
     return table.get(name)
 
 
@@ -95,6 +128,13 @@ def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
             per_iter += narrow(n["Vs"], c.get("tv_st", 0.0))
             total += substeps * traversal(n["Vs"], F, c.get("tv_st", 0.0))
         total += substeps * (refit(V, F) + n_iters * per_iter) + rebuild(V, F)
+    if "pairs" in n:
+        # velocity pass: compute_velocity + contacts + apply (update_x fused) replace update_vx
+        Vs = n.get("Vs", 0)
+        nq = 2 * V + Vs
+        R = n["vel_records"]
+        total += substeps * (pair_records(n["pairs"], nq) + compute_velocity(V) +
+                             vel_contacts(nq, n["pairs"], V, Vs, R) + vel_apply(V, R) - update_vx(V))
     return total
 
 
