@@ -38,6 +38,11 @@ def parse():
     p.add_argument("--substeps", type=int, default=20)
     p.add_argument("--iters", type=int, default=2)
     p.add_argument("--no-collision", action="store_true")
+    p.add_argument("--no-cache", action="store_true", help="walk the trees at every pass (no candidate cache)")
+    p.add_argument("--cache-margin", type=float, default=None,
+                   help="candidate cache margin as a multiple of the padding (default: the solver's)")
+    p.add_argument("--cache-every", type=int, default=None, help="substeps between candidate cache rebuilds")
+    p.add_argument("--walk-stack", type=int, default=None, help="first-tier triangle walk stack depth (16/24/32/64)")
     p.add_argument("--velocity", action="store_true",
                    help="enable the velocity-level contact and friction pass (enable_velocity_update)")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -156,11 +161,30 @@ def main():
     collision = not a.no_collision
     cloth, terr, sim = build_scene(slx, a.n, collision)
     sim.enable_velocity_update = a.velocity
+    sim.candidate_cache = not a.no_cache
+    if a.walk_stack is not None:
+        abi.call("slx_set_walk_stack", a.walk_stack)
+    if a.cache_margin is not None:
+        sim.cache_margin = a.cache_margin * sim.padding
+    if a.cache_every is not None:
+        sim.cache_every = a.cache_every
     V, E, B, F = cloth.num_verts, cloth.num_edges, cloth.bending_constraint_count, cloth.num_faces
     for _ in range(a.warmup):
         sim.forward(a.substeps, a.iters)
     torch.cuda.synchronize()
 
+    # one profiling frame with every launch timed: the per-kernel breakdown, and
+    # the dominant kernel (largest total time)
+    abi.timer_only(None)
+    abi.timer_enable(4000 + 64)
+    sim.forward(a.substeps, a.iters)
+    torch.cuda.synchronize()
+    kern = {n: abi.timer_read(n) for n in abi.timer_names()}
+    abi.timer_enable(0)
+    dom = max(kern, key=lambda k: kern[k][0])
+    # the timed region: whole frames, with HIP events around the dominant kernel's
+    # launches only (an event pair per launch of every kernel would slow the frame)
+    abi.timer_only(dom)
     abi.timer_enable(4000 * a.steps + 64)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -168,8 +192,9 @@ def main():
         sim.forward(a.substeps, a.iters)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    kern = {n: abi.timer_read(n) for n in abi.timer_names()}
+    dom_timed = abi.timer_read(dom)
     abi.timer_enable(0)
+    abi.timer_only(None)
     status = sim.status()
 
     counts = dict(V=V, E=E, B=B, F=F)
@@ -180,9 +205,16 @@ def main():
         counts.update(Vs=terr.num_verts, Fs=terr.num_faces, cbar=cbar)
     if a.velocity and collision:
         counts.update(velocity_counts(sim, V))
+    caches = (sim._cd or {}).get("caches") if collision else None
+    if caches and (sim.cache_every != 1 or a.iters > 1):
+        cs = {}
+        for k, c in zip(("vt_st", "vt_dy"), caches[:2]):
+            if c is not None and c.count is not None:
+                cnt = c.count[:c.nq].clamp(max=c.cap).float()
+                cs[k] = float(cnt.mean().item())
+        counts["cache"] = cs
     value = V * a.substeps * a.steps / el
-    dom = max(kern, key=lambda k: kern[k][0])
-    tot_ms, cnt = kern[dom]
+    tot_ms, cnt = dom_timed
     avg_s = tot_ms / cnt / 1e3
     nbytes = rl.kernel_bytes(dom, counts)
     achieved = nbytes / avg_s / 1e9 if nbytes else None
@@ -190,7 +222,9 @@ def main():
     # measured HBM bytes per launch of the dominant kernel (two rocprofv3 counter passes,
     # calibrated on a copy kernel), valid for the default workload
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
-    if collision and not a.velocity and a.n == 1024 and a.substeps == 20 and a.iters == 2 and os.path.exists(tpath):
+    default_run = (collision and not a.velocity and not a.no_cache and a.cache_margin is None and a.cache_every is None
+                   and a.walk_stack is None and a.n == 1024 and a.substeps == 20 and a.iters == 2)
+    if default_run and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
         if tk:
             traffic = tk["fetch_bytes"] + tk["write_bytes"]
@@ -214,7 +248,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded grid cloth and sinusoid terrain)",
         "config": {"workload": cfg, "substeps_per_step": a.substeps, "iters": a.iters, "collision": collision,
-                   "velocity_pass": a.velocity,
+                   "velocity_pass": a.velocity, "candidate_cache": sim.candidate_cache,
+                   "cache_margin_m": 0.25 * sim.padding if sim.cache_margin is None else sim.cache_margin,
+                   "cache_every": sim.cache_every,
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": (achieved / 8000.0) if achieved else None, "traffic": traffic, "kernel": dom,
@@ -223,9 +259,11 @@ def main():
                               / rl.HBM_PEAK_BYTES_PER_S,
         "status": status,
         "mean_candidates": cbar,
+        "mean_cached_leaves": counts.get("cache", {}),
         "velocity_pairs": {k: counts[k] for k in ("pairs", "vel_records") if k in counts},
-        "kernels_ms": {k: {"total_ms": round(v[0], 4), "launches": v[1], "avg_us": round(v[0] / max(v[1], 1) * 1e3, 3)}
-                       for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms_profile_frame": {k: {"total_ms": round(v[0], 4), "launches": v[1],
+                                         "avg_us": round(v[0] / max(v[1], 1) * 1e3, 3)}
+                                     for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
     }
     if a.copy_probe:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

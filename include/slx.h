@@ -59,6 +59,9 @@ const char* slx_last_error(void);
  * max_launches = 0 disables the timer. */
 int slx_timer_enable(int max_launches);
 int slx_timer_read(const char* name, double* total_ms, int* count);
+/* Time only the launches named `name` (null or "": all), so that the timed
+ * region carries one event pair per launch of that kernel and nothing else. */
+int slx_timer_only(const char* name);
 int slx_timer_names(char* buf, int buf_len);
 
 /* ---------------- XPBD Jacobi solve (framework/physics/XPBD.py) ---------------- */
@@ -217,6 +220,43 @@ int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float*
                   int32_t* sorted_inst, int32_t* child_a, int32_t* child_b, int32_t* parent,
                   int32_t* range_l, int32_t* range_r, int32_t* roots, int32_t* sched, float* nlo4,
                   float* nhi4, float* quad4, void* ws, size_t ws_bytes, slx_stream_t stream);
+/* Candidate cache of one query set against a triangle tree (no reference
+ * counterpart; the lists it gives are the traversal's, bit for bit).  Within a
+ * frame the tree keeps its topology and its walks emit leaves in one fixed
+ * order, so a walk with the query box enlarged by `margin` can store each
+ * query's leaves, and later passes filter them by the exact box instead of
+ * walking: exact while the query moved d and the tree's vertices D since the
+ * cache was built with d + D below the margin (less a rounding allowance).
+ * Queries the cache cannot serve are walked in full. */
+typedef struct slx_cand_cache {
+  int32_t cap;                           /* leaves stored per query */
+  float margin;                          /* enlargement of the query box (length) */
+  int32_t* leaves;                       /* [cap * nq] leaf node ids, column-major */
+  int32_t* count;                        /* [nq] leaves found (> cap or INT_MAX: incomplete) */
+  float* q0;                             /* float4 [nq] query positions at the build */
+  int32_t* redo;                         /* [nq + 1] queries walked in full ([0] = how many) */
+} slx_cand_cache;
+
+/* Exact candidate lists of nq queries against a triangle tree (as slx_traverse
+ * mode 1) in two tiers: a first walk with a shallow stack (more waves in flight)
+ * and, for the queries it cannot serve, a walk with the full stack.  kind 0 walks
+ * (only cache->redo is used); kind 1 walks with the enlarged box and fills the
+ * cache (call it right after the tree's snapshot, below); kind 2 filters the
+ * cache, disp (device, nullable = 0) being the tree's vertex displacement since
+ * the cache was filled.  order (nullable): processing order of the queries, the
+ * same for every call on one cache (its entries are indexed by that order). */
+int slx_traverse_cached(int kind, int nq, const float* q4, float padding, int n_leaves,
+                        const float* nlo4, const float* nhi4, const float* quad4,
+                        const slx_cand_cache* cache, const uint32_t* disp, int cap, int32_t* cand,
+                        int32_t* cand_n, int32_t* status, const int32_t* order, slx_stream_t stream);
+/* Stack depth of the first-tier walk: 16, 24, 32 (default) or 64 entries. */
+int slx_set_walk_stack(int depth);
+/* x_snap = x (float4 [nv]); disp = 0. */
+int slx_tree_snapshot(int nv, const float* x4, float* x_snap, uint32_t* disp, slx_stream_t stream);
+/* disp = max over vertices of |x - x_snap|_inf (float bits; infinity for NaN). */
+int slx_tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp,
+                          slx_stream_t stream);
+
 /* Traversal of a batched tree: query q starts at roots[q_inst[q]]. */
 int slx_traverse_seg(int nq, const float* q4, const int32_t* q_inst, const int32_t* roots,
                      float padding, int nf, const float* nlo4, const float* nhi4, const float* quad4,
@@ -384,6 +424,10 @@ typedef struct slx_tree {
   const int32_t* vert_order; /* [nv] order in which this tree's mesh's vertices are processed as queries
                                 (nullable; e.g. Morton order, so that a wavefront's queries are neighbours) */
   float* quad4;              /* triangle: 4-wide node array, float4 [8 (nf-1)] (nullable, see below) */
+  /* candidate caches over this tree (slx_frame): its vertices at the build and
+     their displacement since (float bits); nullable */
+  float* x_snap;             /* float4 [nv] */
+  uint32_t* disp;            /* [1] */
 } slx_tree;
 
 typedef struct slx_frame_desc {
@@ -430,6 +474,12 @@ typedef struct slx_frame_desc {
   int32_t pair_cap;                      /* contacting pairs kept per query */
   int32_t* pair_cnt;                     /* [2nv + nvs] */
   float* pair_rec;                       /* float4 [4 pair_cap (2nv + nvs)] */
+  /* candidate caches of the query sets vt_st, vt_dy, tv_st (triangle trees of
+     single meshes; cap 0 = walk every pass).  Rebuilt at the first query of each
+     set in every cache_every-th substep (<= 0: once per frame); the dynamic tree
+     needs x_snap and disp. */
+  slx_cand_cache cache[3];
+  int32_t cache_every;
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

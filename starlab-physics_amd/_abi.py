@@ -25,6 +25,7 @@ SIGNATURES = {
     "slx_timer_enable": [I32],
     "slx_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "slx_timer_names": [C.c_char_p, I32],
+    "slx_timer_only": [C.c_char_p],
     "slx_compute_y": [I32, PTR, PTR, PTR, F32, F32, F32, F32, STREAM],
     "slx_spring_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, STREAM],
     "slx_update_vx": [I32, PTR, PTR, PTR, PTR, F32, F32, STREAM],
@@ -52,6 +53,10 @@ SIGNATURES = {
                       C.c_size_t, STREAM],
     "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
     "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_traverse_cached": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, PTR, STREAM],
+    "slx_set_walk_stack": [I32],
+    "slx_tree_snapshot": [I32, PTR, PTR, PTR, STREAM],
+    "slx_tree_displacement": [I32, PTR, PTR, PTR, STREAM],
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
                            PTR, PTR, I32, PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
@@ -93,7 +98,12 @@ class Tree(C.Structure):
                 ("leaf_r", PTR), ("nlo4", PTR), ("nhi4", PTR), ("n_inst", I32), ("vert_base", PTR),
                 ("face_base", PTR), ("face_inst", PTR), ("vert_inst", PTR), ("sorted_inst", PTR),
                 ("range_l", PTR), ("range_r", PTR), ("roots", PTR), ("sched", PTR),
-                ("vert_order", PTR), ("quad4", PTR)]
+                ("vert_order", PTR), ("quad4", PTR), ("x_snap", PTR), ("disp", PTR)]
+
+
+class CandCache(C.Structure):
+    """slx_cand_cache (include/slx.h)."""
+    _fields_ = [("cap", I32), ("margin", F32), ("leaves", PTR), ("count", PTR), ("q0", PTR), ("redo", PTR)]
 
 
 class FrameDesc(C.Structure):
@@ -108,7 +118,7 @@ class FrameDesc(C.Structure):
                 ("gs_color_off", PTR), ("n_colors", I32), ("dyn", Tree), ("st", Tree),
                 ("cand", PTR), ("cand_n", PTR), ("pair_num", PTR), ("status", PTR), ("ws", PTR),
                 ("ws_bytes", C.c_size_t), ("velocity", I32), ("mu", F32), ("vs4", PTR), ("m_inv_st", PTR),
-                ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR)]
+                ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR), ("cache", CandCache * 3), ("cache_every", I32)]
 
 
 class SlxError(RuntimeError):
@@ -160,6 +170,10 @@ def stream_handle(stream=None) -> int:
 
 def timer_enable(max_launches: int) -> None:
     call("slx_timer_enable", max_launches)
+
+
+def timer_only(name=None) -> None:
+    call("slx_timer_only", name.encode() if name else None)
 
 
 def timer_read(name: str):

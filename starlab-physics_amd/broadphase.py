@@ -34,6 +34,28 @@ def cell_topology(device):
     return _TOPO[key]
 
 
+class CandidateCache:
+    """Device buffers of one query set's candidate cache (slx_cand_cache)."""
+
+    def __init__(self, nq, cap, margin, device):
+        """cap 0: no cache, only the redo list of the two-tier walk."""
+        i32 = dict(dtype=torch.int32, device=device)
+        self.nq, self.cap, self.margin = nq, cap, margin
+        self.leaves = torch.zeros(max(cap * nq, 1), **i32) if cap > 0 else None
+        self.count = torch.zeros(max(nq, 1), **i32) if cap > 0 else None
+        self.q0 = torch.zeros((max(nq, 1), 4), dtype=torch.float32, device=device) if cap > 0 else None
+        self.redo = torch.zeros(nq + 1, **i32)
+
+    def struct(self) -> _abi.CandCache:
+        P = _abi.ptr
+        return _abi.CandCache(int(self.cap), float(self.margin), P(self.leaves), P(self.count), P(self.q0),
+                              P(self.redo))
+
+    def redone(self) -> int:
+        """Queries of the last pass walked in full (synchronises)."""
+        return int(self.redo[0].item())
+
+
 class DeviceTree:
     def __init__(self, mesh, mode="cell"):
         if mode not in ("cell", "tri"):
@@ -94,6 +116,15 @@ class DeviceTree:
         ws = max(ws, _abi.lib().slx_mesh_aabb_ws_bytes())
         self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
         self.status = torch.zeros(1, **i32)
+        # candidate caches over this tree: vertices at the build and their displacement since
+        self.x_snap = None
+        self.disp = None
+
+    def enable_snapshot(self):
+        if self.x_snap is None:
+            self.x_snap = torch.zeros((max(self.mesh.num_verts, 1), 4), dtype=torch.float32,
+                                      device=self.mesh.x4.device)
+            self.disp = torch.zeros(1, dtype=torch.int32, device=self.mesh.x4.device)
 
     def struct(self) -> _abi.Tree:
         P = _abi.ptr
@@ -103,6 +134,7 @@ class DeviceTree:
                       P(self.nlo4), P(self.nhi4))
         t.range_l, t.range_r, t.sched = P(self.range_l), P(self.range_r), P(self.sched)
         t.quad4 = P(self.quad4)
+        t.x_snap, t.disp = P(self.x_snap), P(self.disp)
         t.vert_order = P(getattr(self.mesh, "vert_order", None))
         if self.n_inst > 1:
             I = self.inst

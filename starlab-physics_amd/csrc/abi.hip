@@ -38,6 +38,7 @@ static std::vector<hipEvent_t> g_pool;
 static std::vector<TimerRec> g_recs;
 static size_t g_cap = 0;
 static bool g_on = false;
+static std::string g_only;        // non-empty: time only the launches of this name
 
 bool timer_on() { return g_on; }
 
@@ -46,6 +47,7 @@ void timer_begin(const char* name, hipStream_t s, int* slot) {
   if (!g_on) return;
   std::lock_guard<std::mutex> lk(g_tmu);
   if (g_recs.size() >= g_cap) return;
+  if (!g_only.empty() && g_only != name) return;
   size_t i = g_recs.size();
   TimerRec r{name, g_pool[2 * i], g_pool[2 * i + 1]};
   if (hipEventRecord(r.a, s) != hipSuccess) return;
@@ -76,6 +78,13 @@ extern "C" int slx_timer_enable(int max_launches) {
   g_cap = (size_t)max_launches;
   g_recs.clear();
   g_on = max_launches > 0;
+  return SLX_OK;
+}
+
+extern "C" int slx_timer_only(const char* name) {
+  using namespace slx;
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_only = name ? name : "";
   return SLX_OK;
 }
 

@@ -83,14 +83,34 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
                        t.range_l, t.range_r, t.sched, t.nlo4, t.nhi4, t.quad4, ws, ws_bytes, s);
 }
 
+// a cache struct with its redo list enables the two-tier walk; with leaves too,
+// the cache proper
+bool tiered(const slx_tree& t, const slx_cand_cache& c) {
+  return t.mode == 1 && !batched(t) && t.nf > 0 && c.redo != nullptr;
+}
+bool cacheable(const slx_tree& t, const slx_cand_cache& c) { return tiered(t, c) && c.cap > 0 && c.leaves; }
+
 // queries of a mesh whose vertices belong to instances q_inst (batched trees only)
 // q_order: processing order of the query vertices (nullable)
+// cache (nullable): the set's cache struct; use_cache: filled when *built is
+// false, then filtered; otherwise only its redo list serves the two-tier walk.
+// disp: the tree's displacement since the cache was built (nullable = 0)
 int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst,
           const int32_t* q_order, float pad, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
-          slx_stream_t s) {
+          slx_stream_t s, const slx_cand_cache* cache = nullptr, bool use_cache = false, bool* built = nullptr,
+          const uint32_t* disp = nullptr) {
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
   const bool seg = batched(t);
+  if (cache && tiered(t, *cache)) {
+    int kind = 0;
+    if (use_cache && cacheable(t, *cache)) {
+      kind = *built ? 2 : 1;
+      *built = true;
+    }
+    return traverse_cached(name, kind, nq, q4, pad, nl, t.nlo4, t.nhi4, t.quad4, cache, disp, cap, cand, cand_n,
+                           status, (hipStream_t)s, q_order);
+  }
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
                          t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,
                          (hipStream_t)s, seg ? t.roots : nullptr, seg ? q_inst : nullptr, q_order);
@@ -153,6 +173,17 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     rc = build_tree(d->dyn, nv, d->x4, d->faces, pad, d->status, d->ws, d->ws_bytes, stream);
     if (rc) return rc;
   }
+  // candidate caches: rebuilt at each set's first query of the frame, against the
+  // dynamic tree's vertices as they are now (x_snap) and its boxes as built
+  // a cache pays off only when it is filtered at least once before its rebuild:
+  // the cloth-vertex sets are queried n_iters times per substep, the
+  // obstacle-vertex set once
+  const bool reuse_v = d->cache_every != 1 || d->n_iters > 1, reuse_t = d->cache_every != 1;
+  const bool dyn_cached = coll && ((reuse_v && cacheable(d->dyn, d->cache[1])) ||
+                                   (reuse_t && cacheable(d->dyn, d->cache[2])));
+  SLX_REQUIRE(!dyn_cached || (d->dyn.x_snap && d->dyn.disp), "slx_frame: cached queries of the dynamic tree need "
+              "its x_snap and disp");
+  bool built[3] = {false, false, false};
   if (nvs > 0 && d->rebuild_static) {
     rc = build_tree(d->st, nvs, d->xs4, d->faces_st, pad, d->status, d->ws, d->ws_bytes, stream);
     if (rc) return rc;
@@ -171,6 +202,15 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   }
   for (int s = 0; s < d->n_substeps; ++s) {
     bool refit_pending = false;
+    if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
+      // caches rebuilt in this substep, against x as it is now (the refit below
+      // builds the dynamic tree's boxes from this x)
+      built[0] = built[1] = built[2] = false;
+      if (dyn_cached) {
+        rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);
+        if (rc) return rc;
+      }
+    }
     if (refits && s > 0) {
       // fork: the refit reads x as the previous substep left it
       if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
@@ -179,6 +219,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                          d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4,
                          (slx_stream_t)side->s);
       if (rc) return rc;
+      if (dyn_cached) {                      // how far the refit boxes' vertices moved since the caches
+        rc = slx_tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, (slx_stream_t)side->s);
+        if (rc) return rc;
+      }
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
       refit_pending = true;
       tv_fresh = false;
@@ -212,7 +256,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
-                   n0, d->status, (slx_stream_t)side->q);
+                   n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
         if (rc) return rc;
         if (hipEventRecord(side->qjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
         static_pending = true;
@@ -220,11 +264,11 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       rc = join_refit();                     // the dynamic tree is needed from here on
       if (rc) return rc;
       rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
-                 n1, d->status, stream);
+                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp);
       if (rc) return rc;
       if (nvs > 0 && !tv_fresh) {
         rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
-                   d->status, stream);
+                   d->status, stream, &d->cache[2], reuse_t, &built[2], d->dyn.disp);
         if (rc) return rc;
         tv_fresh = true;
       }

@@ -34,7 +34,11 @@ int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* to
 #ifndef SLX_TRAV_STACK
 #define SLX_TRAV_STACK 64
 #endif
-constexpr int kStack = SLX_TRAV_STACK;   // traversal stack depth (entries), as the reference's 64
+constexpr int kStack = SLX_TRAV_STACK;   // traversal stack depth (entries), twice the reference's 32
+// Shallow stacks of the first-tier triangle walk (LDS per block = depth x 128 x 4
+// bytes, which bounds the waves in flight); queries that overflow one are walked
+// again with kStack entries.
+int g_walk_stack = 32;
 constexpr int kTravBS = 128;     // traversal block size
 
 // ----------------------------------------------------------------- AABBs
@@ -926,10 +930,10 @@ __global__ void __launch_bounds__(kTravBS)
 k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
                const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n,
                int32_t* status, const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst,
-               const int32_t* __restrict__ order) {
+               const int32_t* __restrict__ order, const int32_t* __restrict__ n_active) {
   __shared__ uint32_t s_stack[kStack * kTravBS];
   const int t = blockIdx.x * kTravBS + threadIdx.x;
-  if (t >= nq) return;
+  if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;     // spatially coherent queries share nodes within a wavefront
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   const float3 p = xyz(q4[q]);
@@ -995,30 +999,63 @@ k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, int nl, const f
 // overlaps it; every ancestor of such a leaf overlaps that box too, so the leaves a
 // lane emits, and their order, are exactly those of its own walk (the shared walk
 // visits a superset of its nodes in the same right-first order).
+struct CacheOut {
+  float pad_walk;          // padding of the enlarged box
+  int cap;                 // cache capacity per query
+  int32_t* leaves;         // [cap][nq] by slot
+  int32_t* count;          // [nq] by slot
+  float4* q0;              // [nq] by slot
+  int32_t* redo;           // [1 + nq]
+};
+
+constexpr uint32_t kExactBit = 1u << 31;   // quad walk, cache build: with kEmit, the leaf overlaps the exact box
+constexpr uint32_t kCacheNode = 1u << 31;  // quad walk, cache build: without kEmit, a leaf node for the cache
+
+// BUILD and n_active as for k_traverse_quad (the cache arrays in CacheOut).
+template <bool BUILD>
 __global__ void __launch_bounds__(256)
 k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
                     const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand,
-                    int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order) {
+                    int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
+                    const int32_t* __restrict__ n_active, CacheOut co) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const bool active = t < nq;
+  const bool active = t < nq && (!n_active || t < *n_active);
   const int q = active ? (order ? order[t] : t) : 0;
-  float3 a, b;
+  float3 a, b, aw, bw;
   if (active) {
     const float3 p = xyz(q4[q]);
     const float pp = pad * 1.0f;
     a = make_float3(p.x - pp, p.y - pp, p.z - pp);
     b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+    aw = a;
+    bw = b;
+    if (BUILD) {
+      const float pw = co.pad_walk;
+      aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
+      bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
+      co.q0[t] = make_float4(p.x, p.y, p.z, 0.f);
+    }
   } else {                                   // an empty box: overlaps nothing
     const float inf = __int_as_float(0x7f800000);
-    a = make_float3(inf, inf, inf);
-    b = make_float3(-inf, -inf, -inf);
+    a = aw = make_float3(inf, inf, inf);
+    b = bw = make_float3(-inf, -inf, -inf);
   }
-  int cnt = 0, flags = 0;
+  int cnt = 0, ccnt = 0, flags = 0;
   auto emit = [&](int f) {
     if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
     else flags |= kStatusCandOverflow;
     ++cnt;
+  };
+  // a leaf reached by the walk: to the cache (BUILD) and, if it overlaps the exact box, to the list
+  auto leaf = [&](int node, const float4& lo, const float4& hi) {
+    if (BUILD) {
+      if (ccnt < co.cap) co.leaves[(size_t)ccnt * nq + t] = node;
+      ++ccnt;
+      if (overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
+    } else {
+      emit(__float_as_int(lo.w));
+    }
   };
   const int leaf0 = nl - 1;
   auto expand_word = [&](int l, int r) { return (uint32_t)l | (r >= leaf0 ? kRightLeaf : 0u); };
@@ -1034,10 +1071,10 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, co
   };
   if (nl > 0) {
     const float4 lo = nlo[0], hi = nhi[0];
-    const bool ov = overlap(a, b, xyz(lo), xyz(hi));
+    const bool ov = overlap(aw, bw, xyz(lo), xyz(hi));
     if (__ballot(ov)) {
       if (leaf0 == 0) {
-        if (ov) emit(__float_as_int(lo.w));
+        if (ov) leaf(0, lo, hi);
       } else {
         push(expand_word(__float_as_int(lo.w), __float_as_int(hi.w)));
       }
@@ -1049,29 +1086,37 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, co
     if (e & kEmit) {                         // a left leaf, reported after its right sibling's subtree
       const int L = (int)(e & kLow30);
       const float4 lo = nlo[L], hi = nhi[L];
-      if (overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
+      if (overlap(aw, bw, xyz(lo), xyz(hi))) leaf(L, lo, hi);
       continue;
     }
     const int l = (int)(e & kLow30);
     const int split = l >= leaf0 ? l - leaf0 : l;
     const int r = (e & kRightLeaf) ? leaf0 + split + 1 : split + 1;
     const float4 l0 = nlo[l], l1 = nhi[l], r0 = nlo[r], r1 = nhi[r];
-    const bool ovl = overlap(a, b, xyz(l0), xyz(l1));
-    const bool ovr = overlap(a, b, xyz(r0), xyz(r1));
+    const bool ovl = overlap(aw, bw, xyz(l0), xyz(l1));
+    const bool ovr = overlap(aw, bw, xyz(r0), xyz(r1));
     if (__ballot(ovl))
       push(l >= leaf0 ? ((uint32_t)l | kEmit) : expand_word(__float_as_int(l0.w), __float_as_int(l1.w)));
     if (__ballot(ovr)) {
       if (r >= leaf0) {
-        if (ovr) emit(__float_as_int(r0.w));
+        if (ovr) leaf(r, r0, r1);
       } else {
         push(expand_word(__float_as_int(r0.w), __float_as_int(r1.w)));
       }
     }
   }
-  if (active) {
-    cand_n[q] = cnt < cap ? cnt : cap;
-    if (flags && status) atomicOr(status, flags);
+  if (!active) return;
+  if (BUILD) {
+    const bool over = flags & kStatusStackOverflow;
+    co.count[t] = over ? INT_MAX : ccnt;
+    if (over) {
+      co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
+      return;
+    }
+    flags &= ~kStatusStackOverflow;
   }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (flags && status) atomicOr(status, flags);
 }
 
 }  // namespace slx
@@ -1085,23 +1130,49 @@ namespace slx {
 // (the right subtree before the left, i.e. in decreasing leaf position), and the
 // set of emitted leaves is the same, since a grandchild overlaps the query box only
 // if its parent does (a parent's box is the union of its children's).
+//
+// BUILD (candidate cache, see k_cand_filter): the walk uses the box enlarged by
+// the cache's margin; every leaf it reaches goes to the cache (node ids, first
+// cache cap, indexed by processing slot t so a wavefront's accesses coalesce) and
+// those that overlap the exact box also to the exact list.  A leaf slot pushes
+// two words, its node id (tagged kCacheNode) and then its face (tagged kEmit, with
+// kExactBit when it overlaps the exact box), so both come off the stack together
+// without a load.  A stack overflow marks the cache entry incomplete and sends
+// the query to the redo list.
+// n_active (nullable, device): only the first *n_active entries of order[] are
+// walked (a list of queries built on the device).
+
+//
+// Stack depth S: with co.redo set (a first-tier walk), a query whose stack
+// overflows is appended to co.redo instead (its lists are then written by the
+// second walk); otherwise the overflow sets the status bit.
+template <bool BUILD, int S>
 __global__ void __launch_bounds__(kTravBS)
 k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
                 const float4* __restrict__ nhi, const float4* __restrict__ quad, int cap, int32_t* __restrict__ cand,
                 int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ roots,
-                const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order) {
-  __shared__ uint32_t s_stack[kStack * kTravBS];
+                const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order,
+                const int32_t* __restrict__ n_active, CacheOut co) {
+  __shared__ uint32_t s_stack[S * kTravBS];
   const int t = blockIdx.x * kTravBS + threadIdx.x;
-  if (t >= nq) return;
+  if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   const float3 p = xyz(q4[q]);
   const float pp = pad * 1.0f;
   const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
-  int cnt = 0, flags = 0;
+  float3 aw = a, bw = b;                          // the walk's box
+  if (BUILD) {
+    const float pw = co.pad_walk;
+    aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
+    bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
+    co.q0[t] = make_float4(p.x, p.y, p.z, 0.f);
+  }
+  int cnt = 0, ccnt = 0, flags = 0;
   const int root = roots ? roots[q_inst[q]] : 0;
   if (nl <= 0 || root < 0) {
     cand_n[q] = 0;
+    if (BUILD) co.count[t] = 0;
     return;
   }
   auto emit = [&](int f) {
@@ -1109,18 +1180,30 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const 
     else flags |= kStatusCandOverflow;
     ++cnt;
   };
+  auto keep = [&](int node) {
+    if (ccnt < co.cap) co.leaves[(size_t)ccnt * nq + t] = node;
+    ++ccnt;
+  };
   int sp = 0;
   {
     const float4 lo = nlo[root], hi = nhi[root];
-    if (overlap(a, b, xyz(lo), xyz(hi))) {
-      if (root >= nl - 1) emit(__float_as_int(lo.w));
-      else stk[(sp++) * kTravBS] = (uint32_t)root;
+    if (overlap(aw, bw, xyz(lo), xyz(hi))) {
+      if (root >= nl - 1) {
+        if (BUILD) keep(root);
+        if (!BUILD || overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
+      } else {
+        stk[(sp++) * kTravBS] = (uint32_t)root;
+      }
     }
   }
   while (sp > 0) {
     const uint32_t e = stk[(--sp) * kTravBS];
     if (e & kEmit) {
-      emit((int)(e & kLow30));
+      if (!BUILD || (e & kExactBit)) emit((int)(e & kLow30));
+      continue;
+    }
+    if (BUILD && (e & kCacheNode)) {
+      keep((int)(e & kLow30));
       continue;
     }
     const float4* qn = quad + 8 * (size_t)e;
@@ -1132,12 +1215,27 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const 
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (!overlap(a, b, xyz(lo[k]), xyz(hi[k]))) continue;
-      const int face = __float_as_int(hi[k].w);
-      const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)__float_as_int(lo[k].w);
-      if (sp < kStack) stk[(sp++) * kTravBS] = w;
+      if (!overlap(aw, bw, xyz(lo[k]), xyz(hi[k]))) continue;
+      const int face = __float_as_int(hi[k].w), node = __float_as_int(lo[k].w);
+      if (BUILD && face >= 0) {                   // node word under the face word: popped face first
+        if (sp + 1 < S) {
+          stk[(sp++) * kTravBS] = (uint32_t)node | kCacheNode;
+          stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | (overlap(a, b, xyz(lo[k]), xyz(hi[k])) ? kExactBit : 0u);
+        } else {
+          flags |= kStatusStackOverflow;
+        }
+        continue;
+      }
+      const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)node;
+      if (sp < S) stk[(sp++) * kTravBS] = w;
       else flags |= kStatusStackOverflow;
     }
+    if (co.redo && (flags & kStatusStackOverflow)) break;   // walked again by the second tier
+  }
+  if (BUILD) co.count[t] = (flags & kStatusStackOverflow) ? INT_MAX : ccnt;
+  if (co.redo && (flags & kStatusStackOverflow)) {    // the exact list may be incomplete: walk it again
+    co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
+    return;
   }
   cand_n[q] = cnt < cap ? cnt : cap;
   if (flags && status) atomicOr(status, flags);
@@ -1150,8 +1248,9 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
-                    const int32_t* q_inst, const int32_t* order) {
+                    const int32_t* q_inst, const int32_t* order, const int32_t* n_active) {
   SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
+  SLX_REQUIRE(!n_active || (order && mode == 1), "slx_traverse: a device query count needs a query list");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
   if (nq == 0) return SLX_OK;
@@ -1169,20 +1268,231 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   // roots.  The candidate lists are the same either way.
   else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq) {
     if (quad4 && SLX_TRAV_QUAD)
-      SLX_LAUNCH(name, k_traverse_quad, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
-                 padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n,
-                 status, roots, q_inst, order);
+      SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+                 (const float4*)q4, padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4,
+                 cap, cand, cand_n, status, roots, q_inst, order, n_active, CacheOut{});
     else
       SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
                  padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots, q_inst,
-                 order);
+                 order, n_active);
   }
   else
-    SLX_LAUNCH(name, k_traverse_tri_wave, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
-               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order);
+    SLX_LAUNCH(name, k_traverse_tri_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
+               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order, n_active,
+               CacheOut{});
   return check_launch("slx_traverse");
 }
 }  // namespace slx
+
+namespace slx {
+
+// ===================== candidate cache (Verlet lists over a fixed topology) =====================
+//
+// Within a frame the triangle tree keeps its topology; only its boxes move at the
+// refits.  The walks emit leaves in one fixed order (right subtree first, i.e.
+// decreasing leaf position), so a query's list is "the leaves whose box overlaps
+// the query box, in that order".  A walk with the query box enlarged by a margin
+// m stores the query's leaves (node ids, in walk order) and its position p0.
+// Later, with the tree's vertices displaced by at most D since the snapshot the
+// walk used (x = the vertices the leaf boxes are built from) and the query by d
+// = |p - p0|_inf, every leaf overlapping the exact box now overlapped the
+// enlarged box then, when d + D <= m (a leaf box is the padded box of its
+// vertices, so it moves by at most D; the query box by d).  Filtering the stored
+// leaves by the exact box then gives exactly the walk's list and order.  The test
+// keeps an allowance of 2^-20 of the coordinates' magnitude for rounding.  A
+// query whose condition fails, or whose stored list was incomplete (more than
+// the cache's capacity, or a stack overflow), is appended to a list and walked
+// in full.
+
+__global__ void __launch_bounds__(256)
+k_cand_filter(int nq, const float4* __restrict__ q4, float pad, float margin, const float4* __restrict__ nlo,
+              const float4* __restrict__ nhi, int ccap, const int32_t* __restrict__ cache,
+              const int32_t* __restrict__ cache_n, const float4* __restrict__ q0, const uint32_t* __restrict__ disp,
+              int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* __restrict__ redo,
+              int32_t* status, const int32_t* __restrict__ order) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const bool active = t < nq;
+  const int q = active ? (order ? order[t] : t) : 0;
+  bool ok = false;
+  int n = 0;
+  float3 p = make_float3(0.f, 0.f, 0.f);
+  if (active) {
+    const float3 p0 = xyz(q0[t]);
+    p = xyz(q4[q]);
+    const float dq = fmaxf(fmaxf(fabsf(p.x - p0.x), fabsf(p.y - p0.y)), fabsf(p.z - p0.z));
+    const float D = disp ? __uint_as_float(*disp) : 0.0f;
+    const float mag = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)) + 2.0f * (pad + margin);
+    n = cache_n[t];
+    ok = n <= ccap && (dq + D) + ldexpf(mag, -20) <= margin;   // false for NaN
+  }
+  // queries to walk again, appended with one atomic per wavefront
+  const bool need = active && !ok;
+  const unsigned long long m = __ballot(need);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&redo[0], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (need) redo[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = q;
+  }
+  if (!active || !ok) return;
+  const float pp = pad * 1.0f;
+  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  int cnt = 0;
+  for (int j = 0; j < n; ++j) {
+    const int L = cache[(size_t)j * nq + t];
+    const float4 lo = nlo[L], hi = nhi[L];
+    if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
+    if (cnt < cap) cand[(size_t)cnt * nq + q] = __float_as_int(lo.w);
+    ++cnt;
+  }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (cnt > cap && status) atomicOr(status, kStatusCandOverflow);
+}
+
+__global__ void k_zero_one(int32_t* __restrict__ a) {
+  if (threadIdx.x == 0) a[0] = 0;
+}
+
+// x_snap = x; disp = 0
+__global__ void k_tree_snapshot(int nv, const float4* __restrict__ x4, float4* __restrict__ x_snap,
+                                uint32_t* __restrict__ disp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) x_snap[i] = x4[i];
+  if (i == 0) disp[0] = 0u;
+}
+
+// disp = max_v |x - x_snap|_inf (disp zeroed before; non-negative floats order as
+// their bits)
+// (each workgroup reduces 8 vertices per thread; one atomic per workgroup, skipped
+// when the stored maximum is already larger)
+constexpr int kDispPer = 8;
+__global__ void __launch_bounds__(256)
+k_tree_disp(int nv, const float4* __restrict__ x4, const float4* __restrict__ x_snap, uint32_t* __restrict__ disp) {
+  __shared__ float s_max[4];
+  float d = 0.f;
+  const int i0 = blockIdx.x * (256 * kDispPer) + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kDispPer; ++k) {
+    const int i = i0 + k * 256;
+    if (i < nv) {
+      const float4 x = x4[i], s0 = x_snap[i];
+      float e = fmaxf(fmaxf(fabsf(x.x - s0.x), fabsf(x.y - s0.y)), fabsf(x.z - s0.z));
+      if (e != e) e = __int_as_float(0x7f800000);          // NaN: no cached list stays valid
+      d = fmaxf(d, e);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o, 64));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+    if (d > 0.f && __float_as_uint(d) > __atomic_load_n(disp, __ATOMIC_RELAXED)) atomicMax(disp, __float_as_uint(d));
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ a) {
+  if (threadIdx.x == 0) a[0] = 0u;
+}
+
+// first-tier quad walk with the shallow stack depth in use
+template <bool BUILD>
+int quad_first_tier(const char* name, int nq, const float* q4, float padding, int n_leaves, const float* nlo4,
+                    const float* nhi4, const float* quad4, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
+                    hipStream_t st, const int32_t* order, const CacheOut& co) {
+  const dim3 g(blocks_for(nq, kTravBS)), b(kTravBS);
+#define SLX_QUAD_TIER(D)                                                                                         \
+  SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, n_leaves,            \
+             (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
+             nullptr, order, nullptr, co)
+  if (g_walk_stack <= 16) SLX_QUAD_TIER(16);
+  else if (g_walk_stack <= 24) SLX_QUAD_TIER(24);
+  else if (g_walk_stack <= 32) SLX_QUAD_TIER(32);
+  else SLX_QUAD_TIER(kStack);
+#undef SLX_QUAD_TIER
+  return SLX_OK;
+}
+
+int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
+                    const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
+                    const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
+                    const int32_t* order) {
+  SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
+  SLX_REQUIRE(c && c->redo && (kind == 0 || (c->cap >= 1 && c->leaves && c->count && c->q0)),
+              "slx_traverse_cached: bad cache");
+  SLX_REQUIRE(nq >= 0 && cap >= 0 && c->margin >= 0.f, "slx_traverse_cached: bad sizes");
+  if (nq == 0) return SLX_OK;
+  SLX_REQUIRE(q4 && cand_n && (cap == 0 || cand), "slx_traverse_cached: null pointer");
+  SLX_REQUIRE(n_leaves < (1 << 29), "slx_traverse_cached: tree too large for 30-bit stack words");
+  if (n_leaves <= 0) return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4,
+                                            nullptr, nullptr, nullptr, cap, cand, cand_n, status, st, nullptr,
+                                            nullptr, order);
+  SLX_REQUIRE(nlo4 && nhi4, "slx_traverse_cached: null tree");
+  const bool wave = SLX_TRAV_WAVE && (long long)n_leaves * 8 <= (long long)nq;
+  SLX_REQUIRE(wave || quad4, "slx_traverse_cached: the triangle tree needs its 4-wide node array");
+  if (kind == 0 && wave)                      // the wavefront walk keeps its stack in registers
+    return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
+                           nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order);
+  SLX_LAUNCH("fill", k_zero_one, dim3(1), dim3(64), 0, st, c->redo);
+  int rc = SLX_OK;
+  if (kind == 1) {
+    const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
+    if (wave)
+      SLX_LAUNCH("cache_build_wave", k_traverse_tri_wave<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
+                 (const float4*)q4, padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n,
+                 status, order, nullptr, co);
+    else
+      rc = quad_first_tier<true>("cache_build", nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n,
+                                 status, st, order, co);
+  } else if (kind == 2) {
+    SLX_LAUNCH(name, k_cand_filter, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4, padding,
+               c->margin, (const float4*)nlo4, (const float4*)nhi4, c->cap, c->leaves, c->count,
+               (const float4*)c->q0, disp, cap, cand, cand_n, c->redo, status, order);
+  } else {
+    CacheOut co{};
+    co.redo = c->redo;
+    rc = quad_first_tier<false>(name, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n, status, st,
+                                order, co);
+  }
+  if (rc) return rc;
+  // the queries the first tier could not serve: a full walk over the list built on the device
+  return traverse_launch("walk_redo", 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
+                         nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, c->redo + 1, c->redo);
+}
+
+}  // namespace slx
+
+extern "C" int slx_traverse_cached(int kind, int nq, const float* q4, float padding, int n_leaves, const float* nlo4,
+                                   const float* nhi4, const float* quad4, const slx_cand_cache* cache,
+                                   const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
+                                   const int32_t* order, slx_stream_t stream) {
+  return traverse_cached("traverse_cached", kind, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cache, disp, cap,
+                         cand, cand_n, status, S(stream), order);
+}
+
+extern "C" int slx_set_walk_stack(int depth) {
+  SLX_REQUIRE(depth == 16 || depth == 24 || depth == 32 || depth == 64, "slx_set_walk_stack: 16, 24, 32 or 64");
+  g_walk_stack = depth;
+  return SLX_OK;
+}
+
+extern "C" int slx_tree_snapshot(int nv, const float* x4, float* x_snap, uint32_t* disp, slx_stream_t stream) {
+  SLX_REQUIRE(nv >= 0 && disp && (nv == 0 || (x4 && x_snap)), "slx_tree_snapshot: bad arguments");
+  SLX_LAUNCH("tree_snapshot", k_tree_snapshot, dim3(blocks_for(nv > 0 ? nv : 1, 256)), dim3(256), 0, S(stream), nv,
+             (const float4*)x4, (float4*)x_snap, disp);
+  return check_launch("slx_tree_snapshot");
+}
+
+extern "C" int slx_tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp,
+                                     slx_stream_t stream) {
+  SLX_REQUIRE(nv >= 0 && disp && (nv == 0 || (x4 && x_snap)), "slx_tree_displacement: bad arguments");
+  SLX_LAUNCH("fill", k_zero_u32, dim3(1), dim3(64), 0, S(stream), disp);
+  if (nv > 0)
+    SLX_LAUNCH("tree_disp", k_tree_disp, dim3(blocks_for(nv, 256 * kDispPer)), dim3(256), 0, S(stream), nv,
+               (const float4*)x4, (const float4*)x_snap, disp);
+  return check_launch("slx_tree_displacement");
+}
 
 extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
                             const int32_t* sorted_faces, const float* nlo4, const float* nhi4, const float* quad4,

@@ -116,7 +116,16 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
-                    const int32_t* q_inst = nullptr, const int32_t* order = nullptr);
+                    const int32_t* q_inst = nullptr, const int32_t* order = nullptr,
+                    const int32_t* n_active = nullptr);
+
+// two-tier traversal of a triangle tree with a candidate cache (lbvh.hip): kind 0
+// = walk, 1 = walk with the enlarged box filling the cache, 2 = filter the cache;
+// queries the first tier cannot serve are walked again with the deep stack
+int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
+                    const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
+                    const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
+                    const int32_t* order);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);

@@ -17,6 +17,15 @@ candidates per query of a query set.
   contact apply                56 V   (counts, first-set sums, f; read/write y)
   per-frame rebuild            Morton 20 F + 12 V, radix sort 68 F, Karras 16 F
 
+Candidate cache (triangle trees; cs[set] = mean leaves stored per query, the
+cache's own count):
+  cache build (a walk with the enlarged box, the exact list emitted on the way)
+                               traversal(nq, Ft, c) + (16 + 4 + 4 cs) nq  (query snapshot,
+                               count and leaves written)
+  cache filter                 (16 + 16 + 4 + 4) nq + 4 cs nq + 32 Ft + (4 c + 4) nq
+                               (query, snapshot, count, order; stored leaves; the leaf
+                                boxes once; list and count written)
+
 Velocity pass (enable_velocity_update), with P contacting pairs recorded by the
 last collision pass of a substep and R bucket deltas (two per moving point of
 each second- and third-set pair whose Cv < 0; the benchmark counts every such
@@ -66,6 +75,14 @@ def rebuild(V, F):
     return (20 * F + 12 * V) + 68 * F + 16 * F
 
 
+def cache_build(nq, Ft, c, cs, quad=True):
+    return traversal(nq, Ft, c, quad) + (16 + 4 + 4 * cs) * nq
+
+
+def cache_filter(nq, Ft, c, cs):
+    return 40 * nq + 4 * cs * nq + 32 * Ft + (4 * c + 4) * nq
+
+
 def pair_records(P, nq):
     return 64 * P + 4 * nq
 
@@ -96,11 +113,18 @@ def kernel_bytes(name: str, n: dict):
         "update_vx": update_vx(V),
         "tri_refit": refit(V, F),
     }
+    cs = n.get("cache", {})            # mean stored leaves per query of the cached sets
     if "Vs" in n:
         # the static (terrain) tree is coarse next to the queries: the wavefront walk over binary nodes
         table["traverse_vt_st"] = traversal(V, n["Fs"], c.get("vt_st", 0.0), quad=False)
         table["traverse_tv_st"] = traversal(n["Vs"], F, c.get("tv_st", 0.0))
+        if "vt_st" in cs:
+            table["traverse_vt_st"] = cache_filter(V, n["Fs"], c.get("vt_st", 0.0), cs["vt_st"])
+            table["cache_build_wave"] = cache_build(V, n["Fs"], c.get("vt_st", 0.0), cs["vt_st"], quad=False)
     table["traverse_vt_dy"] = traversal(V, F, c.get("vt_dy", 0.0))
+    if "vt_dy" in cs:
+        table["traverse_vt_dy"] = cache_filter(V, F, c.get("vt_dy", 0.0), cs["vt_dy"])
+        table["cache_build"] = cache_build(V, F, c.get("vt_dy", 0.0), cs["vt_dy"])
     nar = narrow(V, c.get("vt_dy", 0.0))
     if "Vs" in n:
         nar += narrow(V, c.get("vt_st", 0.0)) + narrow(n["Vs"], c.get("tv_st", 0.0))
@@ -122,12 +146,21 @@ def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
     total = substeps * sub
     if collision:
         c = n.get("cbar", {})
-        per_iter = traversal(V, F, c.get("vt_dy", 0.0)) + narrow(V, c.get("vt_dy", 0.0)) + contact_apply(V)
+        cs = n.get("cache", {})
+        per_iter = narrow(V, c.get("vt_dy", 0.0)) + contact_apply(V)
+        walks = n_iters * traversal(V, F, c.get("vt_dy", 0.0))
+        if "vt_dy" in cs:              # one build and (n_iters - 1) filters per substep
+            walks = (cache_build(V, F, c.get("vt_dy", 0.0), cs["vt_dy"]) +
+                     (n_iters - 1) * cache_filter(V, F, c.get("vt_dy", 0.0), cs["vt_dy"]))
         if "Vs" in n:
-            per_iter += traversal(V, n["Fs"], c.get("vt_st", 0.0), quad=False) + narrow(V, c.get("vt_st", 0.0))
-            per_iter += narrow(n["Vs"], c.get("tv_st", 0.0))
+            per_iter += narrow(V, c.get("vt_st", 0.0)) + narrow(n["Vs"], c.get("tv_st", 0.0))
+            if "vt_st" in cs:
+                walks += (cache_build(V, n["Fs"], c.get("vt_st", 0.0), cs["vt_st"], quad=False) +
+                          (n_iters - 1) * cache_filter(V, n["Fs"], c.get("vt_st", 0.0), cs["vt_st"]))
+            else:
+                walks += n_iters * traversal(V, n["Fs"], c.get("vt_st", 0.0), quad=False)
             total += substeps * traversal(n["Vs"], F, c.get("tv_st", 0.0))
-        total += substeps * (refit(V, F) + n_iters * per_iter) + rebuild(V, F)
+        total += substeps * (refit(V, F) + n_iters * per_iter + walks) + rebuild(V, F)
     if "pairs" in n:
         # velocity pass: compute_velocity + contacts + apply (update_x fused) replace update_vx
         Vs = n.get("Vs", 0)

@@ -13,6 +13,14 @@ Extensions (all default to the reference's behaviour):
   pair_cache_size (None)           contacting pairs kept per query for the
                                    velocity pass (None = cache_size, which never
                                    drops one; a smaller value saves memory)
+  candidate_cache (True)           triangle trees: each query set's candidate
+                                   lists come from a per-frame cache built with
+                                   the query box enlarged by cache_margin (None =
+                                   padding / 4), filtered by the exact box each
+                                   pass; the lists are the traversal's, bit for bit
+  cache_capacity (None)            leaves stored per query (None = 2 cache_size)
+  cache_every (1)                  substeps between cache rebuilds (<= 0: once
+                                   per frame)
   status()                         device status word (overflow flags)
 `step` is an alias of `forward`.
 """
@@ -60,6 +68,10 @@ class Solver:
         self.refit_every_substep = False
         self.cache_size = 40
         self.pair_cache_size = None
+        self.candidate_cache = True
+        self.cache_margin = None
+        self.cache_capacity = None
+        self.cache_every = 1
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
@@ -118,6 +130,7 @@ class Solver:
         key = (self.bvh_mode, self.cache_size, self.records_per_vertex)
         if self._cd is not None and self._cd["key"] == key:
             self._pair_buffers(self._cd, pcap)
+            self._cand_caches(self._cd)
             return self._cd
         m, s = self.mesh_dy, self.mesh_st
         nv, nvs = m.num_verts, (s.num_verts if s is not None else 0)
@@ -136,7 +149,30 @@ class Solver:
             s.xs4 = s.x4
         self._cd = cd
         self._pair_buffers(cd, pcap)
+        self._cand_caches(cd)
         return cd
+
+    def _cand_caches(self, cd):
+        """Candidate caches of the three query sets (triangle trees, single meshes)."""
+        from .broadphase import CandidateCache
+        tri = self.bvh_mode == "tri" and cd["dyn"].n_inst <= 1
+        on = self.candidate_cache and tri
+        ccap = (2 * self.cache_size if self.cache_capacity is None else int(self.cache_capacity)) if on else 0
+        margin = _f32(0.25 * self.padding if self.cache_margin is None else self.cache_margin)
+        key = (tri, ccap, margin)
+        if cd.get("cache_key") == key:
+            return
+        cd["cache_key"] = key
+        if not tri:                    # (the two-tier walk and the caches serve triangle trees)
+            cd["caches"] = None
+            return
+        m, s = self.mesh_dy, self.mesh_st
+        nv, nvs = m.num_verts, (s.num_verts if s is not None else 0)
+        if on:
+            cd["dyn"].enable_snapshot()
+        cd["caches"] = [CandidateCache(nv, ccap, margin, self.device) if s is not None else None,
+                        CandidateCache(nv, ccap, margin, self.device),
+                        CandidateCache(nvs, ccap, margin, self.device) if s is not None else None]
 
     def _pair_buffers(self, cd, pcap):
         """Pair records for the velocity pass (allocated once it is enabled)."""
@@ -186,6 +222,11 @@ class Solver:
             d.dyn = cd["dyn"].struct()
             d.cap, d.rec_cap = self.cache_size, cd["rec_cap"]
             d.cand, d.cand_n, d.pair_num = P(cd["cand"]), P(cd["cand_n"]), P(cd["pair_num"])
+            if cd.get("caches"):
+                for k, c in enumerate(cd["caches"]):
+                    if c is not None:
+                        d.cache[k] = c.struct()
+                d.cache_every = int(self.cache_every)
             if self.enable_velocity_update:
                 d.pair_cap, d.pair_cnt, d.pair_rec = cd["pair_cap"], P(cd["pair_cnt"]), P(cd["pair_rec"])
                 if s is not None:
