@@ -9,7 +9,7 @@ Import with importlib (the directory name is not a Python identifier):
                      stiffness_bending=1e3, g=(0, -9.8, 0), dt=1/60)
     sim.forward(n_substeps=20)
 """
-from . import scene, topology  # noqa: F401
+from . import scene, selection, topology  # noqa: F401
 from ._abi import SlxError  # noqa: F401
 from .mesh import Mesh  # noqa: F401
 from .solver import Solver  # noqa: F401

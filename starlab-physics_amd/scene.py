@@ -128,9 +128,11 @@ def concat_offsets(meshes):
     return out
 
 
-def read_obj(path: str) -> TriMesh:
+def read_obj(path: str, dtype=np.float32) -> TriMesh:
     """Vertices and faces of an OBJ file (polygons are fan-triangulated,
-    1-based and negative indices accepted, texture/normal indices ignored)."""
+    1-based and negative indices accepted, texture/normal indices ignored), the
+    faces libigl's read_obj returns (framework/meshio/concat.py:20).  Positions
+    are parsed as doubles and returned as `dtype`."""
     vs, fs = [], []
     with open(path, "r") as fh:
         for line in fh:
@@ -143,18 +145,43 @@ def read_obj(path: str) -> TriMesh:
                     idx.append(k - 1 if k > 0 else len(vs) + k)
                 for t in range(1, len(idx) - 1):
                     fs.append([idx[0], idx[t], idx[t + 1]])
-    return TriMesh(np.array(vs, np.float32).reshape(-1, 3), np.array(fs, np.int32).reshape(-1, 3))
+    return TriMesh(np.array(vs, np.float64).astype(dtype).reshape(-1, 3), np.array(fs, np.int32).reshape(-1, 3))
 
 
 def write_obj(path: str, x: np.ndarray, faces: np.ndarray) -> None:
+    """Vertices and triangles as OBJ text (1-based faces).  Positions keep every
+    bit: float32 arrays are written with 9 significant digits, float64 with 17,
+    so reading the file back gives the same values."""
     d = os.path.dirname(path)
     if d:
         os.makedirs(d, exist_ok=True)
+    x = np.asarray(x)
+    fmt = "v %.17g %.17g %.17g\n" if x.dtype == np.float64 else "v %.9g %.9g %.9g\n"
     with open(path, "w") as fh:
-        for p in np.asarray(x, np.float64):
-            fh.write("v %.9g %.9g %.9g\n" % (p[0], p[1], p[2]))
+        for p in x.astype(np.float64):
+            fh.write(fmt % (p[0], p[1], p[2]))
         for f in np.asarray(faces, np.int64):
             fh.write("f %d %d %d\n" % (f[0] + 1, f[1] + 1, f[2] + 1))
+
+
+def concat_mesh(concat_model_name, model_dir, model_names, translations, scales, out_dir=None):
+    """framework/meshio/concat.py:11-58 concat_mesh: read each model (doubles),
+    scale it, translate it, offset its faces by the running vertex count, write
+    the concatenation as OBJ `concat_model_name` and return the vertex offsets
+    [0, n0, n0 + n1, ..., total].  The reference writes into its models/OBJ
+    folder; here into out_dir (default model_dir)."""
+    offsets, xs, fs, acc = [], [], [], 0
+    for name, t, sc in zip(model_names, translations, scales):
+        m = read_obj(os.path.join(model_dir, name), dtype=np.float64)
+        offsets.append(acc)
+        xs.append(sc * m.x + np.asarray(t, np.float64))
+        fs.append(m.faces.astype(np.int64) + acc)
+        acc += len(m.x)
+    offsets.append(acc)
+    x = np.concatenate(xs, axis=0) if xs else np.zeros((0, 3))
+    f = np.concatenate(fs, axis=0) if fs else np.zeros((0, 3), np.int64)
+    write_obj(os.path.join(model_dir if out_dir is None else out_dir, concat_model_name), x, f)
+    return offsets
 
 
 def rotation_matrix(rot_deg) -> np.ndarray:
