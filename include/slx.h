@@ -290,6 +290,19 @@ int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* 
                        int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                        slx_stream_t stream);
 
+/* ---------------- sewing constraints (XPBD.py:405-441) ---------------- */
+
+/* solve_sewing_constraints_x as a Jacobi pass of its own (zero dx/nc, the
+ * pairs, update_dx; the reference's commented-out call accumulates after its
+ * last update_dx, where nothing would apply it): pair (v1, v2) pulls its points
+ * together with ld = k C / (k (f1 w1 + f2 w2) + 1), C = |y2 - y1|.  pairs int2
+ * [np]; the n_sewn distinct vertices in sew_verts, each with its pair entries
+ * (2 pair + side, in pair order) in sew_list[sew_off[i] .. sew_off[i+1]).  y4.w
+ * must hold f*m_inv.  ws: float4 [n_sewn].  A coincident pair adds no delta. */
+int slx_sewing_pass(int n_sewn, const int32_t* sew_verts, const int32_t* sew_off, const int32_t* sew_list,
+                    const int32_t* pairs, float* y4, const float* fixed, float k, float* ws,
+                    slx_stream_t stream);
+
 /* ---------------- velocity pass (collision_constraints_v.py) ---------------- */
 
 /* compute_velocity, XPBD.py:340-343: v = (1-damping)*f*(y-x)/dt; v4.w = f. */
@@ -480,6 +493,15 @@ typedef struct slx_frame_desc {
      needs x_snap and disp. */
   slx_cand_cache cache[3];
   int32_t cache_every;
+  /* sewing constraints (XPBD.py:418-441), after the collision pass of each
+     iteration (n_sewn = 0: none); k_sew = sewing stiffness * dt^2 */
+  int32_t n_sewn;
+  const int32_t* sew_verts;
+  const int32_t* sew_off;
+  const int32_t* sew_list;
+  const int32_t* sew_pairs;
+  float k_sew;
+  float* sew_ws;
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

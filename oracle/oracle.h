@@ -70,6 +70,10 @@ extern "C" {
                                   int* pair_num_vt_st, int* pair_num_vt_dy,                 \
                                   int* pair_num_tv_st, int* pi0, R* pg0, R* ps0, int* pi1,  \
                                   R* pg1, R* ps1, int* pi2, R* pg2, R* ps2);                \
+  /* XPBD.py:418-441 sewing constraints as a Jacobi pass (zero dx/nc, pairs in               \
+     order, update_dx); pairs int[2 np] */                                                   \
+  void orc_sewing_pass_##S(int nv, int np, const int* pairs, R* y, const R* fixed,          \
+                           const R* m_inv, R k, R* dx, R* nc);                               \
   /* XPBD.py:340-343 compute_velocity */                                                     \
   void orc_compute_velocity_##S(int nv, const R* x, R* v, const R* y, const R* fixed,       \
                                 R damping, R dt);                                           \

@@ -106,6 +106,16 @@ def update_vx(x, v, y, fixed, damping, dt):
     getattr(lib(), "orc_update_vx_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
 
 
+def sewing_pass(y, pairs, fixed, m_inv, k):
+    """Sewing constraints as one Jacobi pass (XPBD.py:418-441); y in place."""
+    ct, s = _real(y.dtype)
+    pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    dx = np.zeros_like(y)
+    nc = np.zeros(len(fixed), y.dtype)
+    getattr(lib(), "orc_sewing_pass_" + s)(len(fixed), len(pr), P(pr), P(y), P(fixed), P(m_inv), ct(k), P(dx), P(nc))
+    return y
+
+
 def compute_velocity(x, v, y, fixed, damping, dt):
     ct, s = _real(x.dtype)
     getattr(lib(), "orc_compute_velocity_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
@@ -378,9 +388,12 @@ class OracleCloth:
 
     def __init__(self, mesh_dy, mesh_st=None, *, dhat, stiffness_stretch, stiffness_bending,
                  g, dt, dtype=np.float32, damping=0.001, padding=0.05, collision=False,
-                 bvh="cell", cap=40, refit_every_substep=False, method="jacobi", velocity=False, mu=0.8):
+                 bvh="cell", cap=40, refit_every_substep=False, method="jacobi", velocity=False, mu=0.8,
+                 sewing=None, sewing_stiffness=None):
         self.method = method
         self.velocity, self.mu = velocity, mu
+        self.sewing = None if sewing is None else np.ascontiguousarray(sewing, np.int32).reshape(-1, 2)
+        self.sewing_stiffness = sewing_stiffness
         self.dt_ = np.dtype(dtype)
         self.m = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh_dy.items()}
         self.m["edges"] = np.ascontiguousarray(self.m["edges"], np.int32)
@@ -458,6 +471,9 @@ class OracleCloth:
                                          cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"),
                                          record=self.velocity)
                     self.last_pairs = res.get("pairs")
+                if self.sewing is not None and len(self.sewing):
+                    ksew = 0.5 * self.kb if self.sewing_stiffness is None else self.sewing_stiffness
+                    sewing_pass(self.y, self.sewing, self.fixed, self.m_inv, R(ksew * dt_sub * dt_sub))
             if not self.velocity:
                 update_vx(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
                 continue

@@ -451,6 +451,38 @@ void FN(collision_pass)(int nv, int nvs, const int* faces_dy, const int* faces_s
                          pn_tv_st, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
 }
 
+/* XPBD.py:418-441 solve_sewing_constraints_x, applied as a Jacobi pass of its
+   own (zero dx/nc, the pairs in order, update_dx): each pair (v1, v2) pulls its
+   points together, C = |y2 - y1|, n = (y1 - y2)/C, S = f1 w1 + f2 w2,
+   ld = k C / (k S + 1), dx1 -= f1 w1 ld n, dx2 += f2 w2 ld n, nc += 1 on both.
+   A coincident pair (C = 0, where the reference normalises a zero vector into
+   NaN) adds no delta and still counts. */
+void FN(sewing_pass)(int nv, int np, const int* pairs, R* y, const R* fixed, const R* m_inv, R k, R* dx,
+                     R* nc) {
+  for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  for (int i = 0; i < nv; ++i) nc[i] = 0;
+  for (int p = 0; p < np; ++p) {
+    int v1 = pairs[2 * p], v2 = pairs[2 * p + 1];
+    R d[3];
+    SUB(y + 3 * v2, y + 3 * v1, d);
+    R C = SQRT(DOT(d, d));
+    R fw1 = fixed[v1] * m_inv[v1], fw2 = fixed[v2] * m_inv[v2];
+    if (C > (R)0) {
+      R inv = (R)1 / C;
+      R n[3] = {-(inv * d[0]), -(inv * d[1]), -(inv * d[2])};     /* (y1 - y2) / C */
+      R schur = fw1 + fw2;
+      R ld = (k * C) / ((k * schur) + (R)1);
+      for (int c = 0; c < 3; ++c) {
+        dx[3 * v1 + c] = dx[3 * v1 + c] - (fw1 * ld) * n[c];
+        dx[3 * v2 + c] = dx[3 * v2 + c] + (fw2 * ld) * n[c];
+      }
+    }
+    nc[v1] = nc[v1] + (R)1;
+    nc[v2] = nc[v2] + (R)1;
+  }
+  CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+}
+
 /* ---------------- velocity pass (collision_constraints_v.py, XPBD.py:283-311, 374-404) ---- */
 
 /* XPBD.py:340-343: v = (1 - damping) * f * (y - x) / dt. */

@@ -60,6 +60,7 @@ SIGNATURES = {
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
                            PTR, PTR, I32, PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
+    "slx_sewing_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, PTR, STREAM],
     "slx_compute_velocity": [I32, PTR, PTR, PTR, PTR, F32, F32, STREAM],
     "slx_update_x": [I32, PTR, PTR, F32, STREAM],
     "slx_velocity_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, I32, PTR, PTR, PTR, F32, I32, PTR, PTR,
@@ -118,7 +119,9 @@ class FrameDesc(C.Structure):
                 ("gs_color_off", PTR), ("n_colors", I32), ("dyn", Tree), ("st", Tree),
                 ("cand", PTR), ("cand_n", PTR), ("pair_num", PTR), ("status", PTR), ("ws", PTR),
                 ("ws_bytes", C.c_size_t), ("velocity", I32), ("mu", F32), ("vs4", PTR), ("m_inv_st", PTR),
-                ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR), ("cache", CandCache * 3), ("cache_every", I32)]
+                ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR), ("cache", CandCache * 3), ("cache_every", I32),
+                ("n_sewn", I32), ("sew_verts", PTR), ("sew_off", PTR), ("sew_list", PTR), ("sew_pairs", PTR),
+                ("k_sew", F32), ("sew_ws", PTR)]
 
 
 class SlxError(RuntimeError):

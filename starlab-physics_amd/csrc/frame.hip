@@ -144,7 +144,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(d->method == 0 || d->n_colors == 0 || (d->gs_edges && d->gs_color_off),
               "slx_frame: Gauss-Seidel colouring missing");
   const bool vel = d->velocity != 0;
-  if (!d->collision && d->method == 0 && !vel) {
+  const bool sew = d->n_sewn > 0;
+  SLX_REQUIRE(!sew || (d->sew_verts && d->sew_off && d->sew_list && d->sew_pairs && d->sew_ws),
+              "slx_frame: sewing needs its pair lists and workspace");
+  if (!d->collision && d->method == 0 && !vel && !sew) {
     // x4_alt / v4_alt: the caller's y4_alt doubles as one ping-pong buffer is not
     // possible (four buffers are needed); use the workspace for the alternates.
     size_t need = 2 * sizeof(float4) * (size_t)d->nv;
@@ -249,7 +252,14 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                      (const int4*)d->gs_edges, d->gs_color_off, d->n_colors, d->k_s, d->k_b, st);
         if (rc) return rc;
       }
-      if (!coll) continue;
+      if (!coll) {
+        if (sew) {
+          rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,
+                               d->k_sew, d->sew_ws, stream);
+          if (rc) return rc;
+        }
+        continue;
+      }
       // broad phase (XPBD.py:186-215); the static-tree queries on their own stream
       bool static_pending = false;
       if (nvs > 0) {
@@ -281,6 +291,11 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                               rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
                               d->rec_cap, d->status, d->ws, d->ws_bytes, stream);
       if (rc) return rc;
+      if (sew) {                             // sewing constraints (XPBD.py:371-372, there commented out)
+        rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,
+                             d->k_sew, d->sew_ws, stream);
+        if (rc) return rc;
+      }
     }
     rc = join_refit();                       // (n_iters == 0: nothing queried the tree)
     if (rc) return rc;

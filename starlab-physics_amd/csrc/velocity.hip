@@ -369,3 +369,68 @@ extern "C" int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const
              w.self_acc, fixed, m_inv, (float4*)v4, (float4*)x4, dt);
   return check_launch("slx_velocity_pass");
 }
+
+// ===================== sewing constraints (XPBD.py:418-441) =====================
+//
+// A Jacobi pass of their own (dx/nc zeroed, the pairs, update_dx): each sewn
+// vertex sums the deltas of its pairs in pair order (the serial scatter's order,
+// from a host-built list), then y += f dx / nc.  A coincident pair adds no delta
+// (the reference would normalise a zero vector) and still counts.
+namespace slx {
+
+__global__ void __launch_bounds__(256)
+k_sew_gather(int n_sewn, const int32_t* __restrict__ sew_verts, const int32_t* __restrict__ sew_off,
+             const int32_t* __restrict__ sew_list, const int2* __restrict__ pairs, const float4* __restrict__ y4,
+             float k, float4* __restrict__ ws) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_sewn) return;
+  float3 dx = zero3();
+  float nc = 0.f;
+  for (int e = sew_off[i]; e < sew_off[i + 1]; ++e) {
+    const int w = sew_list[e], p = w >> 1, side = w & 1;
+    const int2 pr = pairs[p];
+    const float4 y1 = y4[pr.x], y2 = y4[pr.y];
+    const float3 d = sub3(xyz(y2), xyz(y1));
+    const float C = sqrtf(dot3(d, d));
+    if (C > 0.f) {
+      const float inv = 1.0f / C;
+      const float3 n = make_float3(-(inv * d.x), -(inv * d.y), -(inv * d.z));
+      const float fw1 = y1.w, fw2 = y2.w;                    // f * m_inv of each point
+      const float schur = fw1 + fw2;
+      const float ld = (k * C) / ((k * schur) + 1.0f);
+      if (side == 0) dx = sub3(dx, mul3(fw1 * ld, n));
+      else dx = add3(dx, mul3(fw2 * ld, n));
+    }
+    nc = nc + 1.0f;
+  }
+  ws[i] = make_float4(dx.x, dx.y, dx.z, nc);
+}
+
+__global__ void __launch_bounds__(256)
+k_sew_apply(int n_sewn, const int32_t* __restrict__ sew_verts, const float4* __restrict__ ws,
+            const float* __restrict__ fixed, float4* __restrict__ y4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_sewn) return;
+  const int v = sew_verts[i];
+  const float4 a = ws[i];
+  if (!(a.w > 0.f)) return;
+  const float f = fixed[v];
+  const float4 y = y4[v];
+  y4[v] = make_float4(y.x + f * (a.x / a.w), y.y + f * (a.y / a.w), y.z + f * (a.z / a.w), y.w);
+}
+
+}  // namespace slx
+
+extern "C" int slx_sewing_pass(int n_sewn, const int32_t* sew_verts, const int32_t* sew_off, const int32_t* sew_list,
+                               const int32_t* pairs, float* y4, const float* fixed, float k, float* ws,
+                               slx_stream_t stream) {
+  SLX_REQUIRE(n_sewn >= 0, "slx_sewing_pass: bad size");
+  if (n_sewn == 0) return SLX_OK;
+  SLX_REQUIRE(sew_verts && sew_off && sew_list && pairs && y4 && fixed && ws, "slx_sewing_pass: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  SLX_LAUNCH("sew_gather", k_sew_gather, dim3(blocks_for(n_sewn, 256)), dim3(256), 0, st, n_sewn, sew_verts, sew_off,
+             sew_list, (const int2*)pairs, (const float4*)y4, k, (float4*)ws);
+  SLX_LAUNCH("sew_apply", k_sew_apply, dim3(blocks_for(n_sewn, 256)), dim3(256), 0, st, n_sewn, sew_verts,
+             (const float4*)ws, fixed, (float4*)y4);
+  return check_launch("slx_sewing_pass");
+}

@@ -21,6 +21,12 @@ Extensions (all default to the reference's behaviour):
   cache_capacity (None)            leaves stored per query (None = 2 cache_size)
   cache_every (1)                  substeps between cache rebuilds (<= 0: once
                                    per frame)
+  load_sewing_pairs(path | pairs)  sewing pairs (XPBD.py:405-417); with
+  enable_sewing (False)            they are applied after the collision pass of
+                                   every iteration (XPBD.py:418-441, whose call
+                                   the reference leaves commented out), stiffness
+                                   sewing_stiffness (None = 0.5 stiffness_bending,
+                                   the reference's "0.5 YM")
   status()                         device status word (overflow flags)
 `step` is an alias of `forward`.
 """
@@ -72,6 +78,9 @@ class Solver:
         self.cache_margin = None
         self.cache_capacity = None
         self.cache_every = 1
+        self.enable_sewing = False
+        self.sewing_stiffness = None
+        self._sew = None
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
@@ -115,6 +124,25 @@ class Solver:
         _abi.call("slx_frame", _abi.C.byref(d), _abi.stream_handle())
 
     step = forward
+
+    def load_sewing_pairs(self, path=None, pairs=None):
+        """XPBD.py:405-417: pairs from animation/sewing.csv (default path) or an
+        (n, 2) array."""
+        import torch
+        from . import selection, topology
+        if pairs is None:
+            pairs = selection.load_sewing_pairs(path or "animation/sewing.csv")
+        pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        nv = self.mesh_dy.num_verts
+        if len(pr) > nv:
+            raise ValueError("more sewing pairs than dynamic vertices (the reference's field holds V)")
+        verts, offs, ent = topology.sewing_lists(pr, nv)
+        dev = self.device
+        self._sew = dict(pairs=torch.from_numpy(pr.copy()).to(dev), verts=torch.from_numpy(verts).to(dev),
+                         off=torch.from_numpy(offs).to(dev), list=torch.from_numpy(ent).to(dev),
+                         ws=torch.zeros((max(len(verts), 1), 4), dtype=torch.float32, device=dev), n=len(verts),
+                         np=len(pr))
+        return pr
 
     def status(self) -> int:
         """Device status word (SLX_STATUS_* bits); synchronises."""
@@ -231,6 +259,12 @@ class Solver:
                 d.pair_cap, d.pair_cnt, d.pair_rec = cd["pair_cap"], P(cd["pair_cnt"]), P(cd["pair_rec"])
                 if s is not None:
                     d.vs4, d.m_inv_st = P(s.v4), P(s.m_inv)
+        if self.enable_sewing and self._sew is not None and self._sew["np"] > 0:
+            sw = self._sew
+            dt_sub = self.dt / n_substeps
+            ks = 0.5 * self.stiffness_bending if self.sewing_stiffness is None else self.sewing_stiffness
+            d.n_sewn, d.sew_verts, d.sew_off, d.sew_list = sw["n"], P(sw["verts"]), P(sw["off"]), P(sw["list"])
+            d.sew_pairs, d.sew_ws, d.k_sew = P(sw["pairs"]), P(sw["ws"]), _f32(ks * dt_sub * dt_sub)
         # velocity pass (XPBD.py:467-468)
         d.velocity = 1 if self.enable_velocity_update else 0
         d.mu = _f32(self.mu)

@@ -205,3 +205,18 @@ def morton_order(x):
     q = np.clip(((x - lo) / ext * 1023.0).astype(np.int64), 0, 1023)
     code = (_spread10(q[:, 0]) << np.uint64(2)) | (_spread10(q[:, 1]) << np.uint64(1)) | _spread10(q[:, 2])
     return np.argsort(code, kind="stable").astype(np.int32)
+
+
+def sewing_lists(pairs: np.ndarray, nv: int):
+    """Per sewn vertex, its sewing pair entries in pair order (the order in which
+    the reference's pair loop scatters into that vertex, XPBD.py:420-441):
+    returns (vertices [n], offsets [n+1], entries [2 np] of 2 * pair + side)."""
+    pr = np.ascontiguousarray(pairs, np.int64).reshape(-1, 2)
+    if pr.size and (pr.min() < 0 or pr.max() >= nv):
+        raise IndexError(f"sewing pair names a vertex outside [0, {nv})")
+    ent_v = pr.reshape(-1)                       # entry 2p + side -> vertex
+    ent = np.arange(ent_v.size, dtype=np.int64)
+    order = np.lexsort((ent, ent_v))             # by vertex, then by pair order
+    verts, starts = np.unique(ent_v[order], return_index=True)
+    offs = np.concatenate([starts, [ent_v.size]]).astype(np.int32)
+    return verts.astype(np.int32), offs, ent[order].astype(np.int32)

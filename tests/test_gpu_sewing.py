@@ -1,0 +1,59 @@
+"""GPU sewing pass bit-exact against the oracle, alone and inside frames."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_sewing_pass_bit_exact(slx, orc, rng):
+    import torch
+    A = importlib.import_module("starlab-physics_amd._abi")
+    g = slx.scene.grid_cloth(20, jitter=1e-3)
+    cloth = slx.Mesh(x=g.x, faces=g.faces)
+    nv = cloth.num_verts
+    fixed = np.ones(nv, np.float32)
+    fixed[:5] = 0
+    cloth.fixed.copy_(torch.from_numpy(fixed).cuda())
+    cloth._refresh_products()
+    y = cloth.positions() + rng.normal(0, 0.05, (nv, 3)).astype(np.float32)
+    cloth.y4[:, :3] = torch.from_numpy(y).cuda()
+    pairs = rng.integers(0, nv, (150, 2)).astype(np.int32)
+    pairs[7] = [3, 3]                                         # a coincident pair
+    pairs[8] = [0, 1]                                         # both pinned
+    verts, offs, ent = slx.topology.sewing_lists(pairs, nv)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    ws = torch.zeros((len(verts), 4), dtype=torch.float32, device="cuda")
+    k = float(np.float32(0.5 * 1e3 * (1 / 1200) ** 2))
+    P = A.ptr
+    tp, tv, to, te = T(pairs), T(verts), T(offs), T(ent)
+    A.call("slx_sewing_pass", len(verts), P(tv), P(to), P(te), P(tp), P(cloth.y4), P(cloth.fixed), k, P(ws),
+           A.stream_handle())
+    yo = y.copy()
+    orc.sewing_pass(yo, pairs, fixed, cloth.m_inv_np, np.float32(k))
+    np.testing.assert_array_equal(cloth.y4[:, :3].cpu().numpy(), yo)
+    assert np.any(yo != y, axis=1).sum() > 100
+
+
+@pytest.mark.parametrize("collision", [False, True])
+def test_frames_with_sewing_bit_exact(slx, rng, collision):
+    tc = importlib.import_module("test_gpu_collision")
+    cloth, sim, _ = tc.collision_scene(slx, n=20, mode="tri", refit=True)
+    sim.enable_collision_handling = collision
+    nv = cloth.num_verts
+    # sew the cloth's left edge to its right edge
+    left = np.arange(0, nv, 20)
+    pairs = np.stack([left[1:-1], left[1:-1] + 19], 1).astype(np.int32)
+    sim.load_sewing_pairs(pairs=pairs)
+    sim.enable_sewing = True
+    ref = tc.orc_cloth(cloth, sim.mesh_st, sim)
+    ref.collision = collision
+    ref.sewing = pairs
+    gap0 = np.linalg.norm(cloth.positions()[pairs[:, 0]] - cloth.positions()[pairs[:, 1]], axis=1).mean()
+    for _ in range(6):
+        sim.forward(10, 1)
+        ref.forward(10, 1)
+    np.testing.assert_array_equal(cloth.positions(), ref.x)
+    gap = np.linalg.norm(ref.x[pairs[:, 0]] - ref.x[pairs[:, 1]], axis=1).mean()
+    assert gap < 0.5 * gap0, "sewing must pull the edges together"
