@@ -206,6 +206,17 @@ int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves,
                  const float* box6, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
                  slx_stream_t stream);
 
+/* The edge broad phase of XPBD.py:217-232 (commented out in the reference): the
+ * same traversal for ne edge boxes, min/max of x4[edges[2e]], x4[edges[2e+1]]
+ * -+ padding.  Arguments otherwise as slx_traverse (q = edge).  A dynamic tree is
+ * queried with its own grid (the reference's commented query of lbvh_dy passes
+ * the static tree's cell size and origin). */
+int slx_traverse_edges(int mode, int ne, const int32_t* edges, const float* x4, float padding,
+                       int n_leaves, const int32_t* num, const int32_t* sorted_faces,
+                       const float* nlo4, const float* nhi4, const float* quad4, const float* fmin4,
+                       const float* fmax4, const float* box6, int cap, int32_t* cand,
+                       int32_t* cand_n, int32_t* status, slx_stream_t stream);
+
 /* Batched instances (no reference counterpart beyond the mesh concatenation of
  * framework/meshio/concat.py:11-58): instances packed CSR-style (vert_base,
  * face_base [n_inst+1], face_inst [nf]); per-instance boxes box6[6*n_inst];
@@ -290,6 +301,50 @@ int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* 
                        int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                        slx_stream_t stream);
 
+/* Edge-edge sets of one collision pass (the reference's ee_st / ee_dy fields,
+ * XPBD.py:75-89, and loops, 266-281 and 313-330, all commented out there).
+ * Query e (one per dynamic edge) tests the three sides (face_edge_indices) of
+ * each candidate face of the static tree (set 0, __ee_st) and then of the
+ * dynamic tree (set 1, __ee_dy, skipping sides that share a vertex with e).
+ * Candidates column-major [cap][ne] with counts, as slx_traverse_edges writes
+ * them; a null candidate array leaves its set out.  Pair records (the
+ * reference's ee_*_pair fields): pair_num[s*ne + e] (nullable) counts the
+ * contacting pairs (d < dhat) of set s, uncapped; with pair_rec and pair_cnt
+ * non-null the first pair_cap of them are stored (more -> dropped, status bit
+ * SLX_STATUS_PAIR_OVERFLOW), float4 pair_rec[(j*4 + w) * 2ne + s*ne + e] for
+ * j < pair_cnt[s*ne + e]: (g0, schur), (g1, other edge bits), (g2, type bits),
+ * (g3, 0).  18*ne*cap and 6*ne*pair_cap must stay below 2^32. */
+typedef struct slx_edge_sets {
+  int32_t ne;
+  const int32_t* edges_dy;  /* [ne][2] */
+  const int32_t* edges_st;  /* [nes][2] */
+  const int32_t* fe_dy;     /* [F][3] edge ids of each face's sides */
+  const int32_t* fe_st;     /* [Fs][3] */
+  int32_t cap;
+  const int32_t* c_st;
+  const int32_t* n_st;
+  const int32_t* c_dy;
+  const int32_t* n_dy;
+  int32_t pair_cap;
+  int32_t* pair_num;        /* [2 ne] */
+  int32_t* pair_cnt;        /* [2 ne] */
+  float* pair_rec;          /* float4 [pair_cap][4][2 ne] */
+} slx_edge_sets;
+
+/* slx_collision_pass with the edge-edge constraints after the vertex-triangle
+ * ones (collision_constraints_x.py:312-507: type 8 only, edges further than 1e-3
+ * from parallel, d = d_EE < dhat, ld = k (dhat - d) / (k schur + 1) on the
+ * dynamic points), all accumulated per vertex in the reference's serial order
+ * before update_dx.  ee null: slx_collision_pass. */
+int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st,
+                          float* y4, const float* xs4, const float* fixed, const float* m_inv,
+                          float dhat, float k_col, int cap, const int32_t* c_vt_st,
+                          const int32_t* n_vt_st, const int32_t* c_vt_dy, const int32_t* n_vt_dy,
+                          const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
+                          int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee,
+                          int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
+                          slx_stream_t stream);
+
 /* ---------------- sewing constraints (XPBD.py:405-441) ---------------- */
 
 /* solve_sewing_constraints_x as a Jacobi pass of its own (zero dx/nc, the
@@ -329,6 +384,18 @@ int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* f
                       float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec, float* x4,
                       float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
                       slx_stream_t stream);
+/* slx_velocity_pass with the edge-edge pairs recorded by slx_collision_pass_ee
+ * after the vertex-triangle ones (collision_constraints_v.py:573-878, normal
+ * only: __ee_st with Cv = g0.v0 + g1.v1 + g3.vs3 -- no g2 term, :782 -- moves the
+ * dynamic edge's points by f w (-Cv/schur) g; __ee_dy with dvn = sum g.v moves all
+ * four by -f w g (dvn/schur); both only when closing).  ee null:
+ * slx_velocity_pass. */
+int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                         const float* vs4, const float* fixed, const float* m_inv,
+                         const float* m_inv_st, float mu, int pair_cap, const int32_t* pair_cnt,
+                         const float* pair_rec, const slx_edge_sets* ee, float* x4, float dt,
+                         int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
+                         slx_stream_t stream);
 
 /* ---------------- graph-coloured Gauss-Seidel ---------------- */
 

@@ -23,6 +23,28 @@
 extern "C" {
 #endif
 
+/* Edge-edge sets of one collision pass (XPBD.py:75-89 pair fields, 217-232 broad
+   phase, 266-281 and 313-330 loops).  Candidate faces per dynamic edge, row-major
+   [ne][cap] with counts; a null candidate array leaves that set out.  Pair
+   records [ne][pcap]: pi int[2] (other edge, type), pg R[12] (g0..g3), ps R[1]
+   (schur), nullable; pn_* (per edge) count the contacting pairs, uncapped.  The
+   velocity pass reads pn_*, pi, pg, ps. */
+typedef struct orc_ee {
+  int ne;                                      /* dynamic edges (queries) */
+  const int* edges_dy;                         /* [ne][2] */
+  const int* edges_st;                         /* static edges [][2] */
+  const int* fe_dy;                            /* face -> edge ids, [F][3] */
+  const int* fe_st;                            /* [Fs][3] */
+  int cap;                                     /* candidate list width */
+  const int *st_c, *st_n, *dy_c, *dy_n;        /* candidate faces of the static / dynamic trees */
+  int pcap;                                    /* pair records kept per edge */
+  int *pn_st, *pn_dy;
+  int* pi_st;
+  void *pg_st, *ps_st;                         /* R arrays */
+  int* pi_dy;
+  void *pg_dy, *ps_dy;
+} orc_ee;
+
 #define ORC_DECL_REAL(R, S)                                                                 \
   /* XPBD.py:115-118 compute_y */                                                            \
   void orc_compute_y_##S(int nv, const R* x, const R* v, const R* fixed, R* y, R dt,        \
@@ -87,7 +109,30 @@ extern "C" {
                              const R* m_inv_st, R mu, int cap, const int* pn0,              \
                              const int* pi0, const R* pg0, const R* ps0, const int* pn1,    \
                              const int* pi1, const R* pg1, const R* ps1, const int* pn2,    \
-                             const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc);
+                             const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc);     \
+  /* distance.py:49-99 d_type_EE (8 = both segment interiors) */                             \
+  int orc_d_type_ee_##S(const R* x0, const R* x1, const R* x2, const R* x3);                \
+  /* distance.py:118-124 d_EE (squared line-line distance) + gradient (own derivation) */    \
+  R orc_dist_ee_##S(const R* x0, const R* x1, const R* x2, const R* x3, R* g12);            \
+  /* orc_collision_pass_rec with the edge-edge loops after the vertex-triangle ones          \
+     (XPBD.py:266-281, collision_constraints_x.py:312-507), then update_dx */               \
+  void orc_collision_pass_ee_##S(int nv, int nvs, const int* faces_dy, const int* faces_st, \
+                                 R* y, const R* xs, const R* fixed, const R* m_inv, R dhat,  \
+                                 R k_col, int cap, const int* vt_st_c, const int* vt_st_n,  \
+                                 const int* vt_dy_c, const int* vt_dy_n,                    \
+                                 const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,      \
+                                 int* pair_num_vt_st, int* pair_num_vt_dy,                  \
+                                 int* pair_num_tv_st, int* pi0, R* pg0, R* ps0, int* pi1,   \
+                                 R* pg1, R* ps1, int* pi2, R* pg2, R* ps2, const orc_ee* ee); \
+  /* orc_velocity_pass with the edge-edge pairs after the vertex-triangle ones              \
+     (XPBD.py:313-330, collision_constraints_v.py:573-878), then update_dv */              \
+  void orc_velocity_pass_ee_##S(int nv, int nvs, const int* faces_dy, const int* faces_st,  \
+                                R* v, const R* vs, const R* fixed, const R* m_inv,          \
+                                const R* m_inv_st, R mu, int cap, const int* pn0,           \
+                                const int* pi0, const R* pg0, const R* ps0, const int* pn1, \
+                                const int* pi1, const R* pg1, const R* ps1, const int* pn2, \
+                                const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc,  \
+                                const orc_ee* ee);
 
 ORC_DECL_REAL(float, f)
 ORC_DECL_REAL(double, d)

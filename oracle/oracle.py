@@ -35,6 +35,8 @@ def lib():
         _LIB.orc_flat_cell_id.restype = C.c_int32
         _LIB.orc_dist_f.restype = C.c_float
         _LIB.orc_dist_d.restype = C.c_double
+        _LIB.orc_dist_ee_f.restype = C.c_float
+        _LIB.orc_dist_ee_d.restype = C.c_double
     return _LIB
 
 
@@ -138,6 +140,73 @@ def dist(dt_, p, t0, t1, t2, dtype=np.float32):
     g = np.zeros(12, dtype)
     d = getattr(lib(), "orc_dist_" + s)(int(dt_), *[P(q) for q in a], P(g))
     return d, g.reshape(4, 3)
+
+
+def d_type_ee(x0, x1, x2, x3, dtype=np.float32):
+    s = "f" if np.dtype(dtype) == np.float32 else "d"
+    a = [np.ascontiguousarray(q, dtype) for q in (x0, x1, x2, x3)]
+    return getattr(lib(), "orc_d_type_ee_" + s)(*[P(q) for q in a])
+
+
+def dist_ee(x0, x1, x2, x3, dtype=np.float32):
+    s = "f" if np.dtype(dtype) == np.float32 else "d"
+    a = [np.ascontiguousarray(q, dtype) for q in (x0, x1, x2, x3)]
+    g = np.zeros(12, dtype)
+    d = getattr(lib(), "orc_dist_ee_" + s)(*[P(q) for q in a], P(g))
+    return d, g.reshape(4, 3)
+
+
+class _EE(C.Structure):
+    _fields_ = [("ne", C.c_int), ("edges_dy", C.c_void_p), ("edges_st", C.c_void_p), ("fe_dy", C.c_void_p),
+                ("fe_st", C.c_void_p), ("cap", C.c_int), ("st_c", C.c_void_p), ("st_n", C.c_void_p),
+                ("dy_c", C.c_void_p), ("dy_n", C.c_void_p), ("pcap", C.c_int), ("pn_st", C.c_void_p),
+                ("pn_dy", C.c_void_p), ("pi_st", C.c_void_p), ("pg_st", C.c_void_p), ("ps_st", C.c_void_p),
+                ("pi_dy", C.c_void_p), ("pg_dy", C.c_void_p), ("ps_dy", C.c_void_p)]
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, np.int32)
+
+
+def _ee_struct(ee, keep, real=None, cap=0, record=False):
+    """Fill an orc_ee from a dict: edges_dy, edges_st, fe_dy, fe_st, optional
+    st=(cand, count) / dy=(cand, count) candidate faces, pcap; with record, fresh
+    pair arrays (returned in `keep`, which also holds every array the struct
+    points to).  For the velocity pass, ee also carries the pair arrays."""
+    E = _EE()
+    ne = len(ee["edges_dy"])
+    E.ne = ne
+    for k in ("edges_dy", "edges_st", "fe_dy", "fe_st"):
+        a = _i32(ee.get(k))
+        keep[k] = a
+        setattr(E, k, P(a))
+    E.cap = cap
+    E.pcap = int(ee.get("pcap", 3 * cap))
+    for nm in ("st", "dy"):
+        cn = ee.get(nm)
+        if cn is not None and "pn_" + nm not in ee:
+            c, n = _i32(cn[0]), _i32(cn[1])
+            keep[nm + "_c"], keep[nm + "_n"] = c, n
+            setattr(E, nm + "_c", P(c))
+            setattr(E, nm + "_n", P(n))
+        if record:
+            if cn is None:
+                continue
+            n1 = max(ne, 1)
+            keep["pn_" + nm] = np.zeros(n1, np.int32)
+            keep["pi_" + nm] = np.zeros((n1, max(E.pcap, 1), 2), np.int32)
+            keep["pg_" + nm] = np.zeros((n1, max(E.pcap, 1), 4, 3), real)
+            keep["ps_" + nm] = np.zeros((n1, max(E.pcap, 1)), real)
+        elif "pn_" + nm in ee and ee["pn_" + nm] is not None:
+            keep["pn_" + nm] = _i32(ee["pn_" + nm])
+            keep["pi_" + nm] = _i32(ee["pi_" + nm])
+            keep["pg_" + nm] = np.ascontiguousarray(ee["pg_" + nm], real)
+            keep["ps_" + nm] = np.ascontiguousarray(ee["ps_" + nm], real)
+        else:
+            continue
+        for f in ("pn_", "pi_", "pg_", "ps_"):
+            setattr(E, f + nm, P(keep[f + nm]))
+    return E
 
 
 def morton3d(x, y, z):
@@ -307,11 +376,19 @@ def greedy_coloring(nv, edges, max_colors=64):
     return color, order, offsets[:nc + 1]
 
 
-def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st, record=False):
+def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st, record=False,
+                   ee=None):
     """Position-level collision pass; with record=True the result also holds the
     pair records of the three query sets ("pairs": per set a dict of num [n],
     fid/dtype [n, cap], g [n, cap, 4, 3], schur [n, cap]), the reference's
-    vt_st_pair/vt_dy_pair/tv_st_pair fields (XPBD.py:51-73)."""
+    vt_st_pair/vt_dy_pair/tv_st_pair fields (XPBD.py:51-73).
+
+    ee: edge-edge sets (XPBD.py:266-281), a dict of edges_dy [ne, 2], edges_st,
+    fe_dy [F, 3], fe_st (face -> edge ids), st / dy = (cand [ne, cap], count) or
+    None, pcap (pair records kept per edge, default 3 cap).  The result's "ee"
+    holds per set the contacting pair count pn_* (uncapped) and the records
+    pi_* (other edge, type), pg_*, ps_* (with record=True), ready for
+    velocity_pass."""
     ct, s = _real(y.dtype)
     nv = len(fixed)
     nvs = 0 if xs is None else len(xs)
@@ -329,11 +406,24 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     (a, an), (b, bn), (c, cnn) = cn(vt_st), cn(vt_dy), cn(tv_st)
     xs_ = np.zeros((1, 3), y.dtype) if xs is None else np.ascontiguousarray(xs, y.dtype)
     fst = np.zeros((1, 3), np.int32) if faces_st is None else np.ascontiguousarray(faces_st, np.int32)
-    getattr(lib(), "orc_collision_pass_rec_" + s)(
-        nv, nvs, P(mesh_dy["faces"]), P(fst), P(y), P(xs_), P(fixed), P(m_inv), ct(dhat), ct(k_col), cap,
-        P(a), P(an), P(b), P(bn), P(c), P(cnn), P(dx), P(nc), P(pn[0]), P(pn[1]), P(pn[2]),
-        *[P(arr) for r in rec for arr in r])
+    args = (nv, nvs, P(mesh_dy["faces"]), P(fst), P(y), P(xs_), P(fixed), P(m_inv), ct(dhat), ct(k_col), cap,
+            P(a), P(an), P(b), P(bn), P(c), P(cnn), P(dx), P(nc), P(pn[0]), P(pn[1]), P(pn[2]),
+            *[P(arr) for r in rec for arr in r])
     out = dict(dx=dx, nc=nc, pair_num=pn)
+    if ee is None:
+        getattr(lib(), "orc_collision_pass_rec_" + s)(*args)
+    else:
+        keep = {}
+        E = _ee_struct(ee, keep, real=y.dtype, cap=cap, record=True)
+        if not record:
+            for nm in ("st", "dy"):
+                for f in ("pi_", "pg_", "ps_"):
+                    setattr(E, f + nm, None)
+                    keep.pop(f + nm, None)
+        getattr(lib(), "orc_collision_pass_ee_" + s)(*args, C.byref(E))
+        out["ee"] = dict(edges_dy=keep["edges_dy"], edges_st=keep["edges_st"], pcap=E.pcap,
+                         **{k: keep.get(k) for nm in ("st", "dy") for k in
+                            ("pn_" + nm, "pi_" + nm, "pg_" + nm, "ps_" + nm)})
     if record:
         present = (vt_st is not None, vt_dy is not None, tv_st is not None)
         out["pairs"] = [dict(num=pn[k], fid=rec[k][0][..., 0], dtype=rec[k][0][..., 1], g=rec[k][1],
@@ -342,12 +432,15 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     return out
 
 
-def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, cap, nvs=None):
+def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, cap, nvs=None, ee=None):
     """solve_constraints_v (XPBD.py:374-381, 283-311; collision_constraints_v.py
     5-570; update_dv 397-404) over pair records from collision_pass(record=True);
     v is updated in place.  pairs None: no collision, update_dv alone.
     vs None: static vertices at rest.  nvs: static vertex count (default: from
-    vs, m_inv_st, the static-vertex pair records or the static faces)."""
+    vs, m_inv_st, the static-vertex pair records or the static faces).
+    ee: collision_pass(record=True, ee=...)["ee"] -- the edge-edge pairs follow
+    the vertex-triangle ones (XPBD.py:313-330; collision_constraints_v.py
+    573-878)."""
     ct, s = _real(v.dtype)
     nv = len(fixed)
     if nvs is None:
@@ -370,9 +463,14 @@ def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, c
     for k in range(3):
         p = None if pairs is None else pairs[k]
         args += [None, None, None, None] if p is None else [P(p["num"]), P(p["pi"]), P(p["g"]), P(p["schur"])]
-    getattr(lib(), "orc_velocity_pass_" + s)(
-        nv, nvs, P(mesh_dy["faces"]), P(fst), P(v), P(vs_), P(fixed), P(m_inv), P(mst), ct(mu), cap, *args,
-        P(dv), P(nc))
+    call = (nv, nvs, P(mesh_dy["faces"]), P(fst), P(v), P(vs_), P(fixed), P(m_inv), P(mst), ct(mu), cap, *args,
+            P(dv), P(nc))
+    if ee is None:
+        getattr(lib(), "orc_velocity_pass_" + s)(*call)
+    else:
+        keep = {}
+        E = _ee_struct(ee, keep, real=v.dtype)
+        getattr(lib(), "orc_velocity_pass_ee_" + s)(*call, C.byref(E))
     return dict(dv=dv, nc=nc)
 
 

@@ -404,14 +404,14 @@ static void CAT(vt_dy_, S)(int vd, int fd, const int* faces_dy, R* y, const R* f
   CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fd, dtype, z, (R)0);
 }
 
-/* XPBD.py:236-264 + update_dx.  pi_k/pg_k/ps_k (nullable) receive the pair
-   records of query set k (0 vt_st, 1 vt_dy, 2 tv_st) for the velocity pass. */
-void FN(collision_pass_rec)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
-                            const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
-                            const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
-                            const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
-                            int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0, R* pg0, R* ps0,
-                            int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2) {
+/* The three vertex-triangle loops of solve_collision_constraints_x
+   (XPBD.py:236-264), dx/nc/pair counts zeroed first. */
+static void CAT(vt_loops_, S)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
+                              const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
+                              const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
+                              const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
+                              int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0, R* pg0, R* ps0,
+                              int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2) {
   for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
   for (int i = 0; i < nv; ++i) { nc[i] = 0; pn_vt_st[i] = 0; pn_vt_dy[i] = 0; }
   for (int i = 0; i < nvs; ++i) pn_tv_st[i] = 0;
@@ -438,6 +438,205 @@ void FN(collision_pass_rec)(int nv, int nvs, const int* faces_dy, const int* fac
         CAT(tv_st_, S)(tv_st_c[v * cap + j], v, faces_dy, y, xs, fixed, m_inv, dhat, k_col, dx, nc, cap, pn_tv_st,
                        pi2, pg2, ps2);
     }
+}
+
+/* XPBD.py:236-264 + update_dx.  pi_k/pg_k/ps_k (nullable) receive the pair
+   records of query set k (0 vt_st, 1 vt_dy, 2 tv_st) for the velocity pass. */
+void FN(collision_pass_rec)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y,
+                            const R* xs, const R* fixed, const R* m_inv, R dhat, R k_col, int cap,
+                            const int* vt_st_c, const int* vt_st_n, const int* vt_dy_c,
+                            const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
+                            int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0, R* pg0, R* ps0,
+                            int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2) {
+  CAT(vt_loops_, S)(nv, nvs, faces_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st_c, vt_st_n,
+                    vt_dy_c, vt_dy_n, tv_st_c, tv_st_n, dx, nc, pn_vt_st, pn_vt_dy, pn_tv_st, pi0, pg0, ps0, pi1,
+                    pg1, ps1, pi2, pg2, ps2);
+  CAT(update_dx_, S)(nv, y, fixed, dx, nc);
+}
+
+/* ---------------- edge-edge (distance.py:49-124; collision_constraints_x.py:312-507) ---- */
+
+/* distance.py:49-99 d_type_EE: region of the closest points of segments
+   (x0, x1) and (x2, x3) (Sunday's segment-segment parametrisation); 8 = both
+   interiors, the only type the edge-edge constraints act on. */
+int FN(d_type_ee)(const R* x0, const R* x1, const R* x2, const R* x3) {
+  R u[3], v[3], w[3], uv[3];
+  SUB(x1, x0, u);
+  SUB(x3, x2, v);
+  SUB(x0, x2, w);
+  R a = DOT(u, u), b = DOT(u, v), c = DOT(v, v), d = DOT(u, w), e = DOT(v, w);
+  R D = a * c - b * b;
+  R tD = D, tN;
+  int ty = 8;
+  R sN = b * e - c * d;
+  if (sN <= (R)0) {
+    tN = e;
+    tD = c;
+    ty = 2;
+  } else if (sN >= D) {
+    tN = e + b;
+    tD = c;
+    ty = 5;
+  } else {
+    tN = a * e - b * d;
+    CROSS(u, v, uv);
+    if (tN > (R)0 && tN < tD && (DOT(uv, w) < (R)1e-4f || DOT(uv, uv) < ((R)1.0e-20f * a) * c)) {
+      if (sN < D / (R)2) {
+        tN = e;
+        tD = c;
+        ty = 2;
+      } else {
+        tN = e + b;
+        tD = c;
+        ty = 5;
+      }
+    }
+  }
+  if (tN <= (R)0) {
+    if (-d <= (R)0) ty = 0;
+    else if (-d >= a) ty = 3;
+    else ty = 6;
+  } else if (tN >= tD) {
+    if ((-d + b) <= (R)0) ty = 1;
+    else if ((-d + b) >= a) ty = 4;
+    else ty = 7;
+  }
+  return ty;
+}
+
+/* distance.py:118-124 d_EE: squared distance of the lines through (x0, x1) and
+   (x2, x3), d = s^2 / N with u = x1 - x0, v = x3 - x2, w = x2 - x0, b = u x v,
+   s = w . b, N = b . b.  g[12] = its gradient (derived here): with
+   ds/d(x0..x3) = (-b - v x w, v x w, b - w x u, w x u) and
+   dN/d(x0..x3) = 2 (-(v x b), v x b, -(b x u), b x u),
+   g_k = (2 s / N) ds_k - (s^2 / N^2) dN_k. */
+R FN(dist_ee)(const R* x0, const R* x1, const R* x2, const R* x3, R* g) {
+  R u[3], v[3], w[3], b[3], vw[3], wu[3], vb[3], bu[3];
+  SUB(x1, x0, u);
+  SUB(x3, x2, v);
+  SUB(x2, x0, w);
+  CROSS(u, v, b);
+  R s = DOT(w, b), N = DOT(b, b);
+  R d = (s * s) / N;
+  CROSS(v, w, vw);
+  CROSS(w, u, wu);
+  CROSS(v, b, vb);
+  CROSS(b, u, bu);
+  R p = ((R)2 * s) / N, q = (s * s) / (N * N);
+  for (int c = 0; c < 3; ++c) {
+    g[c] = p * (-b[c] - vw[c]) - q * ((R)2 * -vb[c]);
+    g[3 + c] = p * vw[c] - q * ((R)2 * vb[c]);
+    g[6 + c] = p * (b[c] - wu[c]) - q * ((R)2 * -bu[c]);
+    g[9 + c] = p * wu[c] - q * ((R)2 * bu[c]);
+  }
+  return d;
+}
+
+/* One edge pair, collision_constraints_x.py:312-424 (__ee_dy, st = 0) and
+   :426-507 (__ee_st, st = 1).  Only type 8 acts, and only for edges further
+   than 1e-3 from parallel (|x01 x x23| > 1e-3); then d = d_EE and, if d < dhat,
+   the compliant projection ld = k (dhat - d) / (k schur + 1) moves the dynamic
+   points (all four for __ee_dy, the first edge's two for __ee_st; schur over
+   the same points).  Returns 1 (and g, schur) when d < dhat. */
+static int CAT(ee_pair_, S)(int st, const int* vid, const R* p0, const R* p1, const R* p2, const R* p3,
+                            const R* fixed, const R* m_inv, R dhat, R k, R* dx, R* nc, int* dtype_out, R* g,
+                            R* schur_out) {
+  int dtype = FN(d_type_ee)(p0, p1, p2, p3);
+  *dtype_out = dtype;
+  for (int c = 0; c < 12; ++c) g[c] = 0;
+  *schur_out = 0;
+  if (dtype != 8) return 0;
+  R a01[3], a23[3], cr[3];
+  SUB(p0, p1, a01);
+  SUB(p2, p3, a23);
+  CROSS(a01, a23, cr);
+  if (!(SQRT(DOT(cr, cr)) > (R)1e-3f)) return 0;
+  R d = FN(dist_ee)(p0, p1, p2, p3, g);
+  if (!(d < dhat)) {
+    for (int c = 0; c < 12; ++c) g[c] = 0;
+    return 0;
+  }
+  int nmov = st ? 2 : 4;
+  R schur = 0;
+  for (int s_ = 0; s_ < nmov; ++s_) {
+    R term = (fixed[vid[s_]] * m_inv[vid[s_]]) * DOT(g + 3 * s_, g + 3 * s_);
+    schur = s_ == 0 ? term : schur + term;
+  }
+  R ld = (k * (dhat - d)) / ((k * schur) + (R)1);
+  for (int s_ = 0; s_ < nmov; ++s_) {
+    int v = vid[s_];
+    R a = (fixed[v] * m_inv[v]) * ld;
+    for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s_ + c];
+  }
+  for (int s_ = 0; s_ < nmov; ++s_) nc[vid[s_]] = nc[vid[s_]] + (R)1;
+  *schur_out = schur;
+  return 1;
+}
+
+/* The edge-edge loops of solve_collision_constraints_x (XPBD.py:266-281): each
+   dynamic edge e against the three edges (face_edge_indices) of each candidate
+   face, static set (__ee_st) then dynamic set (__ee_dy, skipping edges that
+   share a vertex with e, XPBD.py:130-137).  A contacting pair is counted in
+   pn_*[e] (the reference writes past its 40-entry pair fields) and stored, while
+   fewer than pcap are, as (other edge, type), g0..g3, schur at [e][pcap]. */
+static void CAT(ee_loops_, S)(const R* y, const R* xs, const R* fixed, const R* m_inv, R dhat, R k, R* dx, R* nc,
+                              const orc_ee* E) {
+  for (int set = 0; set < 2; ++set) {
+    const int* C = set == 0 ? E->st_c : E->dy_c;
+    const int* Nn = set == 0 ? E->st_n : E->dy_n;
+    int* pn = set == 0 ? E->pn_st : E->pn_dy;
+    if (!C) continue;
+    int* pi = set == 0 ? E->pi_st : E->pi_dy;
+    R* pg = (R*)(set == 0 ? E->pg_st : E->pg_dy);
+    R* ps = (R*)(set == 0 ? E->ps_st : E->ps_dy);
+    const int* fe = set == 0 ? E->fe_st : E->fe_dy;
+    const int* E2 = set == 0 ? E->edges_st : E->edges_dy;
+    const R* P2 = set == 0 ? xs : y;
+    for (int e = 0; e < E->ne; ++e) {
+      int n = Nn[e] < E->cap ? Nn[e] : E->cap;
+      int v0 = E->edges_dy[2 * e], v1 = E->edges_dy[2 * e + 1];
+      for (int j = 0; j < n; ++j) {
+        int f = C[e * E->cap + j];
+        for (int kk = 0; kk < 3; ++kk) {
+          int e2 = fe[3 * f + kk];
+          int v2 = E2[2 * e2], v3 = E2[2 * e2 + 1];
+          if (set == 1 && (v0 == v2 || v0 == v3 || v1 == v2 || v1 == v3)) continue;
+          int vid[4] = {v0, v1, v2, v3};
+          R g[12], schur;
+          int dtype;
+          if (!CAT(ee_pair_, S)(set == 0, vid, y + 3 * v0, y + 3 * v1, P2 + 3 * v2, P2 + 3 * v3, fixed, m_inv, dhat,
+                                k, dx, nc, &dtype, g, &schur))
+            continue;
+          int at = pn[e];
+          if (pi && at < E->pcap) {
+            size_t r = (size_t)e * (size_t)E->pcap + (size_t)at;
+            pi[2 * r] = e2;
+            pi[2 * r + 1] = dtype;
+            for (int c = 0; c < 12; ++c) pg[12 * r + c] = g[c];
+            ps[r] = schur;
+          }
+          pn[e] = at + 1;
+        }
+      }
+    }
+  }
+}
+
+/* solve_collision_constraints_x with the edge-edge loops after the
+   vertex-triangle ones (XPBD.py:236-281), then update_dx. */
+void FN(collision_pass_ee)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* y, const R* xs,
+                           const R* fixed, const R* m_inv, R dhat, R k_col, int cap, const int* vt_st_c,
+                           const int* vt_st_n, const int* vt_dy_c, const int* vt_dy_n, const int* tv_st_c,
+                           const int* tv_st_n, R* dx, R* nc, int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0,
+                           R* pg0, R* ps0, int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2, const orc_ee* E) {
+  CAT(vt_loops_, S)(nv, nvs, faces_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st_c, vt_st_n,
+                    vt_dy_c, vt_dy_n, tv_st_c, tv_st_n, dx, nc, pn_vt_st, pn_vt_dy, pn_tv_st, pi0, pg0, ps0, pi1,
+                    pg1, ps1, pi2, pg2, ps2);
+  for (int e = 0; e < E->ne; ++e) {
+    if (E->pn_st) E->pn_st[e] = 0;
+    if (E->pn_dy) E->pn_dy[e] = 0;
+  }
+  CAT(ee_loops_, S)(y, xs, fixed, m_inv, dhat, k_col, dx, nc, E);
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
 }
 
@@ -694,11 +893,11 @@ static void CAT(vel_vt_dy_, S)(int v0, int fd, int dtype, const R* g, R schur, c
    (XPBD.py:397-404): dv /= nc where nc > 0, v += f dv where m_inv > 0.
    pn_k null: set k absent.  vs: static vertex velocities (zeros in the
    reference). */
-void FN(velocity_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* v, const R* vs,
-                       const R* fixed, const R* m_inv, const R* m_inv_st, R mu, int cap,
-                       const int* pn0, const int* pi0, const R* pg0, const R* ps0,
-                       const int* pn1, const int* pi1, const R* pg1, const R* ps1,
-                       const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc) {
+static void CAT(vel_vt_loops_, S)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* v, const R* vs,
+                                  const R* fixed, const R* m_inv, const R* m_inv_st, R mu, int cap,
+                                  const int* pn0, const int* pi0, const R* pg0, const R* ps0,
+                                  const int* pn1, const int* pi1, const R* pg1, const R* ps1,
+                                  const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc) {
   for (int i = 0; i < 3 * nv; ++i) dv[i] = 0;
   for (int i = 0; i < nv; ++i) nc[i] = 0;
   if (pn0)
@@ -722,12 +921,89 @@ void FN(velocity_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st
         CAT(vel_tv_st_, S)(q, pi2[2 * at], pi2[2 * at + 1], pg2 + 12 * at, ps2[at], faces_dy, v, vs, fixed,
                            m_inv, mu, dv, nc);
       }
+}
+
+/* update_dv (XPBD.py:397-404): dv /= nc where nc > 0, v += f dv where m_inv > 0. */
+static void CAT(update_dv_, S)(int nv, R* v, const R* fixed, const R* m_inv, R* dv, const R* nc) {
   for (int i = 0; i < nv; ++i) {
     if (nc[i] > (R)0)
       for (int c = 0; c < 3; ++c) dv[3 * i + c] = dv[3 * i + c] / nc[i];
     if (m_inv[i] > (R)0)
       for (int c = 0; c < 3; ++c) v[3 * i + c] = v[3 * i + c] + fixed[i] * dv[3 * i + c];
   }
+}
+
+void FN(velocity_pass)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* v, const R* vs,
+                       const R* fixed, const R* m_inv, const R* m_inv_st, R mu, int cap,
+                       const int* pn0, const int* pi0, const R* pg0, const R* ps0,
+                       const int* pn1, const int* pi1, const R* pg1, const R* ps1,
+                       const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc) {
+  CAT(vel_vt_loops_, S)(nv, nvs, faces_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, cap, pn0, pi0, pg0, ps0,
+                        pn1, pi1, pg1, ps1, pn2, pi2, pg2, ps2, dv, nc);
+  CAT(update_dv_, S)(nv, v, fixed, m_inv, dv, nc);
+}
+
+/* collision_constraints_v.py:573-819 __ee_st (type 8, the only one recorded):
+   Cv = g0.v0 + g1.v1 + g3.vs3 -- g2 does not enter (:782) -- and if Cv < 0,
+   ld = -Cv / schur, dv_k += f_k w_k ld g_k on the dynamic edge's points.
+   :821-878 __ee_dy: dvn = sum g_k.v_k and if dvn < 0, ld = dvn / schur,
+   dv_k -= f_k w_k g_k ld on all four points.  Normal only, nc += 1 per point. */
+static void CAT(vel_ee_loops_, S)(const R* v, const R* vs, const R* fixed, const R* m_inv, R* dv, R* nc,
+                                  const orc_ee* E) {
+  for (int set = 0; set < 2; ++set) {
+    const int* pn = set == 0 ? E->pn_st : E->pn_dy;
+    if (!pn) continue;
+    const int* pi = set == 0 ? E->pi_st : E->pi_dy;
+    const R* pg = (const R*)(set == 0 ? E->pg_st : E->pg_dy);
+    const R* ps = (const R*)(set == 0 ? E->ps_st : E->ps_dy);
+    for (int e = 0; e < E->ne; ++e) {
+      int n = pn[e] < E->pcap ? pn[e] : E->pcap;
+      int v0 = E->edges_dy[2 * e], v1 = E->edges_dy[2 * e + 1];
+      for (int j = 0; j < n; ++j) {
+        size_t r = (size_t)e * (size_t)E->pcap + (size_t)j;
+        int e2 = pi[2 * r];
+        const R* g = pg + 12 * r;
+        R schur = ps[r];
+        if (set == 0) {
+          int v3 = E->edges_st[2 * e2 + 1];
+          R Cv = (DOT(g, v + 3 * v0) + DOT(g + 3, v + 3 * v1)) + DOT(g + 9, vs + 3 * v3);
+          if (!(Cv < (R)0)) continue;
+          R ld = -Cv / schur;
+          int vid[2] = {v0, v1};
+          for (int s_ = 0; s_ < 2; ++s_) {
+            R d[3], a = (fixed[vid[s_]] * m_inv[vid[s_]]) * ld;
+            for (int c = 0; c < 3; ++c) d[c] = a * g[3 * s_ + c];
+            CAT(vadd_, S)(dv, nc, vid[s_], d);
+          }
+        } else {
+          int vid[4] = {v0, v1, E->edges_dy[2 * e2], E->edges_dy[2 * e2 + 1]};
+          R dvn = ((DOT(g, v + 3 * vid[0]) + DOT(g + 3, v + 3 * vid[1])) + DOT(g + 6, v + 3 * vid[2])) +
+                  DOT(g + 9, v + 3 * vid[3]);
+          if (!(dvn < (R)0)) continue;
+          R ld = dvn / schur;
+          for (int s_ = 0; s_ < 4; ++s_) {
+            R d[3], a = fixed[vid[s_]] * m_inv[vid[s_]];
+            for (int c = 0; c < 3; ++c) d[c] = -((a * g[3 * s_ + c]) * ld);
+            CAT(vadd_, S)(dv, nc, vid[s_], d);
+          }
+        }
+      }
+    }
+  }
+}
+
+/* solve_constraints_v with the edge-edge pairs after the vertex-triangle ones
+   (XPBD.py:283-330), then update_dv. */
+void FN(velocity_pass_ee)(int nv, int nvs, const int* faces_dy, const int* faces_st, R* v, const R* vs,
+                          const R* fixed, const R* m_inv, const R* m_inv_st, R mu, int cap,
+                          const int* pn0, const int* pi0, const R* pg0, const R* ps0,
+                          const int* pn1, const int* pi1, const R* pg1, const R* ps1,
+                          const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc,
+                          const orc_ee* E) {
+  CAT(vel_vt_loops_, S)(nv, nvs, faces_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, cap, pn0, pi0, pg0, ps0,
+                        pn1, pi1, pg1, ps1, pn2, pi2, pg2, ps2, dv, nc);
+  CAT(vel_ee_loops_, S)(v, vs, fixed, m_inv, dv, nc, E);
+  CAT(update_dv_, S)(nv, v, fixed, m_inv, dv, nc);
 }
 
 #undef FN

@@ -53,6 +53,8 @@ SIGNATURES = {
                       C.c_size_t, STREAM],
     "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
     "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
+    "slx_traverse_edges": [I32, I32, PTR, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR,
+                           PTR, STREAM],
     "slx_traverse_cached": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, PTR, STREAM],
     "slx_set_walk_stack": [I32],
     "slx_tree_snapshot": [I32, PTR, PTR, PTR, STREAM],
@@ -60,11 +62,15 @@ SIGNATURES = {
     "slx_collision_ws_bytes": [I32, I32, I32],
     "slx_collision_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
                            PTR, PTR, I32, PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
+    "slx_collision_pass_ee": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, F32, I32, PTR, PTR, PTR, PTR, PTR,
+                              PTR, PTR, I32, PTR, PTR, PTR, I32, PTR, PTR, C.c_size_t, STREAM],
     "slx_sewing_pass": [I32, PTR, PTR, PTR, PTR, PTR, PTR, F32, PTR, STREAM],
     "slx_compute_velocity": [I32, PTR, PTR, PTR, PTR, F32, F32, STREAM],
     "slx_update_x": [I32, PTR, PTR, F32, STREAM],
     "slx_velocity_pass": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, I32, PTR, PTR, PTR, F32, I32, PTR, PTR,
                           C.c_size_t, STREAM],
+    "slx_velocity_pass_ee": [I32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, F32, I32, PTR, PTR, PTR, PTR, F32, I32,
+                             PTR, PTR, C.c_size_t, STREAM],
     "slx_frame_ws_bytes": [PTR],
     "slx_frame": [PTR, STREAM],
     "slx_seg_build_ws_bytes": [I32, I32],
@@ -105,6 +111,13 @@ class Tree(C.Structure):
 class CandCache(C.Structure):
     """slx_cand_cache (include/slx.h)."""
     _fields_ = [("cap", I32), ("margin", F32), ("leaves", PTR), ("count", PTR), ("q0", PTR), ("redo", PTR)]
+
+
+class EdgeSets(C.Structure):
+    """slx_edge_sets (include/slx.h)."""
+    _fields_ = [("ne", I32), ("edges_dy", PTR), ("edges_st", PTR), ("fe_dy", PTR), ("fe_st", PTR), ("cap", I32),
+                ("c_st", PTR), ("n_st", PTR), ("c_dy", PTR), ("n_dy", PTR), ("pair_cap", I32), ("pair_num", PTR),
+                ("pair_cnt", PTR), ("pair_rec", PTR)]
 
 
 class FrameDesc(C.Structure):

@@ -215,3 +215,23 @@ class DeviceTree:
                   P(self.nhi4), P(self.quad4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n),
                   P(st), _abi.stream_handle())
         return cand, cand_n
+
+    def query_edges(self, edges, x4, padding, cap, cand=None, cand_n=None, status=None):
+        """Candidate faces of the edge boxes min/max(x4[e0], x4[e1]) -+ padding
+        (edges: [ne*2] int32 device), column-major (cap, ne) -- the edge broad
+        phase of XPBD.py:217-232."""
+        ne = edges.numel() // 2
+        dev = x4.device
+        if self.n_inst > 1:
+            raise ValueError("edge queries of batched trees are not supported")
+        if cand is None:
+            cand = torch.full((max(cap, 1), max(ne, 1)), -1, dtype=torch.int32, device=dev)
+        if cand_n is None:
+            cand_n = torch.zeros(max(ne, 1), dtype=torch.int32, device=dev)
+        st = self.status if status is None else status
+        P = _abi.ptr
+        _abi.call("slx_traverse_edges", 0 if self.mode == "cell" else 1, ne, P(edges), P(x4), float(padding),
+                  self.n_leaves if self.nf > 0 else 0, P(self.num), P(self.sorted_faces), P(self.nlo4),
+                  P(self.nhi4), P(self.quad4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n),
+                  P(st), _abi.stream_handle())
+        return cand, cand_n

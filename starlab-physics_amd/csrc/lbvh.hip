@@ -667,14 +667,33 @@ __global__ void k_add_in_place(int n, const int32_t* __restrict__ a, int32_t* __
 
 // ----------------------------------------------------------------- traversal
 
+// Query box of query q: the point q4[q] +- pad, or, with edge queries (qe
+// non-null, two vertex ids per query), the box of the edge's end points q4[qe[2q]],
+// q4[qe[2q+1]] +- pad (the edge broad phase of XPBD.py:217-232, commented out
+// there: min/max of the end points minus/plus padding).
+__device__ __forceinline__ void query_box(const float4* __restrict__ q4, const int32_t* __restrict__ qe, int q,
+                                          float pad, float3& a, float3& b) {
+  const float pp = pad * 1.0f;
+  if (qe) {
+    const float3 p0 = xyz(q4[qe[2 * q]]), p1 = xyz(q4[qe[2 * q + 1]]);
+    a = make_float3(fminf(p0.x, p1.x) - pp, fminf(p0.y, p1.y) - pp, fminf(p0.z, p1.z) - pp);
+    b = make_float3(fmaxf(p0.x, p1.x) + pp, fmaxf(p0.y, p1.y) + pp, fmaxf(p0.z, p1.z) + pp);
+  } else {
+    const float3 p = xyz(q4[q]);
+    a = make_float3(p.x - pp, p.y - pp, p.z - pp);
+    b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  }
+}
+
 // traverse_cell_bvh_single, lbvh_cell.py:661-714, for a batch of query points
-// p +- pad.  CELL: skip queries whose centre cell id exceeds 32767, skip empty
+// p +- pad (or edge boxes, query_box).  CELL: skip queries whose centre cell id exceeds 32767, skip empty
 // nodes, leaves list faces [l, r] of sorted; else triangle leaves (one face).
 // Children are pushed a then b, so b's subtree is visited first.  Candidates go
 // to cand[j * nq + q] (column-major) up to cap; overflow sets a status bit.
 template <bool CELL>
 __global__ void __launch_bounds__(kTravBS)
-k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32_t* __restrict__ num,
+k_traverse(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe, int nl,
+           const int32_t* __restrict__ num,
            const int32_t* __restrict__ sorted, const float4* __restrict__ nlo, const float4* __restrict__ nhi,
            const float4* __restrict__ fmin, const float4* __restrict__ fmax, const float* __restrict__ box,
            int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status,
@@ -684,9 +703,8 @@ k_traverse(int nq, const float4* __restrict__ q4, float pad, int nl, const int32
   if (t >= nq) return;
   const int q = order ? order[t] : t;
   int32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
-  const float3 p = xyz(q4[q]);
-  const float pp = pad * 1.0f;
-  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  float3 a, b;
+  query_box(q4, qe, q, pad, a, b);
   int cnt = 0, flags = 0;
   if (CELL) {
     float3 c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
@@ -927,7 +945,8 @@ namespace slx {
 constexpr uint32_t kRightLeaf = 1u << 31, kEmit = 1u << 30, kLow30 = kEmit - 1;
 
 __global__ void __launch_bounds__(kTravBS)
-k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe, int nl,
+               const float4* __restrict__ nlo,
                const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n,
                int32_t* status, const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst,
                const int32_t* __restrict__ order, const int32_t* __restrict__ n_active) {
@@ -936,9 +955,8 @@ k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, int nl, const f
   if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;     // spatially coherent queries share nodes within a wavefront
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
-  const float3 p = xyz(q4[q]);
-  const float pp = pad * 1.0f;
-  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  float3 a, b;
+  query_box(q4, qe, q, pad, a, b);
   int cnt = 0, flags = 0;
   const int root = roots ? roots[q_inst[q]] : 0;
   if (nl <= 0 || root < 0) {
@@ -1014,7 +1032,8 @@ constexpr uint32_t kCacheNode = 1u << 31;  // quad walk, cache build: without kE
 // BUILD and n_active as for k_traverse_quad (the cache arrays in CacheOut).
 template <bool BUILD>
 __global__ void __launch_bounds__(256)
-k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe, int nl,
+                    const float4* __restrict__ nlo,
                     const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand,
                     int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
                     const int32_t* __restrict__ n_active, CacheOut co) {
@@ -1024,13 +1043,11 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, int nl, co
   const int q = active ? (order ? order[t] : t) : 0;
   float3 a, b, aw, bw;
   if (active) {
-    const float3 p = xyz(q4[q]);
-    const float pp = pad * 1.0f;
-    a = make_float3(p.x - pp, p.y - pp, p.z - pp);
-    b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+    query_box(q4, qe, q, pad, a, b);
     aw = a;
     bw = b;
-    if (BUILD) {
+    if (BUILD) {                             // point queries only
+      const float3 p = xyz(q4[q]);
       const float pw = co.pad_walk;
       aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
       bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
@@ -1148,7 +1165,8 @@ namespace slx {
 // second walk); otherwise the overflow sets the status bit.
 template <bool BUILD, int S>
 __global__ void __launch_bounds__(kTravBS)
-k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const float4* __restrict__ nlo,
+k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe, int nl,
+                const float4* __restrict__ nlo,
                 const float4* __restrict__ nhi, const float4* __restrict__ quad, int cap, int32_t* __restrict__ cand,
                 int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ roots,
                 const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order,
@@ -1158,11 +1176,11 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, int nl, const 
   if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
-  const float3 p = xyz(q4[q]);
-  const float pp = pad * 1.0f;
-  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+  float3 a, b;
+  query_box(q4, qe, q, pad, a, b);
   float3 aw = a, bw = b;                          // the walk's box
-  if (BUILD) {
+  if (BUILD) {                                    // point queries only
+    const float3 p = xyz(q4[q]);
     const float pw = co.pad_walk;
     aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
     bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
@@ -1248,8 +1266,10 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
-                    const int32_t* q_inst, const int32_t* order, const int32_t* n_active) {
+                    const int32_t* q_inst, const int32_t* order, const int32_t* n_active,
+                    const int32_t* qedges) {
   SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
+  SLX_REQUIRE(!qedges || !roots, "slx_traverse: edge queries of batched trees are not supported");
   SLX_REQUIRE(!n_active || (order && mode == 1), "slx_traverse: a device query count needs a query list");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
@@ -1260,8 +1280,9 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   SLX_REQUIRE(mode == 0 || n_leaves < (1 << 29), "slx_traverse: triangle tree too large for 30-bit stack words");
   if (mode == 0)
     SLX_LAUNCH(name, k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
-               (const float4*)q4, padding, n_leaves, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4,
-               (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots, q_inst, order);
+               (const float4*)q4, padding, qedges, n_leaves, num, sorted_faces, (const float4*)nlo4,
+               (const float4*)nhi4, (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots,
+               q_inst, order);
   // One walk per lane, unless the tree is coarse next to the queries (far fewer
   // leaves than queries: the lanes of a wavefront then overlap the same few
   // leaves and the shared walk is short).  Batched trees: lanes have different
@@ -1269,17 +1290,17 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq) {
     if (quad4 && SLX_TRAV_QUAD)
       SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
-                 (const float4*)q4, padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4,
-                 cap, cand, cand_n, status, roots, q_inst, order, n_active, CacheOut{});
+                 (const float4*)q4, padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
+                 (const float4*)quad4, cap, cand, cand_n, status, roots, q_inst, order, n_active, CacheOut{});
     else
       SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
-                 padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots, q_inst,
-                 order, n_active);
+                 padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots,
+                 q_inst, order, n_active);
   }
   else
     SLX_LAUNCH(name, k_traverse_tri_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
-               padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order, n_active,
-               CacheOut{});
+               padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order,
+               n_active, CacheOut{});
   return check_launch("slx_traverse");
 }
 }  // namespace slx
@@ -1403,7 +1424,7 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
                     hipStream_t st, const int32_t* order, const CacheOut& co) {
   const dim3 g(blocks_for(nq, kTravBS)), b(kTravBS);
 #define SLX_QUAD_TIER(D)                                                                                         \
-  SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, n_leaves,            \
+  SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, nullptr, n_leaves,   \
              (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
              nullptr, order, nullptr, co)
   if (g_walk_stack <= 16) SLX_QUAD_TIER(16);
@@ -1440,8 +1461,8 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
     const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
     if (wave)
       SLX_LAUNCH("cache_build_wave", k_traverse_tri_wave<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
-                 (const float4*)q4, padding, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n,
-                 status, order, nullptr, co);
+                 (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand,
+                 cand_n, status, order, nullptr, co);
     else
       rc = quad_first_tier<true>("cache_build", nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n,
                                  status, st, order, co);
@@ -1501,6 +1522,17 @@ extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, in
   return traverse_launch(mode == 0 ? "traverse_cell" : "traverse_tri", mode, nq, q4, padding, n_leaves, num,
                          sorted_faces, nlo4, nhi4, quad4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream),
                          nullptr, nullptr, nullptr);
+}
+
+extern "C" int slx_traverse_edges(int mode, int ne, const int32_t* edges, const float* x4, float padding,
+                                  int n_leaves, const int32_t* num, const int32_t* sorted_faces, const float* nlo4,
+                                  const float* nhi4, const float* quad4, const float* fmin4, const float* fmax4,
+                                  const float* box6, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
+                                  slx_stream_t stream) {
+  SLX_REQUIRE(ne == 0 || (edges && x4), "slx_traverse_edges: null edges or positions");
+  return traverse_launch(mode == 0 ? "traverse_ee_cell" : "traverse_ee", mode, ne, x4, padding, n_leaves, num,
+                         sorted_faces, nlo4, nhi4, quad4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream),
+                         nullptr, nullptr, nullptr, nullptr, edges);
 }
 
 // ===================== batched instances: one tree, one subtree per instance =====================

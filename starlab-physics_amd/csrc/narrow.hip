@@ -310,13 +310,18 @@ extern "C" size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap) {
   return contact_ws_layout(nv, rec_cap, nullptr, nullptr);
 }
 
-extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
-                                  const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col,
-                                  int cap, const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
-                                  const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st,
-                                  int32_t* pair_num, int pair_cap, int32_t* pair_cnt, float* pair_rec,
-                                  int rec_cap, int32_t* status, void* ws, size_t ws_bytes, slx_stream_t stream) {
+extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
+                                     const float* xs4, const float* fixed, const float* m_inv, float dhat,
+                                     float k_col, int cap, const int32_t* c_vt_st, const int32_t* n_vt_st,
+                                     const int32_t* c_vt_dy, const int32_t* n_vt_dy, const int32_t* c_tv_st,
+                                     const int32_t* n_tv_st, int32_t* pair_num, int pair_cap, int32_t* pair_cnt,
+                                     float* pair_rec, const slx_edge_sets* ee, int rec_cap, int32_t* status,
+                                     void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
+  if (ee) {
+    int rc = ee_check(ee, false);
+    if (rc) return rc;
+  }
   if (nv == 0) return SLX_OK;
   SLX_REQUIRE(faces_dy && y4 && fixed && m_inv && ws, "slx_collision_pass: null pointer");
   SLX_REQUIRE(!(c_vt_st || c_tv_st) || (faces_st && xs4), "slx_collision_pass: static mesh missing");
@@ -342,7 +347,23 @@ extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, cons
     SLX_LAUNCH("contacts", k_contacts<false>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
                (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
                w.self_acc, pair_num, 0, nullptr, nullptr, status);
+  if (ee) {
+    int rc = ee_position_launch(nv, ee, (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w, rec_cap,
+                                status, st);
+    if (rc) return rc;
+  }
   SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt,
              w.brec, w.self_acc, fixed, (float4*)y4);
   return check_launch("slx_collision_pass");
+}
+
+extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
+                                  const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col,
+                                  int cap, const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
+                                  const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st,
+                                  int32_t* pair_num, int pair_cap, int32_t* pair_cnt, float* pair_rec,
+                                  int rec_cap, int32_t* status, void* ws, size_t ws_bytes, slx_stream_t stream) {
+  return slx_collision_pass_ee(nv, nvs, faces_dy, faces_st, y4, xs4, fixed, m_inv, dhat, k_col, cap, c_vt_st, n_vt_st,
+                               c_vt_dy, n_vt_dy, c_tv_st, n_tv_st, pair_num, pair_cap, pair_cnt, pair_rec, nullptr,
+                               rec_cap, status, ws, ws_bytes, stream);
 }

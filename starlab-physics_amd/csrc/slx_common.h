@@ -117,7 +117,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
                     const int32_t* q_inst = nullptr, const int32_t* order = nullptr,
-                    const int32_t* n_active = nullptr);
+                    const int32_t* n_active = nullptr, const int32_t* qedges = nullptr);
 
 // two-tier traversal of a triangle tree with a candidate cache (lbvh.hip): kind 0
 // = walk, 1 = walk with the enlarged box filling the cache, 2 = filter the cache;
@@ -126,6 +126,15 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order);
+
+// edge-edge phase (edges.hip): argument checks; the vertex-triangle bucket
+// records folded into self_acc, then the edge-edge deltas into the emptied buckets
+int ee_check(const slx_edge_sets* ee, bool velocity);
+int ee_position_launch(int nv, const slx_edge_sets* ee, const float4* y4, const float4* xs4, const float* fixed,
+                       const float* m_inv, float dhat, float k, const ContactWs& w, int bucket, int32_t* status,
+                       hipStream_t st);
+int ee_velocity_launch(int nv, const slx_edge_sets* ee, const float4* v4, const float4* vs4, const float* fixed,
+                       const float* m_inv, const ContactWs& w, int bucket, int32_t* status, hipStream_t st);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);

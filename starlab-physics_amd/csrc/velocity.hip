@@ -337,14 +337,19 @@ extern "C" int slx_update_x(int nv, float* x4, const float* v4, float dt, slx_st
   return check_launch("slx_update_x");
 }
 
-extern "C" int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
-                                 const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
-                                 float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec, float* x4,
-                                 float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
-                                 slx_stream_t stream) {
+extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                                    const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
+                                    float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec,
+                                    const slx_edge_sets* ee, float* x4, float dt, int rec_cap, int32_t* status,
+                                    void* ws, size_t ws_bytes, slx_stream_t stream) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && pair_cap >= 0 && rec_cap >= 1, "slx_velocity_pass: bad sizes");
   if (nv == 0) return SLX_OK;
   SLX_REQUIRE(v4 && fixed && m_inv, "slx_velocity_pass: null pointer");
+  if (ee) {
+    SLX_REQUIRE(pair_cnt, "slx_velocity_pass_ee: edge pairs come with the vertex-triangle pair records");
+    int rc = ee_check(ee, true);
+    if (rc) return rc;
+  }
   hipStream_t st = (hipStream_t)stream;
   const bool pairs = pair_cnt != nullptr;
   if (!pairs) {
@@ -365,9 +370,22 @@ extern "C" int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const
   SLX_LAUNCH("vel_contacts", k_vel_contacts, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, nv, nvs, faces_dy,
              faces_st, (const float4*)v4, (const float4*)vs4, fixed, m_inv, m_inv_st, mu, pair_cap, pair_cnt,
              (const float4*)pair_rec, rec_cap, w.bcnt, w.brec, w.self_acc, status);
+  if (ee) {
+    int rc = ee_velocity_launch(nv, ee, (const float4*)v4, (const float4*)vs4, fixed, m_inv, w, rec_cap, status, st);
+    if (rc) return rc;
+  }
   SLX_LAUNCH("vel_apply", k_vel_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt, w.brec,
              w.self_acc, fixed, m_inv, (float4*)v4, (float4*)x4, dt);
   return check_launch("slx_velocity_pass");
+}
+
+extern "C" int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                                 const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
+                                 float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec, float* x4,
+                                 float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes,
+                                 slx_stream_t stream) {
+  return slx_velocity_pass_ee(nv, nvs, faces_dy, faces_st, v4, vs4, fixed, m_inv, m_inv_st, mu, pair_cap, pair_cnt,
+                              pair_rec, nullptr, x4, dt, rec_cap, status, ws, ws_bytes, stream);
 }
 
 // ===================== sewing constraints (XPBD.py:418-441) =====================

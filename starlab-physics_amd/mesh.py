@@ -89,6 +89,18 @@ class Mesh:
             x=self.x4[:, :3], v=self.v4[:, :3], y=self.y4[:, :3], fixed=self.fixed,
             m_inv=self.m_inv, x0=self.x0)
 
+    def face_edge_indices(self):
+        """framework/meshio/meshTaichiWrapper.py:115-119 face_edge_indices: the
+        edge ids of each face's sides (v0v1, v1v2, v2v0), [F*3] int32 on the
+        device, built on first use."""
+        fe = getattr(self, "_face_edges", None)
+        if fe is None:
+            import torch
+            self.fe_np = _topo.face_edges(self.fid_np, self.eid_np)
+            fe = torch.from_numpy(self.fe_np.reshape(-1)).to(self.torch_device)
+            self._face_edges = fe
+        return fe
+
     def _refresh_products(self):
         """x4.w = y4.w = fixed*m_inv, v4.w = fixed (the products the kernels read)."""
         fw = self.fixed * self.m_inv

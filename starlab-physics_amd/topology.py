@@ -32,6 +32,30 @@ def edges_from_faces(faces: np.ndarray) -> np.ndarray:
     return e.astype(np.int32)
 
 
+def face_edges(faces: np.ndarray, edges: np.ndarray) -> np.ndarray:
+    """Edge ids of each face's sides (v0v1, v1v2, v2v0) in `edges` (lexicographic
+    (lo, hi) rows, as edges_from_faces gives): the reference's
+    face_edge_indices (framework/meshio/meshTaichiWrapper.py:115-119), whose
+    order within a face follows meshtaichi's relation order; this package fixes
+    it to the sides in vertex order."""
+    f = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    if f.shape[0] == 0:
+        return np.zeros((0, 3), np.int32)
+    n = int(max(f.max(), e.max() if e.size else 0)) + 1
+    key = e[:, 0] * n + e[:, 1]
+    if np.any(key[1:] <= key[:-1]):
+        raise ValueError("edges must be unique (lo, hi) rows in lexicographic order")
+    sides = np.stack([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=1)     # [F, 3, 2]
+    lo, hi = sides.min(axis=2), sides.max(axis=2)
+    q = lo * n + hi
+    at = np.searchsorted(key, q)
+    at = np.minimum(at, len(key) - 1)
+    if not np.all(key[at] == q):
+        raise ValueError("a face side is missing from the edge list")
+    return at.astype(np.int32)
+
+
 def bending_pairs(faces: np.ndarray) -> np.ndarray:
     """Opposite-vertex pairs across interior edges.
 
