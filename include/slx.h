@@ -569,6 +569,16 @@ typedef struct slx_frame_desc {
   const int32_t* sew_pairs;
   float k_sew;
   float* sew_ws;
+  /* edge-edge constraints (XPBD.py:217-232, 266-281, 313-330, commented out
+     there; ee.ne = 0: off).  Each collision pass queries the dynamic edges'
+     boxes against the static tree (when there is a static mesh) and the dynamic
+     tree, into the caller's buffers ee.c_st / ee.n_st / ee.c_dy / ee.n_dy
+     ([ee.cap][ne] and [ne]; written by the frame), and applies the edge-edge
+     constraints after the vertex-triangle ones; the pass that records the
+     vertex-triangle pairs for the velocity pass records the edge pairs too
+     (ee.pair_cnt / ee.pair_rec), and the velocity pass applies them.  Single
+     meshes only (no batched trees). */
+  slx_edge_sets ee;
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

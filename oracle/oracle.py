@@ -487,8 +487,13 @@ class OracleCloth:
     def __init__(self, mesh_dy, mesh_st=None, *, dhat, stiffness_stretch, stiffness_bending,
                  g, dt, dtype=np.float32, damping=0.001, padding=0.05, collision=False,
                  bvh="cell", cap=40, refit_every_substep=False, method="jacobi", velocity=False, mu=0.8,
-                 sewing=None, sewing_stiffness=None):
+                 sewing=None, sewing_stiffness=None, edge_edge=False, ee_pcap=None):
+        """edge_edge: the edge-edge constraints of XPBD.py:217-232, 266-281,
+        313-330 (mesh_dy and mesh_st then also carry edges [E, 2] and fe [F, 3],
+        the edge ids of each face's sides); ee_pcap: edge pairs kept per edge for
+        the velocity pass (default cap)."""
         self.method = method
+        self.edge_edge, self.ee_pcap = edge_edge, (cap if ee_pcap is None else ee_pcap)
         self.velocity, self.mu = velocity, mu
         self.sewing = None if sewing is None else np.ascontiguousarray(sewing, np.int32).reshape(-1, 2)
         self.sewing_stiffness = sewing_stiffness
@@ -538,6 +543,18 @@ class OracleCloth:
         c, n, fl = self.tree_dy.query(qmin, qmax, self.cap)
         out["vt_dy"] = (c, n)
         self.stats["overflow"] += fl
+        if self.edge_edge:
+            # edge boxes: min/max of the end points -+ padding (XPBD.py:217-232)
+            e = self.m["edges"]
+            emin = np.minimum(yf[e[:, 0]], yf[e[:, 1]]) - pad
+            emax = np.maximum(yf[e[:, 0]], yf[e[:, 1]]) + pad
+            if self.tree_st is not None:
+                c, n, fl = self.tree_st.query(emin, emax, self.cap)
+                out["ee_st"] = (c, n)
+                self.stats["overflow"] += fl
+            c, n, fl = self.tree_dy.query(emin, emax, self.cap)
+            out["ee_dy"] = (c, n)
+            self.stats["overflow"] += fl
         return out
 
     def forward(self, n_substeps, n_iters=1):
@@ -564,11 +581,19 @@ class OracleCloth:
                 if self.collision:
                     cands = self._broadphase()
                     xs = None if self.st is None else np.asarray(self.st["x"], self.dt_)
+                    ee = None
+                    if self.edge_edge:
+                        st = self.st
+                        ee = dict(edges_dy=self.m["edges"], fe_dy=self.m["fe"],
+                                  edges_st=None if st is None else st["edges"],
+                                  fe_st=None if st is None else st["fe"],
+                                  st=cands.get("ee_st"), dy=cands["ee_dy"], pcap=self.ee_pcap)
                     res = collision_pass(self.m, None if self.st is None else self.st["faces"], self.y, xs,
                                          self.fixed, self.m_inv, R(self.dhat), R(1e8), self.cap,
                                          cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"),
-                                         record=self.velocity)
+                                         record=self.velocity, ee=ee)
                     self.last_pairs = res.get("pairs")
+                    self.last_ee = res.get("ee")
                 if self.sewing is not None and len(self.sewing):
                     ksew = 0.5 * self.kb if self.sewing_stiffness is None else self.sewing_stiffness
                     sewing_pass(self.y, self.sewing, self.fixed, self.m_inv, R(ksew * dt_sub * dt_sub))
@@ -581,6 +606,7 @@ class OracleCloth:
             vs = None if st is None else np.ascontiguousarray(st.get("v", np.zeros_like(st["x"])), self.dt_)
             mst = None if st is None or "m_inv" not in st else np.asarray(st["m_inv"], self.dt_)
             velocity_pass(self.m, None if st is None else st["faces"], self.v, vs, self.fixed, self.m_inv, mst,
-                          R(np.float32(self.mu)), self.last_pairs if self.collision else None, self.cap)
+                          R(np.float32(self.mu)), self.last_pairs if self.collision else None, self.cap,
+                          ee=self.last_ee if self.collision and self.edge_edge else None)
             update_x(self.x, self.v, dts)
         return self.x

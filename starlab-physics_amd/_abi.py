@@ -134,7 +134,7 @@ class FrameDesc(C.Structure):
                 ("ws_bytes", C.c_size_t), ("velocity", I32), ("mu", F32), ("vs4", PTR), ("m_inv_st", PTR),
                 ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR), ("cache", CandCache * 3), ("cache_every", I32),
                 ("n_sewn", I32), ("sew_verts", PTR), ("sew_off", PTR), ("sew_list", PTR), ("sew_pairs", PTR),
-                ("k_sew", F32), ("sew_ws", PTR)]
+                ("k_sew", F32), ("sew_ws", PTR), ("ee", EdgeSets)]
 
 
 class SlxError(RuntimeError):

@@ -6,9 +6,10 @@
 //                 the first query of the dynamic tree; the static-tree queries
 //                 run on another side stream beside the dynamic-tree queries]
 //                spring pass with compute_y fused in
-//                per iteration: [spring pass] broad phase (3 query sets),
-//                               collision pass (the last one records the
-//                               contacting pairs when the velocity pass is on)
+//                per iteration: [spring pass] broad phase (3 query sets, and
+//                               the edge sets when on), collision pass (the
+//                               last one records the contacting pairs when the
+//                               velocity pass is on)
 //                compute_velocity + update_x, or with enable_velocity_update
 //                compute_velocity, velocity pass, update_x (XPBD.py:465-469)
 //  without collision (and without the velocity pass) the substeps run as two
@@ -168,9 +169,34 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(d->ws_bytes >= slx_frame_ws_bytes(d), "slx_frame: workspace too small");
   SLX_REQUIRE(!vel || !coll || (d->pair_cnt && d->pair_rec && d->pair_cap >= 0),
               "slx_frame: the velocity pass needs pair records");
+  const bool ee = coll && d->ee.ne > 0;
+  if (ee) {
+    SLX_REQUIRE(!batched(d->dyn), "slx_frame: edge-edge constraints need a single (not batched) dynamic mesh");
+    SLX_REQUIRE(d->ee.c_dy && d->ee.n_dy && (d->nvs == 0 || (d->ee.c_st && d->ee.n_st)),
+                "slx_frame: edge-edge constraints need their candidate buffers");
+    SLX_REQUIRE(!vel || (d->ee.pair_cnt && d->ee.pair_rec), "slx_frame: the velocity pass needs edge pair records");
+  }
   const int nv = d->nv, nvs = coll ? d->nvs : 0, cap = d->cap;
   const float pad = d->padding;
   int rc;
+  // the edge sets of a pass: static set only with a static mesh; pair records
+  // only on the recording pass
+  slx_edge_sets ee_rec = d->ee, ee_norec = d->ee;
+  if (ee) {
+    if (nvs == 0) ee_rec.c_st = ee_norec.c_st = ee_rec.n_st = ee_norec.n_st = nullptr;
+    ee_norec.pair_cnt = nullptr;
+    ee_norec.pair_rec = nullptr;
+    if (!vel) ee_rec = ee_norec;
+  }
+  // edge boxes of the current y against a tree (XPBD.py:217-232)
+  auto ee_query = [&](const slx_tree& t, const char* name, const float* y, const int32_t* c, const int32_t* n,
+                      hipStream_t qs) {
+    int nl = t.nf <= 0 ? 0 : (t.mode == 0 ? kCells : t.nf);
+    return traverse_launch(name, t.mode, d->ee.ne, y, pad, nl, t.num, t.sorted_faces, t.nlo4,
+                           t.nhi4, t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, d->ee.cap,
+                           (int32_t*)c, (int32_t*)n, d->status, qs, nullptr, nullptr, nullptr, nullptr,
+                           d->ee.edges_dy);
+  };
   // per-frame rebuild from x (XPBD.py:448-458)
   if (coll) {
     rc = build_tree(d->dyn, nv, d->x4, d->faces, pad, d->status, d->ws, d->ws_bytes, stream);
@@ -268,6 +294,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
                    n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
         if (rc) return rc;
+        if (ee) {
+          rc = ee_query(d->st, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q);
+          if (rc) return rc;
+        }
         if (hipEventRecord(side->qjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
         static_pending = true;
       }
@@ -282,14 +312,19 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         if (rc) return rc;
         tv_fresh = true;
       }
+      if (ee) {
+        rc = ee_query(d->dyn, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st);
+        if (rc) return rc;
+      }
       if (static_pending && hipStreamWaitEvent(st, side->qjoin, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: join failed");
-      // narrow phase + projection + update_dx (XPBD.py:236-264, 365-369)
+      // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369)
       const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
-      rc = slx_collision_pass(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
-                              d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
-                              rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
-                              d->rec_cap, d->status, d->ws, d->ws_bytes, stream);
+      rc = slx_collision_pass_ee(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
+                                 d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
+                                 rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
+                                 ee ? (rec ? &ee_rec : &ee_norec) : nullptr, d->rec_cap, d->status, d->ws,
+                                 d->ws_bytes, stream);
       if (rc) return rc;
       if (sew) {                             // sewing constraints (XPBD.py:371-372, there commented out)
         rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,
@@ -302,9 +337,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     if (vel) {
       rc = slx_compute_velocity(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
       if (rc) return rc;
-      rc = slx_velocity_pass(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st, d->mu,
-                             d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec, d->x4, d->dt,
-                             d->rec_cap > 0 ? d->rec_cap : 1, d->status, d->ws, d->ws_bytes, stream);
+      rc = slx_velocity_pass_ee(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st,
+                                d->mu, d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec,
+                                ee ? &ee_rec : nullptr, d->x4, d->dt, d->rec_cap > 0 ? d->rec_cap : 1, d->status,
+                                d->ws, d->ws_bytes, stream);
     } else {
       rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
     }

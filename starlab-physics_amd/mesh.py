@@ -149,7 +149,7 @@ class Mesh:
         fixed = self.fixed.cpu().numpy() if self.device is not None else np.ones(self.num_verts, np.float32)
         return dict(x=x.astype(np.float32), x0=self.x0_np, faces=self.fid_np, edges=self.eid_np,
                     l0=self.l0_np, bend=self.bend_np, bl0=self.bending_l0_np, fixed=fixed,
-                    m_inv=self.m_inv_np)
+                    m_inv=self.m_inv_np, fe=_topo.face_edges(self.fid_np, self.eid_np))
 
     @classmethod
     def from_trimesh(cls, tm, is_static=False, device="cuda", **kw):

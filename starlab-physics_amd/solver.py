@@ -27,6 +27,15 @@ Extensions (all default to the reference's behaviour):
                                    the reference leaves commented out), stiffness
                                    sewing_stiffness (None = 0.5 stiffness_bending,
                                    the reference's "0.5 YM")
+  enable_edge_edge (False)         edge-edge constraints (XPBD.py:217-232, 266-281,
+                                   313-330, commented out in the reference): each
+                                   collision pass also queries the dynamic edges'
+                                   boxes and applies __ee_st / __ee_dy after the
+                                   vertex-triangle constraints; with the velocity
+                                   pass, their velocity-level counterparts
+  ee_pair_cache_size (None)        edge pairs kept per edge and set for the
+                                   velocity pass (None = cache_size, the
+                                   reference's 40-wide fields)
   status()                         device status word (overflow flags)
 `step` is an alias of `forward`.
 """
@@ -81,6 +90,8 @@ class Solver:
         self.enable_sewing = False
         self.sewing_stiffness = None
         self._sew = None
+        self.enable_edge_edge = False
+        self.ee_pair_cache_size = None
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
@@ -202,6 +213,45 @@ class Solver:
                         CandidateCache(nv, ccap, margin, self.device),
                         CandidateCache(nvs, ccap, margin, self.device) if s is not None else None]
 
+    def _edge_buffers(self, cd):
+        """Edge-edge candidate lists, counts and pair records (allocated once the
+        constraints are enabled)."""
+        import torch
+        m, s = self.mesh_dy, self.mesh_st
+        ne, cap = m.num_edges, self.cache_size
+        pcap = cap if self.ee_pair_cache_size is None else int(self.ee_pair_cache_size)
+        key = (cap, pcap, self.enable_velocity_update)
+        if cd.get("ee_key") == key:
+            return cd["ee"]
+        i32 = dict(dtype=torch.int32, device=self.device)
+        e = dict(ne=ne, cap=cap, pcap=pcap, fe_dy=m.face_edge_indices(),
+                 fe_st=s.face_edge_indices() if s is not None else None,
+                 c_st=torch.full((max(cap * ne, 1),), -1, **i32) if s is not None else None,
+                 n_st=torch.zeros(max(ne, 1), **i32) if s is not None else None,
+                 c_dy=torch.full((max(cap * ne, 1),), -1, **i32), n_dy=torch.zeros(max(ne, 1), **i32),
+                 pair_num=torch.zeros(max(2 * ne, 1), **i32))
+        if self.enable_velocity_update:
+            e["pair_cnt"] = torch.zeros(max(2 * ne, 1), **i32)
+            e["pair_rec"] = torch.zeros(max(16 * pcap * 2 * ne, 16), dtype=torch.float32, device=self.device)
+        cd["ee"], cd["ee_key"] = e, key
+        return e
+
+    def edge_pairs(self):
+        """Last edge-edge pair counts (uncapped) and, with the velocity pass, the
+        recorded pairs: dict with num [2ne] (static set, then dynamic set per
+        edge), cnt [2ne], g [2ne, pcap, 4, 3], schur, edge (the other edge's id),
+        type."""
+        import torch
+        e = self._cd["ee"]
+        ne, pc = e["ne"], e["pcap"]
+        out = dict(num=e["pair_num"][:2 * ne])
+        if "pair_cnt" in e:
+            r = e["pair_rec"][:16 * pc * 2 * ne].view(pc, 4, 2 * ne, 4).permute(2, 0, 1, 3)
+            out.update(cnt=e["pair_cnt"][:2 * ne], g=r[..., :3], schur=r[:, :, 0, 3],
+                       edge=r[:, :, 1, 3].contiguous().view(torch.int32),
+                       type=r[:, :, 2, 3].contiguous().view(torch.int32))
+        return out
+
     def _pair_buffers(self, cd, pcap):
         """Pair records for the velocity pass (allocated once it is enabled)."""
         import torch
@@ -259,6 +309,14 @@ class Solver:
                 d.pair_cap, d.pair_cnt, d.pair_rec = cd["pair_cap"], P(cd["pair_cnt"]), P(cd["pair_rec"])
                 if s is not None:
                     d.vs4, d.m_inv_st = P(s.v4), P(s.m_inv)
+            if self.enable_edge_edge and m.num_edges > 0:
+                if cd["dyn"].n_inst > 1:
+                    raise ValueError("edge-edge constraints need a single (not batched) dynamic mesh")
+                e = self._edge_buffers(cd)
+                d.ee = _abi.EdgeSets(e["ne"], P(m.edge_indices), P(s.edge_indices) if s is not None else None,
+                                     P(e["fe_dy"]), P(e["fe_st"]), e["cap"], P(e["c_st"]), P(e["n_st"]),
+                                     P(e["c_dy"]), P(e["n_dy"]), e["pcap"], P(e["pair_num"]), P(e.get("pair_cnt")),
+                                     P(e.get("pair_rec")))
         if self.enable_sewing and self._sew is not None and self._sew["np"] > 0:
             sw = self._sew
             dt_sub = self.dt / n_substeps

@@ -233,3 +233,57 @@ def test_velocity_pass_ee_bit_exact(slx, orc):
     got = r["cloth"].v4[:, :3].cpu().numpy()
     assert np.any(got != r["v"])
     np.testing.assert_array_equal(got, r["vo"])
+
+
+def ee_frame_scene(slx, mode, velocity):
+    """ee_scene's meshes 0.11 apart (just outside sqrt(dhat)) under gravity, two
+    corners of the lower grid pinned, the edge-edge constraints on: the lower
+    grid sags onto the static one and the upper grid onto it; the oracle driver
+    on the same inputs."""
+    import torch
+    from oracle import oracle as orc
+    cloth, obst = ee_scene(slx, n=12, gap=0.11)
+    sim = slx.Solver(cloth, obst, dHat=DHAT, stiffness_stretch=1e4, stiffness_bending=1e2,
+                     g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+    sim.enable_collision_handling = True
+    sim.enable_edge_edge = True
+    sim.enable_velocity_update = velocity
+    sim.bvh_mode = mode
+    sim.padding = 0.06
+    sim.records_per_vertex = 128
+    sel = torch.zeros(cloth.num_verts, device="cuda")
+    sel[0] = 1
+    sel[11] = 1
+    sim.set_fixed_vertices(sel)
+    st = dict(x=obst.positions(), faces=obst.fid_np, m_inv=obst.m_inv_np, edges=obst.eid_np,
+              fe=slx.topology.face_edges(obst.fid_np, obst.eid_np))
+    ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
+                          stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
+                          collision=True, bvh=mode, cap=sim.cache_size, velocity=velocity, mu=sim.mu,
+                          edge_edge=True)
+    return cloth, sim, ref
+
+
+@pytest.mark.parametrize("mode,velocity", [("tri", False), ("tri", True), ("cell", True)])
+def test_frames_with_edge_edge_bit_exact(slx, mode, velocity):
+    cloth, sim, ref = ee_frame_scene(slx, mode, velocity)
+    pairs = 0
+    for frame in range(6):
+        print(f"[ee frame] mode={mode} velocity={velocity} frame={frame}", flush=True)
+        before = ref.stats["overflow"]
+        sim.clear_status()
+        sim.forward(4)
+        ref.forward(4)
+        st = sim.status()
+        assert st & ~1 == 0, (frame, st)
+        assert (st & 1 != 0) == (ref.stats["overflow"] > before), (frame, st)
+        np.testing.assert_array_equal(cloth.positions(), ref.x)
+        e = sim.edge_pairs()
+        num = e["num"].cpu().numpy()
+        ne = cloth.num_edges
+        np.testing.assert_array_equal(num[:ne], ref.last_ee["pn_st"])
+        np.testing.assert_array_equal(num[ne:], ref.last_ee["pn_dy"])
+        pairs += int(num.sum())
+    assert pairs > 50, "the layers must touch"
+    if velocity:
+        np.testing.assert_array_equal(cloth.v4[:, :3].cpu().numpy(), ref.v.astype(np.float32))
