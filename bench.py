@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-friction", action="store_true",
                    help="turn the velocity-level contact and friction pass off (enable_velocity_update False); "
                         "the frictionless drape slides into the terrain's valleys and bunches up within ~10 frames")
+    p.add_argument("--edge-edge", action="store_true",
+                   help="turn the edge-edge constraints on (enable_edge_edge; commented out in the reference)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--copy-probe", action="store_true",
@@ -105,11 +107,12 @@ def cpu_baseline(cloth, terr, sim, iters, seconds):
     velocity/position update, the velocity pass when enabled), repeated for
     about `seconds`."""
     from oracle import oracle as orc
-    st = None if terr is None else dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
+    st = None if terr is None else dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np,
+                                        edges=terr.eid_np, fe=terr.host_dict()["fe"])
     ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt / 20, padding=sim.padding,
                           collision=terr is not None, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
-                          velocity=sim.enable_velocity_update, mu=sim.mu)
+                          velocity=sim.enable_velocity_update, mu=sim.mu, edge_edge=sim.enable_edge_edge)
     t0 = time.perf_counter()
     reps = 0
     while True:
@@ -121,6 +124,8 @@ def cpu_baseline(cloth, terr, sim, iters, seconds):
     what = "with terrain + self collision" if terr is not None else "collision off"
     if sim.enable_velocity_update:
         what += ", velocity pass"
+    if sim.enable_edge_edge:
+        what += ", edge-edge constraints"
     return dict(value=cloth.num_verts * reps / el, unit="vertex-substeps/s", cores=1, kind="port",
                 sample=f"{reps} substeps x {iters} Jacobi iterations of the {cloth.num_verts}-vertex cloth ({what}) "
                        f"on the serial FP32 oracle, {el:.1f} s")
@@ -166,6 +171,7 @@ def main():
     a.velocity = collision and not a.no_friction
     sim.enable_velocity_update = a.velocity
     sim.candidate_cache = not a.no_cache
+    sim.enable_edge_edge = collision and a.edge_edge
     if a.walk_stack is not None:
         abi.call("slx_set_walk_stack", a.walk_stack)
     if a.cache_margin is not None:
@@ -209,6 +215,14 @@ def main():
         counts.update(Vs=terr.num_verts, Fs=terr.num_faces, cbar=cbar)
     if a.velocity and collision:
         counts.update(velocity_counts(sim, V))
+    ee_info = {}
+    if sim.enable_edge_edge:
+        e = sim._cd["ee"]
+        ce = {"ee_dy": float(e["n_dy"][:E].float().mean().item())}
+        if e["n_st"] is not None:
+            ce["ee_st"] = float(e["n_st"][:E].float().mean().item())
+        counts["ee_cbar"] = ce
+        ee_info = dict(mean_candidates=ce, contacting_pairs_last_pass=int(e["pair_num"][:2 * E].sum().item()))
     caches = (sim._cd or {}).get("caches") if collision else None
     if caches and (sim.cache_every != 1 or a.iters > 1):
         cs = {}
@@ -227,7 +241,8 @@ def main():
     # calibrated on a copy kernel), valid for the default workload
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
     default_run = (collision and a.velocity and not a.no_cache and a.cache_margin is None and a.cache_every is None
-                   and a.walk_stack is None and a.n == 1024 and a.substeps == 20 and a.iters == 2)
+                   and a.walk_stack is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
+                   and not a.edge_edge)
     if default_run and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
         if tk:
@@ -238,6 +253,8 @@ def main():
             "rebuilt per frame and refit per substep" if collision else "collision off")
     if a.velocity:
         cfg += ", velocity-level contact and friction pass (mu 0.8) so the drape rests"
+    if sim.enable_edge_edge:
+        cfg += ", edge-edge constraints (edge broad phase over both trees, static and dynamic sets)"
     out = {
         "metric": "vertex-substeps/s",
         "value": value,
@@ -252,7 +269,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded grid cloth and sinusoid terrain)",
         "config": {"workload": cfg, "substeps_per_step": a.substeps, "iters": a.iters, "collision": collision,
-                   "velocity_pass": a.velocity, "candidate_cache": sim.candidate_cache,
+                   "velocity_pass": a.velocity, "edge_edge": sim.enable_edge_edge,
+                   "candidate_cache": sim.candidate_cache,
                    "cache_margin_m": 0.25 * sim.padding if sim.cache_margin is None else sim.cache_margin,
                    "cache_every": sim.cache_every,
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
@@ -265,6 +283,7 @@ def main():
         "mean_candidates": cbar,
         "mean_cached_leaves": counts.get("cache", {}),
         "velocity_pairs": {k: counts[k] for k in ("pairs", "vel_records") if k in counts},
+        "edge_edge": ee_info,
         "kernels_ms_profile_frame": {k: {"total_ms": round(v[0], 4), "launches": v[1],
                                          "avg_us": round(v[0] / max(v[1], 1) * 1e3, 3)}
                                      for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},

@@ -329,6 +329,12 @@ typedef struct slx_edge_sets {
   int32_t* pair_num;        /* [2 ne] */
   int32_t* pair_cnt;        /* [2 ne] */
   float* pair_rec;          /* float4 [pair_cap][4][2 ne] */
+  const int32_t* order;     /* nullable: [ne] order in which slx_frame walks the dynamic tree for the
+                               edges (a spatially coherent one lets a wavefront's queries share tree
+                               nodes); results do not depend on it */
+  int32_t* redo;            /* nullable: [2 (1 + ne)] lists of the edges a shallow-stack walk could
+                               not serve (static tree, dynamic tree), for slx_frame's two-tier
+                               edge walk of triangle trees */
 } slx_edge_sets;
 
 /* slx_collision_pass with the edge-edge constraints after the vertex-triangle

@@ -117,7 +117,8 @@ __device__ __forceinline__ void push_rec(int t, float3 d, uint32_t key, int buck
 }
 
 // One thread per (set, edge): the sides of its candidate faces in order
-// (collision_constraints_x.py:312-507).
+// (collision_constraints_x.py:312-507).  (Threads take the edges in index order:
+// the candidate lists are read coalesced.)
 __global__ void __launch_bounds__(256)
 k_ee_contacts(EdgeArgs A, const float4* __restrict__ y4, const float4* __restrict__ xs4,
               const float* __restrict__ fixed, const float* __restrict__ m_inv, float dhat, float k, int bucket,

@@ -188,10 +188,19 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     ee_norec.pair_rec = nullptr;
     if (!vel) ee_rec = ee_norec;
   }
-  // edge boxes of the current y against a tree (XPBD.py:217-232)
-  auto ee_query = [&](const slx_tree& t, const char* name, const float* y, const int32_t* c, const int32_t* n,
-                      hipStream_t qs) {
+  // edge boxes of the current y against a tree (XPBD.py:217-232); triangle trees
+  // take the two-tier walk when the caller gave redo lists (set 0 static, 1 dynamic)
+  auto ee_query = [&](const slx_tree& t, int set, const char* name, const float* y, const int32_t* c,
+                      const int32_t* n, hipStream_t qs) {
     int nl = t.nf <= 0 ? 0 : (t.mode == 0 ? kCells : t.nf);
+    if (t.mode == 1 && d->ee.redo && nl > 0) {
+      slx_cand_cache walk{};
+      walk.redo = d->ee.redo + (size_t)set * (1 + (size_t)d->ee.ne);
+      // the coherent order pays for its scattered list writes on the deep dynamic tree only
+      return traverse_cached(name, 0, d->ee.ne, y, pad, nl, t.nlo4, t.nhi4, t.quad4, &walk, nullptr, d->ee.cap,
+                             (int32_t*)c, (int32_t*)n, d->status, qs, set == 1 ? d->ee.order : nullptr,
+                             d->ee.edges_dy);
+    }
     return traverse_launch(name, t.mode, d->ee.ne, y, pad, nl, t.num, t.sorted_faces, t.nlo4,
                            t.nhi4, t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, d->ee.cap,
                            (int32_t*)c, (int32_t*)n, d->status, qs, nullptr, nullptr, nullptr, nullptr,
@@ -295,7 +304,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                    n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
         if (rc) return rc;
         if (ee) {
-          rc = ee_query(d->st, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q);
+          rc = ee_query(d->st, 0, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q);
           if (rc) return rc;
         }
         if (hipEventRecord(side->qjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
@@ -313,7 +322,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         tv_fresh = true;
       }
       if (ee) {
-        rc = ee_query(d->dyn, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st);
+        rc = ee_query(d->dyn, 1, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st);
         if (rc) return rc;
       }
       if (static_pending && hipStreamWaitEvent(st, side->qjoin, 0) != hipSuccess)

@@ -125,7 +125,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order);
+                    const int32_t* order, const int32_t* qedges = nullptr);
 
 // edge-edge phase (edges.hip): argument checks; the vertex-triangle bucket
 // records folded into self_acc, then the edge-edge deltas into the emptied buckets

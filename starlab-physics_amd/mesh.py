@@ -101,6 +101,17 @@ class Mesh:
             self._face_edges = fe
         return fe
 
+    def edge_order(self):
+        """Edges in a spatially coherent order (by the Morton rank of their first
+        vertex), [E] int32 on the device, built on first use."""
+        eo = getattr(self, "_edge_order", None)
+        if eo is None:
+            import torch
+            vo = self.vert_order.cpu().numpy()
+            eo = torch.from_numpy(_topo.edge_order(self.eid_np, vo)).to(self.torch_device)
+            self._edge_order = eo
+        return eo
+
     def _refresh_products(self):
         """x4.w = y4.w = fixed*m_inv, v4.w = fixed (the products the kernels read)."""
         fw = self.fixed * self.m_inv

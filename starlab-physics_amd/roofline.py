@@ -36,6 +36,17 @@ pair, an upper bound):
                                (counts, records + face ids, velocities, f and w,
                                 first-set sums, bucket deltas + counters)
   velocity apply               92 V + 16 R   (counts, sums, f, w; read/write v and x; buckets)
+
+Edge-edge constraints (enable_edge_edge), ne dynamic edges, ce[set] mean
+candidate faces per edge, Q edge pairs in contact (the position pass's
+count), Re bucket deltas of the edge sets:
+  edge traversal (per tree)    44 ne + 4 ce ne + tree   (edge ids 8, end points 32, count 4;
+                               list; the tree as for the vertex queries)
+  edge-edge contacts (per set) (48 + 40 ce) ne + 20 Re  (count, own ids and end points, pair
+                               count; per candidate face its id, its sides' ids and their
+                               end-point ids; bucket deltas + counters)
+  bucket fold                  40 V + 16 Rv   (counts, running sums read and written; the
+                               vertex-triangle records, Rv of them, read)
 """
 from __future__ import annotations
 
@@ -99,6 +110,19 @@ def vel_apply(V, R):
     return 92 * V + 16 * R
 
 
+def edge_traversal(ne, Ft, ce, quad=True):
+    tree = 128 * max(Ft - 1, 0) + 32 if quad else 32 * (2 * Ft - 1)
+    return 44 * ne + 4 * ce * ne + tree
+
+
+def ee_contacts(ne, ce, Re=0):
+    return (48 + 40 * ce) * ne + 20 * Re
+
+
+def bucket_fold(V, Rv=0):
+    return 40 * V + 16 * Rv
+
+
 def kernel_bytes(name: str, n: dict):
     """Algorithmic bytes of one launch of kernel `name` (None if not modelled)."""
     V, E, B, F = n["V"], n["E"], n["B"], n["F"]
@@ -136,6 +160,13 @@ def kernel_bytes(name: str, n: dict):
         table["compute_velocity"] = compute_velocity(V)
         table["vel_contacts"] = vel_contacts(nq, n["pairs"], V, Vs, n["vel_records"])
         table["vel_apply"] = vel_apply(V, n["vel_records"])
+    if "ee_cbar" in n:
+        ce = n["ee_cbar"]
+        table["traverse_ee_dy"] = edge_traversal(E, F, ce.get("ee_dy", 0.0))
+        table["ee_contacts"] = sum(ee_contacts(E, ce[k]) for k in ("ee_st", "ee_dy") if k in ce)
+        table["bucket_fold"] = bucket_fold(V)
+        if "Fs" in n:
+            table["traverse_ee_st"] = edge_traversal(E, n["Fs"], ce.get("ee_st", 0.0), quad=False)
     return table.get(name)
 
 
@@ -168,6 +199,13 @@ def step_bytes(n: dict, n_iters: int, substeps: int, collision: bool) -> float:
         R = n["vel_records"]
         total += substeps * (pair_records(n["pairs"], nq) + compute_velocity(V) +
                              vel_contacts(nq, n["pairs"], V, Vs, R) + vel_apply(V, R) - update_vx(V))
+    if collision and "ee_cbar" in n:
+        # edge-edge constraints: per pass the edge queries of both trees, the two sets, the fold
+        ce = n["ee_cbar"]
+        per = edge_traversal(E, F, ce.get("ee_dy", 0.0)) + ee_contacts(E, ce.get("ee_dy", 0.0)) + bucket_fold(V)
+        if "Fs" in n:
+            per += edge_traversal(E, n["Fs"], ce.get("ee_st", 0.0), quad=False) + ee_contacts(E, ce.get("ee_st", 0.0))
+        total += substeps * n_iters * per
     return total
 
 

@@ -220,7 +220,7 @@ class Solver:
         m, s = self.mesh_dy, self.mesh_st
         ne, cap = m.num_edges, self.cache_size
         pcap = cap if self.ee_pair_cache_size is None else int(self.ee_pair_cache_size)
-        key = (cap, pcap, self.enable_velocity_update)
+        key = (cap, pcap, self.enable_velocity_update, self.bvh_mode)
         if cd.get("ee_key") == key:
             return cd["ee"]
         i32 = dict(dtype=torch.int32, device=self.device)
@@ -229,7 +229,8 @@ class Solver:
                  c_st=torch.full((max(cap * ne, 1),), -1, **i32) if s is not None else None,
                  n_st=torch.zeros(max(ne, 1), **i32) if s is not None else None,
                  c_dy=torch.full((max(cap * ne, 1),), -1, **i32), n_dy=torch.zeros(max(ne, 1), **i32),
-                 pair_num=torch.zeros(max(2 * ne, 1), **i32))
+                 pair_num=torch.zeros(max(2 * ne, 1), **i32), order=m.edge_order(),
+                 redo=torch.zeros(2 * (1 + ne), **i32) if self.bvh_mode == "tri" else None)
         if self.enable_velocity_update:
             e["pair_cnt"] = torch.zeros(max(2 * ne, 1), **i32)
             e["pair_rec"] = torch.zeros(max(16 * pcap * 2 * ne, 16), dtype=torch.float32, device=self.device)
@@ -316,7 +317,7 @@ class Solver:
                 d.ee = _abi.EdgeSets(e["ne"], P(m.edge_indices), P(s.edge_indices) if s is not None else None,
                                      P(e["fe_dy"]), P(e["fe_st"]), e["cap"], P(e["c_st"]), P(e["n_st"]),
                                      P(e["c_dy"]), P(e["n_dy"]), e["pcap"], P(e["pair_num"]), P(e.get("pair_cnt")),
-                                     P(e.get("pair_rec")))
+                                     P(e.get("pair_rec")), P(e["order"]), P(e["redo"]))
         if self.enable_sewing and self._sew is not None and self._sew["np"] > 0:
             sw = self._sew
             dt_sub = self.dt / n_substeps

@@ -56,6 +56,15 @@ def face_edges(faces: np.ndarray, edges: np.ndarray) -> np.ndarray:
     return at.astype(np.int32)
 
 
+def edge_order(edges: np.ndarray, vert_order: np.ndarray) -> np.ndarray:
+    """A spatially coherent processing order of the edges: by the position of
+    their first vertex in vert_order (a Morton order of the vertices), stable."""
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    rank = np.empty(len(vert_order), np.int64)
+    rank[np.asarray(vert_order, np.int64)] = np.arange(len(vert_order))
+    return np.argsort(rank[e[:, 0]], kind="stable").astype(np.int32)
+
+
 def bending_pairs(faces: np.ndarray) -> np.ndarray:
     """Opposite-vertex pairs across interior edges.
 

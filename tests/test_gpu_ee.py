@@ -235,25 +235,25 @@ def test_velocity_pass_ee_bit_exact(slx, orc):
     np.testing.assert_array_equal(got, r["vo"])
 
 
-def ee_frame_scene(slx, mode, velocity):
+def ee_frame_scene(slx, mode, velocity, padding=0.06, n=12):
     """ee_scene's meshes 0.11 apart (just outside sqrt(dhat)) under gravity, two
     corners of the lower grid pinned, the edge-edge constraints on: the lower
     grid sags onto the static one and the upper grid onto it; the oracle driver
     on the same inputs."""
     import torch
     from oracle import oracle as orc
-    cloth, obst = ee_scene(slx, n=12, gap=0.11)
+    cloth, obst = ee_scene(slx, n=n, gap=0.11)
     sim = slx.Solver(cloth, obst, dHat=DHAT, stiffness_stretch=1e4, stiffness_bending=1e2,
                      g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
     sim.enable_collision_handling = True
     sim.enable_edge_edge = True
     sim.enable_velocity_update = velocity
     sim.bvh_mode = mode
-    sim.padding = 0.06
+    sim.padding = padding
     sim.records_per_vertex = 128
     sel = torch.zeros(cloth.num_verts, device="cuda")
     sel[0] = 1
-    sel[11] = 1
+    sel[n - 1] = 1
     sim.set_fixed_vertices(sel)
     st = dict(x=obst.positions(), faces=obst.fid_np, m_inv=obst.m_inv_np, edges=obst.eid_np,
               fe=slx.topology.face_edges(obst.fid_np, obst.eid_np))
@@ -287,3 +287,23 @@ def test_frames_with_edge_edge_bit_exact(slx, mode, velocity):
     assert pairs > 50, "the layers must touch"
     if velocity:
         np.testing.assert_array_equal(cloth.v4[:, :3].cpu().numpy(), ref.v.astype(np.float32))
+
+
+def test_frames_with_edge_edge_shallow_walk(slx):
+    """A 16-entry first-tier stack, a deeper tree (two 40 x 40 grids) and large
+    boxes (padding 0.3): edge queries the first tier cannot serve go through the
+    redo list and the full walk; the frames stay bit-exact."""
+    A = abi()
+    cloth, sim, ref = ee_frame_scene(slx, "tri", True, padding=0.3, n=40)
+    A.call("slx_set_walk_stack", 16)
+    try:
+        for frame in range(3):
+            print(f"[ee frame shallow] frame={frame}", flush=True)
+            sim.forward(4)
+            ref.forward(4)
+            np.testing.assert_array_equal(cloth.positions(), ref.x)
+        redo = sim._cd["ee"]["redo"].cpu().numpy()
+        ne = cloth.num_edges
+        assert redo[1 + ne] > 0, "the shallow stack must overflow for some edge query"
+    finally:
+        A.call("slx_set_walk_stack", 32)
