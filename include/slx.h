@@ -210,12 +210,17 @@ int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves,
  * same traversal for ne edge boxes, min/max of x4[edges[2e]], x4[edges[2e+1]]
  * -+ padding.  Arguments otherwise as slx_traverse (q = edge).  A dynamic tree is
  * queried with its own grid (the reference's commented query of lbvh_dy passes
- * the static tree's cell size and origin). */
+ * the static tree's cell size and origin).  order (nullable, [ne]): the order in
+ * which edges are walked; redo (nullable, [1 + ne], triangle trees): the two-tier
+ * walk (slx_set_walk_stack entries first, the edges that overflow it listed in
+ * redo and walked again with the full stack; redo[0] = their count).  The lists
+ * do not depend on either. */
 int slx_traverse_edges(int mode, int ne, const int32_t* edges, const float* x4, float padding,
                        int n_leaves, const int32_t* num, const int32_t* sorted_faces,
                        const float* nlo4, const float* nhi4, const float* quad4, const float* fmin4,
                        const float* fmax4, const float* box6, int cap, int32_t* cand,
-                       int32_t* cand_n, int32_t* status, slx_stream_t stream);
+                       int32_t* cand_n, int32_t* status, const int32_t* order, int32_t* redo,
+                       slx_stream_t stream);
 
 /* Batched instances (no reference counterpart beyond the mesh concatenation of
  * framework/meshio/concat.py:11-58): instances packed CSR-style (vert_base,

@@ -54,7 +54,7 @@ SIGNATURES = {
     "slx_tri_refit": [I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, STREAM],
     "slx_traverse": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, STREAM],
     "slx_traverse_edges": [I32, I32, PTR, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR,
-                           PTR, STREAM],
+                           PTR, PTR, PTR, STREAM],
     "slx_traverse_cached": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, PTR, STREAM],
     "slx_set_walk_stack": [I32],
     "slx_tree_snapshot": [I32, PTR, PTR, PTR, STREAM],

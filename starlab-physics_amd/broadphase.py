@@ -216,10 +216,11 @@ class DeviceTree:
                   P(st), _abi.stream_handle())
         return cand, cand_n
 
-    def query_edges(self, edges, x4, padding, cap, cand=None, cand_n=None, status=None):
+    def query_edges(self, edges, x4, padding, cap, cand=None, cand_n=None, status=None, order=None, redo=None):
         """Candidate faces of the edge boxes min/max(x4[e0], x4[e1]) -+ padding
         (edges: [ne*2] int32 device), column-major (cap, ne) -- the edge broad
-        phase of XPBD.py:217-232."""
+        phase of XPBD.py:217-232.  order: walk order of the edges; redo ([1 + ne]
+        int32, triangle trees): the two-tier walk.  Same lists either way."""
         ne = edges.numel() // 2
         dev = x4.device
         if self.n_inst > 1:
@@ -233,5 +234,5 @@ class DeviceTree:
         _abi.call("slx_traverse_edges", 0 if self.mode == "cell" else 1, ne, P(edges), P(x4), float(padding),
                   self.n_leaves if self.nf > 0 else 0, P(self.num), P(self.sorted_faces), P(self.nlo4),
                   P(self.nhi4), P(self.quad4), P(self.fmin4), P(self.fmax4), P(self.box6), cap, P(cand), P(cand_n),
-                  P(st), _abi.stream_handle())
+                  P(st), P(order), P(redo), _abi.stream_handle())
         return cand, cand_n

@@ -96,6 +96,7 @@ bool cacheable(const slx_tree& t, const slx_cand_cache& c) { return tiered(t, c)
 // cache (nullable): the set's cache struct; use_cache: filled when *built is
 // false, then filtered; otherwise only its redo list serves the two-tier walk.
 // disp: the tree's displacement since the cache was built (nullable = 0)
+// (the frame keeps every redo counter at zero between passes: prezeroed walks)
 int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst,
           const int32_t* q_order, float pad, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
           slx_stream_t s, const slx_cand_cache* cache = nullptr, bool use_cache = false, bool* built = nullptr,
@@ -110,7 +111,7 @@ int query(const char* name, const slx_tree& t, int nq, const float* q4, const in
       *built = true;
     }
     return traverse_cached(name, kind, nq, q4, pad, nl, t.nlo4, t.nhi4, t.quad4, cache, disp, cap, cand, cand_n,
-                           status, (hipStream_t)s, q_order);
+                           status, (hipStream_t)s, q_order, nullptr, true);
   }
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
                          t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,
@@ -199,7 +200,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       // the coherent order pays for its scattered list writes on the deep dynamic tree only
       return traverse_cached(name, 0, d->ee.ne, y, pad, nl, t.nlo4, t.nhi4, t.quad4, &walk, nullptr, d->ee.cap,
                              (int32_t*)c, (int32_t*)n, d->status, qs, set == 1 ? d->ee.order : nullptr,
-                             d->ee.edges_dy);
+                             d->ee.edges_dy, true);
     }
     return traverse_launch(name, t.mode, d->ee.ne, y, pad, nl, t.num, t.sorted_faces, t.nlo4,
                            t.nhi4, t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, d->ee.cap,
@@ -234,19 +235,34 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
   SideStream* side = nullptr;
   const bool refits = coll && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0 && d->n_substeps > 1;
+  // The buckets' counters and the walks' redo counters are zeroed once here (the
+  // tree builds above reuse the workspace); from then on every apply kernel leaves
+  // them at zero for the next pass, so no pass needs a zeroing launch of its own.
+  ZeroList after{};
   if (coll) {
     rc = side_stream(&side);
     if (rc) return rc;
+    auto add = [&](int32_t* p) {
+      if (p) after.p[after.n++] = p;
+    };
+    for (int k = 0; k < 3; ++k) add(d->cache[k].redo);
+    if (ee && d->ee.redo) {
+      add(d->ee.redo);
+      add(d->ee.redo + 1 + (size_t)d->ee.ne);
+    }
+    rc = frame_zero(nv, d->rec_cap, d->ws, after, st);
+    if (rc) return rc;
   }
   for (int s = 0; s < d->n_substeps; ++s) {
-    bool refit_pending = false;
+    bool refit_pending = false, snapped = false;
     if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
       // caches rebuilt in this substep, against x as it is now (the refit below
       // builds the dynamic tree's boxes from this x)
       built[0] = built[1] = built[2] = false;
       if (dyn_cached) {
-        rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);
+        rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);   // (zeroes disp)
         if (rc) return rc;
+        snapped = true;
       }
     }
     if (refits && s > 0) {
@@ -258,7 +274,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                          (slx_stream_t)side->s);
       if (rc) return rc;
       if (dyn_cached) {                      // how far the refit boxes' vertices moved since the caches
-        rc = slx_tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, (slx_stream_t)side->s);
+        rc = tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, !snapped, side->s);
         if (rc) return rc;
       }
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
@@ -329,11 +345,11 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         return fail(SLX_EHIP, "slx_frame: join failed");
       // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369)
       const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
-      rc = slx_collision_pass_ee(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
-                                 d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
-                                 rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
-                                 ee ? (rec ? &ee_rec : &ee_norec) : nullptr, d->rec_cap, d->status, d->ws,
-                                 d->ws_bytes, stream);
+      rc = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
+                               d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
+                               rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
+                               ee ? (rec ? &ee_rec : &ee_norec) : nullptr, d->rec_cap, d->status, d->ws,
+                               d->ws_bytes, st, true, after);
       if (rc) return rc;
       if (sew) {                             // sewing constraints (XPBD.py:371-372, there commented out)
         rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,
@@ -346,10 +362,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     if (vel) {
       rc = slx_compute_velocity(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
       if (rc) return rc;
-      rc = slx_velocity_pass_ee(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st,
-                                d->mu, d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec,
-                                ee ? &ee_rec : nullptr, d->x4, d->dt, d->rec_cap > 0 ? d->rec_cap : 1, d->status,
-                                d->ws, d->ws_bytes, stream);
+      rc = velocity_pass_impl(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st,
+                              d->mu, d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec,
+                              ee ? &ee_rec : nullptr, d->x4, d->dt, d->rec_cap > 0 ? d->rec_cap : 1, d->status,
+                              d->ws, d->ws_bytes, st, coll);
     } else {
       rc = slx_update_vx(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
     }

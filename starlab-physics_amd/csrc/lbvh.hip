@@ -264,14 +264,33 @@ struct Sched {
   __device__ int top_node(int s) const { return p[8 + 5 * (size_t)n + s]; }
   __device__ int top_up(int s) const { return p[8 + 6 * (size_t)n + s]; }
   __device__ int32_t* gflag(int s) const { return const_cast<int32_t*>(p) + 8 + 7 * (size_t)n + s; }
+  // 4-wide lines of even-depth nodes only (single trees: walks start at node 0 and
+  // pop only grandchildren, so odd-depth lines are never read)
+  __device__ bool even_lines() const { return p[4] != 0; }
+  __device__ int sub_parity(int i) const { return p[8 + 8 * (size_t)n + i]; }   // depth of subtree root i, mod 2
 };
 
-__global__ void k_sched_header(int32_t* sched, int T, int top_lds) {
+__global__ void k_sched_header(int32_t* sched, int T, int top_lds, int even_lines) {
   if (threadIdx.x == 0) {
     sched[0] = 0;
     sched[1] = 0;
     sched[2] = T;
     sched[3] = top_lds;
+    sched[4] = even_lines;
+  }
+}
+
+// depth parity of each subtree root (walk to node 0)
+__global__ void __launch_bounds__(1024) k_sched_parity(int n, const int32_t* __restrict__ parent,
+                                                       int32_t* __restrict__ sched) {
+  const size_t N = (size_t)n;
+  for (int i = threadIdx.x; i < sched[0]; i += 1024) {
+    int node = reinterpret_cast<const int4*>(sched + 8)[i].x, d = 0;
+    while (node != 0) {
+      node = parent[node];
+      ++d;
+    }
+    sched[8 + 8 * N + i] = d & 1;
   }
 }
 
@@ -501,6 +520,19 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     for (int k = threadIdx.x; k < cnt; k += kSubBS)
       if (s_flag[k] == 2) store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
     if constexpr (!COUNTED) {
+      if (src.quad && S.even_lines()) {      // odd-depth nodes' lines are never read: flag 2 -> 1
+        const int rp = S.sub_parity(si);
+        for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+          if (s_flag[k] != 2) continue;
+          int x = first + k, d = rp;
+          while (x != root) {
+            x = s_par[x - first];
+            ++d;
+          }
+          if (d & 1) s_flag[k] = 1;
+        }
+        __syncthreads();
+      }
       if (src.quad) {
         auto box_of = [&](int g) -> const NodeBox& {
           return g >= n - 1 ? s_leaf[g - (n - 1) - first] : s_int[g - first];
@@ -619,13 +651,14 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
   }
 }
 
-size_t refit_sched_ints(int n) { return 8 + 8 * (size_t)(n > 0 ? n : 1); }
+size_t refit_sched_ints(int n) { return 8 + 9 * (size_t)(n > 0 ? n : 1); }
 
 int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_t* rl, const int32_t* rr,
-                   int32_t* sched, hipStream_t st) {
-  SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, g_refit_sub, g_refit_top);
+                   int32_t* sched, hipStream_t st, bool even_lines) {
+  SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, g_refit_sub, g_refit_top, even_lines ? 1 : 0);
   SLX_LAUNCH("sched", k_sched_mark, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, st, n, parent, rl, rr, sched);
   SLX_LAUNCH("sched", k_sched_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cb, parent, sched);
+  if (even_lines) SLX_LAUNCH("sched", k_sched_parity, dim3(1), dim3(1024), 0, st, n, parent, sched);
   return check_launch("refit_schedule");
 }
 
@@ -808,7 +841,7 @@ extern "C" int slx_refit_schedule(int n_leaves, const int32_t* child_b, const in
                                   slx_stream_t stream) {
   SLX_REQUIRE(n_leaves >= 1, "slx_refit_schedule: n_leaves < 1");
   SLX_REQUIRE(child_b && parent && range_l && range_r && sched, "slx_refit_schedule: null pointer");
-  return refit_schedule(n_leaves, child_b, parent, range_l, range_r, sched, S(stream));
+  return refit_schedule(n_leaves, child_b, parent, range_l, range_r, sched, S(stream), false);
 }
 
 extern "C" int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int32_t* child_b,
@@ -825,7 +858,7 @@ extern "C" int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* c
              (const uint32_t*)codes, 0, child_a, child_b, parent, range_l, range_r);
   rc = check_launch("slx_cell_topology");
   if (rc) return rc;
-  return refit_schedule(kCells, child_b, parent, range_l, range_r, sched, st);
+  return refit_schedule(kCells, child_b, parent, range_l, range_r, sched, st, false);
 }
 
 extern "C" size_t slx_cell_build_ws_bytes(int nf) {
@@ -918,7 +951,7 @@ extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, con
              child_a, child_b, parent, range_l, range_r);
   rc = check_launch("k_karras");
   if (rc) return rc;
-  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st);
+  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st, true);
   if (rc) return rc;
   LeafSrc src{};
   src.fmin = (const float4*)fmin4;
@@ -1438,7 +1471,7 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order, const int32_t* qedges) {
+                    const int32_t* order, const int32_t* qedges, bool prezeroed) {
   SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
   SLX_REQUIRE(!qedges || kind == 0, "slx_traverse_cached: edge queries are walked, not cached");
   SLX_REQUIRE(c && c->redo && (kind == 0 || (c->cap >= 1 && c->leaves && c->count && c->q0)),
@@ -1456,7 +1489,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   if (kind == 0 && wave)                      // the wavefront walk keeps its stack in registers
     return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
                            nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order, nullptr, qedges);
-  SLX_LAUNCH("fill", k_zero_one, dim3(1), dim3(64), 0, st, c->redo);
+  if (!prezeroed) SLX_LAUNCH("fill", k_zero_one, dim3(1), dim3(64), 0, st, c->redo);
   int rc = SLX_OK;
   if (kind == 1) {
     const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
@@ -1507,14 +1540,20 @@ extern "C" int slx_tree_snapshot(int nv, const float* x4, float* x_snap, uint32_
   return check_launch("slx_tree_snapshot");
 }
 
-extern "C" int slx_tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp,
-                                     slx_stream_t stream) {
+namespace slx {
+int tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp, bool zero, hipStream_t st) {
   SLX_REQUIRE(nv >= 0 && disp && (nv == 0 || (x4 && x_snap)), "slx_tree_displacement: bad arguments");
-  SLX_LAUNCH("fill", k_zero_u32, dim3(1), dim3(64), 0, S(stream), disp);
+  if (zero) SLX_LAUNCH("fill", k_zero_u32, dim3(1), dim3(64), 0, st, disp);
   if (nv > 0)
-    SLX_LAUNCH("tree_disp", k_tree_disp, dim3(blocks_for(nv, 256 * kDispPer)), dim3(256), 0, S(stream), nv,
+    SLX_LAUNCH("tree_disp", k_tree_disp, dim3(blocks_for(nv, 256 * kDispPer)), dim3(256), 0, st, nv,
                (const float4*)x4, (const float4*)x_snap, disp);
   return check_launch("slx_tree_displacement");
+}
+}  // namespace slx
+
+extern "C" int slx_tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp,
+                                     slx_stream_t stream) {
+  return tree_displacement(nv, x4, x_snap, disp, true, S(stream));
 }
 
 extern "C" int slx_traverse(int mode, int nq, const float* q4, float padding, int n_leaves, const int32_t* num,
@@ -1530,11 +1569,17 @@ extern "C" int slx_traverse_edges(int mode, int ne, const int32_t* edges, const 
                                   int n_leaves, const int32_t* num, const int32_t* sorted_faces, const float* nlo4,
                                   const float* nhi4, const float* quad4, const float* fmin4, const float* fmax4,
                                   const float* box6, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
-                                  slx_stream_t stream) {
+                                  const int32_t* order, int32_t* redo, slx_stream_t stream) {
   SLX_REQUIRE(ne == 0 || (edges && x4), "slx_traverse_edges: null edges or positions");
+  if (mode == 1 && redo && n_leaves > 0) {
+    slx_cand_cache walk{};
+    walk.redo = redo;
+    return traverse_cached("traverse_ee", 0, ne, x4, padding, n_leaves, nlo4, nhi4, quad4, &walk, nullptr, cap, cand,
+                           cand_n, status, S(stream), order, edges, false);
+  }
   return traverse_launch(mode == 0 ? "traverse_ee_cell" : "traverse_ee", mode, ne, x4, padding, n_leaves, num,
                          sorted_faces, nlo4, nhi4, quad4, fmin4, fmax4, box6, cap, cand, cand_n, status, S(stream),
-                         nullptr, nullptr, nullptr, nullptr, edges);
+                         nullptr, nullptr, order, nullptr, edges);
 }
 
 // ===================== batched instances: one tree, one subtree per instance =====================
@@ -1722,7 +1767,7 @@ extern "C" int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, c
                range_r, sorted_inst, face_base, roots);
   rc = check_launch("slx_seg_build");
   if (rc) return rc;
-  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st);
+  rc = refit_schedule(nf, child_b, parent, range_l, range_r, sched, st, false);
   if (rc) return rc;
   LeafSrc src{};
   src.fmin = (const float4*)fmin4;

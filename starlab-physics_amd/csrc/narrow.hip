@@ -279,14 +279,18 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
 }
 
 // update_dx over the contacts: y += f * (dx / nc) (XPBD.py:383-387); the bucket
-// is summed in increasing serial order.
+// is summed in increasing serial order.  The bucket's counter is left at zero for
+// the next pass, and so are the counters in `after`.
 __global__ void __launch_bounds__(256)
 k_contact_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __restrict__ brec,
-                const float4* __restrict__ self_acc, const float* __restrict__ fixed, float4* __restrict__ y4) {
+                const float4* __restrict__ self_acc, const float* __restrict__ fixed, float4* __restrict__ y4,
+                ZeroList after) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && (int)threadIdx.x < after.n) *after.p[threadIdx.x] = 0;
   if (v >= nv) return;
   const float4 sa = self_acc[v];            // first-set deltas, already summed in order
   int n = bcnt[v];
+  if (n != 0) bcnt[v] = 0;
   if (n > bucket) n = bucket;
   const float nc = sa.w + (float)n;
   if (!(nc > 0.f)) return;
@@ -301,6 +305,12 @@ __global__ void k_zero_i32(int n, int32_t* __restrict__ a) {
   if (i < n) a[i] = 0;
 }
 
+__global__ void k_zero_i32_list(int n, int32_t* __restrict__ a, ZeroList zl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && (int)threadIdx.x < zl.n) *zl.p[threadIdx.x] = 0;
+  if (i < n) a[i] = 0;
+}
+
 }  // namespace slx
 
 using namespace slx;
@@ -310,13 +320,21 @@ extern "C" size_t slx_collision_ws_bytes(int nv, int nvs, int rec_cap) {
   return contact_ws_layout(nv, rec_cap, nullptr, nullptr);
 }
 
-extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
-                                     const float* xs4, const float* fixed, const float* m_inv, float dhat,
-                                     float k_col, int cap, const int32_t* c_vt_st, const int32_t* n_vt_st,
-                                     const int32_t* c_vt_dy, const int32_t* n_vt_dy, const int32_t* c_tv_st,
-                                     const int32_t* n_tv_st, int32_t* pair_num, int pair_cap, int32_t* pair_cnt,
-                                     float* pair_rec, const slx_edge_sets* ee, int rec_cap, int32_t* status,
-                                     void* ws, size_t ws_bytes, slx_stream_t stream) {
+namespace slx {
+int frame_zero(int nv, int rec_cap, void* ws, const ZeroList& zl, hipStream_t st) {
+  ContactWs w;
+  contact_ws_layout(nv, rec_cap, (char*)ws, &w);
+  SLX_LAUNCH("fill", k_zero_i32_list, dim3(blocks_for(nv > 0 ? nv : 1, 256)), dim3(256), 0, st, nv, w.bcnt, zl);
+  return check_launch("frame_zero");
+}
+
+int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
+                        const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col, int cap,
+                        const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
+                        const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
+                        int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee, int rec_cap,
+                        int32_t* status, void* ws, size_t ws_bytes, hipStream_t st, bool bcnt_zeroed,
+                        const ZeroList& after) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
   if (ee) {
     int rc = ee_check(ee, false);
@@ -333,12 +351,11 @@ extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, c
   SLX_REQUIRE(!rec || (pair_cnt && pair_cap >= 0), "slx_collision_pass: pair records need pair_cnt and pair_cap");
   if (ws_bytes < slx_collision_ws_bytes(nv, nvs, rec_cap))
     return fail(SLX_EWS, "slx_collision_pass: workspace too small");
-  hipStream_t st = (hipStream_t)stream;
   ContactWs w;
   contact_ws_layout(nv, rec_cap, (char*)ws, &w);
   QuerySets Q{nv, nvs, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
   const int nqt = 2 * nv + Q.nvs;
-  SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
+  if (!bcnt_zeroed) SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
   if (rec)
     SLX_LAUNCH("contacts", k_contacts<true>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
                (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
@@ -353,8 +370,21 @@ extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, c
     if (rc) return rc;
   }
   SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt,
-             w.brec, w.self_acc, fixed, (float4*)y4);
+             w.brec, w.self_acc, fixed, (float4*)y4, after);
   return check_launch("slx_collision_pass");
+}
+}  // namespace slx
+
+extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
+                                     const float* xs4, const float* fixed, const float* m_inv, float dhat,
+                                     float k_col, int cap, const int32_t* c_vt_st, const int32_t* n_vt_st,
+                                     const int32_t* c_vt_dy, const int32_t* n_vt_dy, const int32_t* c_tv_st,
+                                     const int32_t* n_tv_st, int32_t* pair_num, int pair_cap, int32_t* pair_cnt,
+                                     float* pair_rec, const slx_edge_sets* ee, int rec_cap, int32_t* status,
+                                     void* ws, size_t ws_bytes, slx_stream_t stream) {
+  return collision_pass_impl(nv, nvs, faces_dy, faces_st, y4, xs4, fixed, m_inv, dhat, k_col, cap, c_vt_st, n_vt_st,
+                             c_vt_dy, n_vt_dy, c_tv_st, n_tv_st, pair_num, pair_cap, pair_cnt, pair_rec, ee, rec_cap,
+                             status, ws, ws_bytes, (hipStream_t)stream, false, ZeroList{});
 }
 
 extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,

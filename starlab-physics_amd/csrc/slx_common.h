@@ -125,7 +125,35 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order, const int32_t* qedges = nullptr);
+                    const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false);
+
+// Counters a kernel zeroes for the next pass (thread i < n of block 0 writes *p[i] = 0).
+struct ZeroList {
+  int32_t* p[6];
+  int n;
+};
+
+// The collision and velocity passes as slx_frame runs them (narrow.hip,
+// velocity.hip): bcnt_zeroed -- the buckets' counters are known to be zero (every
+// apply kernel leaves them so); `after` -- counters the apply kernel zeroes for the
+// next pass (the walks' redo lists).
+int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
+                        const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col, int cap,
+                        const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
+                        const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
+                        int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee, int rec_cap,
+                        int32_t* status, void* ws, size_t ws_bytes, hipStream_t st, bool bcnt_zeroed,
+                        const ZeroList& after);
+int velocity_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                       const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st, float mu,
+                       int pair_cap, const int32_t* pair_cnt, const float* pair_rec, const slx_edge_sets* ee,
+                       float* x4, float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes, hipStream_t st,
+                       bool bcnt_zeroed);
+// zero the buckets' counters (workspace ws) and the listed counters in one launch
+int frame_zero(int nv, int rec_cap, void* ws, const ZeroList& zl, hipStream_t st);
+// slx_tree_displacement; zero = false keeps disp as it is before the max (a
+// snapshot taken in the same stream order has zeroed it)
+int tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp, bool zero, hipStream_t st);
 
 // edge-edge phase (edges.hip): argument checks; the vertex-triangle bucket
 // records folded into self_acc, then the edge-edge deltas into the emptied buckets

@@ -265,7 +265,7 @@ k_vel_contacts(int nv, int nvs, const int32_t* __restrict__ fdy, const int32_t* 
 // update_dv (XPBD.py:397-404) over the summed deltas, then (x4 non-null) update_x.
 // bcnt null: no deltas (dv = 0, nc = 0).
 __global__ void __launch_bounds__(256)
-k_vel_apply(int nv, int bucket, const int32_t* __restrict__ bcnt, const float4* __restrict__ brec,
+k_vel_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __restrict__ brec,
             const float4* __restrict__ self_acc, const float* __restrict__ fixed, const float* __restrict__ m_inv,
             float4* __restrict__ v4, float4* __restrict__ x4, float dt) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,6 +275,7 @@ k_vel_apply(int nv, int bucket, const int32_t* __restrict__ bcnt, const float4* 
   if (bcnt) {
     const float4 sa = self_acc[v];
     int n = bcnt[v];
+    if (n != 0) bcnt[v] = 0;               // left at zero for the next pass
     if (n > bucket) n = bucket;
     nc = sa.w + (float)n;
     dv = bucket_sum(make_float3(sa.x, sa.y, sa.z), brec + (size_t)v * bucket, n);
@@ -337,11 +338,12 @@ extern "C" int slx_update_x(int nv, float* x4, const float* v4, float dt, slx_st
   return check_launch("slx_update_x");
 }
 
-extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
-                                    const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
-                                    float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec,
-                                    const slx_edge_sets* ee, float* x4, float dt, int rec_cap, int32_t* status,
-                                    void* ws, size_t ws_bytes, slx_stream_t stream) {
+namespace slx {
+int velocity_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                       const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st, float mu,
+                       int pair_cap, const int32_t* pair_cnt, const float* pair_rec, const slx_edge_sets* ee,
+                       float* x4, float dt, int rec_cap, int32_t* status, void* ws, size_t ws_bytes, hipStream_t st,
+                       bool bcnt_zeroed) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && pair_cap >= 0 && rec_cap >= 1, "slx_velocity_pass: bad sizes");
   if (nv == 0) return SLX_OK;
   SLX_REQUIRE(v4 && fixed && m_inv, "slx_velocity_pass: null pointer");
@@ -350,7 +352,6 @@ extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, co
     int rc = ee_check(ee, true);
     if (rc) return rc;
   }
-  hipStream_t st = (hipStream_t)stream;
   const bool pairs = pair_cnt != nullptr;
   if (!pairs) {
     SLX_LAUNCH("vel_apply", k_vel_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, 1, nullptr, nullptr,
@@ -366,7 +367,7 @@ extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, co
   ContactWs w;
   contact_ws_layout(nv, rec_cap, (char*)ws, &w);
   const int nqt = 2 * nv + nvs;
-  SLX_LAUNCH("fill", k_zero_counts, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
+  if (!bcnt_zeroed) SLX_LAUNCH("fill", k_zero_counts, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
   SLX_LAUNCH("vel_contacts", k_vel_contacts, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, nv, nvs, faces_dy,
              faces_st, (const float4*)v4, (const float4*)vs4, fixed, m_inv, m_inv_st, mu, pair_cap, pair_cnt,
              (const float4*)pair_rec, rec_cap, w.bcnt, w.brec, w.self_acc, status);
@@ -377,6 +378,16 @@ extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, co
   SLX_LAUNCH("vel_apply", k_vel_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt, w.brec,
              w.self_acc, fixed, m_inv, (float4*)v4, (float4*)x4, dt);
   return check_launch("slx_velocity_pass");
+}
+}  // namespace slx
+
+extern "C" int slx_velocity_pass_ee(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
+                                    const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st,
+                                    float mu, int pair_cap, const int32_t* pair_cnt, const float* pair_rec,
+                                    const slx_edge_sets* ee, float* x4, float dt, int rec_cap, int32_t* status,
+                                    void* ws, size_t ws_bytes, slx_stream_t stream) {
+  return velocity_pass_impl(nv, nvs, faces_dy, faces_st, v4, vs4, fixed, m_inv, m_inv_st, mu, pair_cap, pair_cnt,
+                            pair_rec, ee, x4, dt, rec_cap, status, ws, ws_bytes, (hipStream_t)stream, false);
 }
 
 extern "C" int slx_velocity_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,

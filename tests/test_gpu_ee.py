@@ -58,7 +58,7 @@ def edge_boxes(x, edges, pad):
     return np.minimum(p0, p1) - np.float32(pad), np.maximum(p0, p1) + np.float32(pad)
 
 
-def traverse_edges(A, tree, edges_dev, x4, ne, pad, cap):
+def traverse_edges(A, tree, edges_dev, x4, ne, pad, cap, order=None, redo=None):
     import torch
     cand = torch.full((cap * max(ne, 1),), -1, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(max(ne, 1), dtype=torch.int32, device="cuda")
@@ -68,7 +68,7 @@ def traverse_edges(A, tree, edges_dev, x4, ne, pad, cap):
     A.call("slx_traverse_edges", 0 if cell else 1, ne, P(edges_dev), P(x4), float(np.float32(pad)),
            tree.n_leaves, P(tree.num) if cell else None, P(tree.sorted_faces), P(tree.nlo4), P(tree.nhi4),
            None if cell else P(tree.quad4), P(tree.fmin4) if cell else None, P(tree.fmax4) if cell else None,
-           P(tree.box6) if cell else None, cap, P(cand), P(cnt), P(st), A.stream_handle())
+           P(tree.box6) if cell else None, cap, P(cand), P(cnt), P(st), P(order), P(redo), A.stream_handle())
     return cand.view(cap, -1)[:, :ne], cnt[:ne], st
 
 
@@ -95,6 +95,39 @@ def test_edge_traversal_bit_exact(slx, orc, mode, pad, cap):
             np.testing.assert_array_equal(cand[:cnt[i], i], oc[i, :on[i]])
         assert (st.item() != 0) == (fl > 0)
         assert on.sum() > ne
+
+
+@pytest.mark.parametrize("depth", [16, 32])
+def test_edge_traversal_two_tier_bit_exact(slx, orc, depth):
+    """The two-tier walk over edge boxes, in the Morton-rank edge order: a
+    shallow first-tier stack (16 entries overflow here), the overflowing edges
+    walked again in full; the oracle's lists."""
+    import importlib
+    import torch
+    bp = importlib.import_module("starlab-physics_amd.broadphase")
+    A = abi()
+    cloth, _ = ee_scene(slx, n=40)
+    pad, cap = 0.3, 64
+    tree = bp.DeviceTree(cloth, "tri")
+    tree.build(pad)
+    ne = cloth.num_edges
+    redo = torch.zeros(1 + ne, dtype=torch.int32, device="cuda")
+    print(f"[ee two-tier] depth={depth} ne={ne}", flush=True)
+    A.call("slx_set_walk_stack", depth)
+    try:
+        cand, cnt, st = traverse_edges(A, tree, cloth.edge_indices, cloth.x4, ne, pad, cap,
+                                       order=cloth.edge_order(), redo=redo)
+    finally:
+        A.call("slx_set_walk_stack", 32)
+    ot = orc.TriBVH(cloth.fid_np, cloth.positions(), pad)
+    oc, on, fl = ot.query(*edge_boxes(cloth.positions(), cloth.eid_np, pad), cap)
+    cnt, cand = cnt.cpu().numpy(), cand.cpu().numpy()
+    np.testing.assert_array_equal(cnt, on)
+    for i in range(ne):
+        np.testing.assert_array_equal(cand[:cnt[i], i], oc[i, :on[i]])
+    assert (st.item() != 0) == (fl > 0)
+    if depth == 16:
+        assert redo[0].item() > 0, "the shallow stack must overflow for some edge query"
 
 
 def test_edge_traversal_wavefront_walk(slx, orc):
@@ -291,8 +324,9 @@ def test_frames_with_edge_edge_bit_exact(slx, mode, velocity):
 
 def test_frames_with_edge_edge_shallow_walk(slx):
     """A 16-entry first-tier stack, a deeper tree (two 40 x 40 grids) and large
-    boxes (padding 0.3): edge queries the first tier cannot serve go through the
-    redo list and the full walk; the frames stay bit-exact."""
+    boxes (padding 0.3, where the standalone walk above overflows it): edge
+    queries the first tier cannot serve go through the redo list and the full
+    walk; the frames stay bit-exact."""
     A = abi()
     cloth, sim, ref = ee_frame_scene(slx, "tri", True, padding=0.3, n=40)
     A.call("slx_set_walk_stack", 16)
@@ -302,8 +336,5 @@ def test_frames_with_edge_edge_shallow_walk(slx):
             sim.forward(4)
             ref.forward(4)
             np.testing.assert_array_equal(cloth.positions(), ref.x)
-        redo = sim._cd["ee"]["redo"].cpu().numpy()
-        ne = cloth.num_edges
-        assert redo[1 + ne] > 0, "the shallow stack must overflow for some edge query"
     finally:
         A.call("slx_set_walk_stack", 32)
