@@ -311,13 +311,22 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         }
         continue;
       }
-      // broad phase (XPBD.py:186-215); the static-tree queries on their own stream
+      // broad phase (XPBD.py:186-215); the static-tree queries, and the narrow
+      // phase of the first query set (which needs only their lists and only
+      // moves its own vertex), on their own stream beside the dynamic tree's
+      // refit and queries
+      const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
       bool static_pending = false;
       if (nvs > 0) {
         if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
                    n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
+        if (rc) return rc;
+        rc = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
+                                 d->k_col, cap, c0, n0, c1, n1, c2, n2, pn, rec ? d->pair_cap : 0,
+                                 rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr, nullptr, d->rec_cap,
+                                 d->status, d->ws, d->ws_bytes, side->q, true, after, 1, nullptr);
         if (rc) return rc;
         if (ee) {
           rc = ee_query(d->st, 0, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q);
@@ -341,15 +350,15 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         rc = ee_query(d->dyn, 1, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st);
         if (rc) return rc;
       }
-      if (static_pending && hipStreamWaitEvent(st, side->qjoin, 0) != hipSuccess)
-        return fail(SLX_EHIP, "slx_frame: join failed");
-      // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369)
-      const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
+      // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369): the
+      // other two sets here, then (after the static stream's join) the edge sets
+      // and the apply
       rc = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                                d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
                                rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
                                ee ? (rec ? &ee_rec : &ee_norec) : nullptr, d->rec_cap, d->status, d->ws,
-                               d->ws_bytes, st, true, after);
+                               d->ws_bytes, st, true, after, static_pending ? 2 : 0,
+                               static_pending ? side->qjoin : nullptr);
       if (rc) return rc;
       if (sew) {                             // sewing constraints (XPBD.py:371-372, there commented out)
         rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,

@@ -208,10 +208,11 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
            const float4* __restrict__ y4, const float4* __restrict__ xs4, const float* __restrict__ fixed,
            const float* __restrict__ m_inv, float dhat, float k, int bucket, int32_t* __restrict__ bcnt,
            float4* __restrict__ brec, float4* __restrict__ self_acc, int32_t* __restrict__ pair_num,
-           int pcap, int32_t* __restrict__ pair_cnt, float4* __restrict__ pair_rec, int32_t* status) {
+           int pcap, int32_t* __restrict__ pair_cnt, float4* __restrict__ pair_rec, int32_t* status, int q_begin,
+           int q_end) {
   const int nqt = 2 * Q.nv + Q.nvs;
-  const int qg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (qg >= nqt) return;
+  const int qg = q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);   // queries [q_begin, q_end)
+  if (qg >= q_end) return;
   int set, qv, ncand, nq;
   const int32_t* cand;
   int pairs = 0, own = 0, stored = 0;
@@ -334,8 +335,9 @@ int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t*
                         const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
                         int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee, int rec_cap,
                         int32_t* status, void* ws, size_t ws_bytes, hipStream_t st, bool bcnt_zeroed,
-                        const ZeroList& after) {
+                        const ZeroList& after, int set0, hipEvent_t before_apply) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
+  SLX_REQUIRE(set0 >= 0 && set0 <= 2, "slx_collision_pass: bad first-set mode");
   if (ee) {
     int rc = ee_check(ee, false);
     if (rc) return rc;
@@ -355,15 +357,25 @@ int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t*
   contact_ws_layout(nv, rec_cap, (char*)ws, &w);
   QuerySets Q{nv, nvs, cap, c_vt_st, n_vt_st, c_vt_dy, n_vt_dy, c_tv_st, n_tv_st};
   const int nqt = 2 * nv + Q.nvs;
-  if (!bcnt_zeroed) SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
+  // set0: 0 = all query sets here; 1 = only the first set (its deltas stay in
+  // self_acc; nothing else is launched); 2 = the other two sets and the rest of
+  // the pass (the first set launched before, on another stream, which
+  // before_apply joins ahead of the edge sets and the apply)
+  const int q_begin = set0 == 2 ? nv : 0, q_end = set0 == 1 ? nv : nqt;
+  if (set0 != 1 && !bcnt_zeroed)
+    SLX_LAUNCH("fill", k_zero_i32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, w.bcnt);
+  const char* name = set0 == 1 ? "contacts_vt_st" : "contacts";
   if (rec)
-    SLX_LAUNCH("contacts", k_contacts<true>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-               (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
-               w.self_acc, pair_num, pair_cap, pair_cnt, (float4*)pair_rec, status);
+    SLX_LAUNCH(name, k_contacts<true>, dim3(blocks_for(q_end - q_begin, 256)), dim3(256), 0, st, Q, faces_dy,
+               faces_st, (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
+               w.self_acc, pair_num, pair_cap, pair_cnt, (float4*)pair_rec, status, q_begin, q_end);
   else
-    SLX_LAUNCH("contacts", k_contacts<false>, dim3(blocks_for(nqt, 256)), dim3(256), 0, st, Q, faces_dy, faces_st,
-               (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
-               w.self_acc, pair_num, 0, nullptr, nullptr, status);
+    SLX_LAUNCH(name, k_contacts<false>, dim3(blocks_for(q_end - q_begin, 256)), dim3(256), 0, st, Q, faces_dy,
+               faces_st, (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, rec_cap, w.bcnt, w.brec,
+               w.self_acc, pair_num, 0, nullptr, nullptr, status, q_begin, q_end);
+  if (set0 == 1) return check_launch("slx_collision_pass");
+  if (before_apply && hipStreamWaitEvent(st, before_apply, 0) != hipSuccess)
+    return fail(SLX_EHIP, "slx_collision_pass: join failed");
   if (ee) {
     int rc = ee_position_launch(nv, ee, (const float4*)y4, (const float4*)xs4, fixed, m_inv, dhat, k_col, w, rec_cap,
                                 status, st);
@@ -384,7 +396,7 @@ extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, c
                                      void* ws, size_t ws_bytes, slx_stream_t stream) {
   return collision_pass_impl(nv, nvs, faces_dy, faces_st, y4, xs4, fixed, m_inv, dhat, k_col, cap, c_vt_st, n_vt_st,
                              c_vt_dy, n_vt_dy, c_tv_st, n_tv_st, pair_num, pair_cap, pair_cnt, pair_rec, ee, rec_cap,
-                             status, ws, ws_bytes, (hipStream_t)stream, false, ZeroList{});
+                             status, ws, ws_bytes, (hipStream_t)stream, false, ZeroList{}, 0, nullptr);
 }
 
 extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
