@@ -340,6 +340,16 @@ typedef struct slx_edge_sets {
   int32_t* redo;            /* nullable: [2 (1 + ne)] lists of the edges a shallow-stack walk could
                                not serve (static tree, dynamic tree), for slx_frame's two-tier
                                edge walk of triangle trees */
+  /* the edge gate (slx_frame only, triangle trees; both nullable = off): type 8
+     needs (u x v).w >= 1e-4, and |(u x v).w| <= |u| |v| |w| is bounded per edge
+     from the longest edges and the largest y - x displacement of the pass; an
+     edge whose bound, with rounding allowances, stays below 1e-4 is neither
+     walked nor tested (its candidate list is not written; it could not have
+     acted).  The dynamic set is gated in substeps whose dynamic tree was refit
+     from the current x.  slx_collision_pass_ee ignores these fields. */
+  uint32_t* gate;           /* [8] scratch */
+  int32_t* live;            /* [2 (1 + ne)] lists of the edges the gate keeps (static, dynamic) */
+  int32_t nes;              /* static edges (the rows of edges_st) */
 } slx_edge_sets;
 
 /* slx_collision_pass with the edge-edge constraints after the vertex-triangle

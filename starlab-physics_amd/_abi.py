@@ -117,7 +117,8 @@ class EdgeSets(C.Structure):
     """slx_edge_sets (include/slx.h)."""
     _fields_ = [("ne", I32), ("edges_dy", PTR), ("edges_st", PTR), ("fe_dy", PTR), ("fe_st", PTR), ("cap", I32),
                 ("c_st", PTR), ("n_st", PTR), ("c_dy", PTR), ("n_dy", PTR), ("pair_cap", I32), ("pair_num", PTR),
-                ("pair_cnt", PTR), ("pair_rec", PTR), ("order", PTR), ("redo", PTR)]
+                ("pair_cnt", PTR), ("pair_rec", PTR), ("order", PTR), ("redo", PTR), ("gate", PTR), ("live", PTR),
+                ("nes", I32)]
 
 
 class FrameDesc(C.Structure):

@@ -87,12 +87,36 @@ __device__ __forceinline__ float dist_ee(float3 x0, float3 x1, float3 x2, float3
 
 struct EdgeArgs {
   int ne, cap, pcap;
+  const uint32_t* gate;          // the edge gate's state (nullable: no gate)
   const int32_t *edges_dy, *edges_st, *fe_dy, *fe_st;
   const int32_t *c_st, *n_st, *c_dy, *n_dy;
   int32_t* pair_num;
   int32_t* pair_cnt;
   float4* pair_rec;
 };
+
+// (the edge gate's predicate; the gate itself is described further below)
+__device__ __forceinline__ uint32_t max_key(float v) {          // v >= 0 or NaN
+  return v != v ? 0x7f800000u : __float_as_uint(v);
+}
+
+// edge e of set `set` (0 static, 1 dynamic) with |u| = lu cannot have an acting pair
+__device__ __forceinline__ bool ee_gated(const uint32_t* __restrict__ gate, int set, float lu) {
+  if (!gate || !(gate[4] & (set ? 2u : 1u))) return false;
+  const float pad = __uint_as_float(gate[5]);
+  const float C = fmaxf(fmaxf(__uint_as_float(gate[1]), __uint_as_float(gate[3])), __uint_as_float(gate[7]));
+  const float slack = ldexpf(C, -20);
+  float b;
+  if (set) {
+    const float LX = __uint_as_float(gate[0]), D = __uint_as_float(gate[2]);
+    const float LV = LX + 3.4641016f * D;                           // |v| <= LX + 2 sqrt(3) D
+    b = (lu * (LV + slack)) * (lu + 1.7320508f * (((LX + 2.0f * pad) + D) + slack));
+  } else {
+    const float LS = __uint_as_float(gate[6]);
+    b = (lu * (LS + slack)) * (lu + 1.7320508f * ((LS + 2.0f * pad) + slack));
+  }
+  return b * 1.001f < 1e-4f;                                       // false for NaN
+}
 
 // A vertex's bucket records so far folded into its running sum (in key order,
 // as the apply kernels would add them) and count; the bucket is emptied.
@@ -141,6 +165,10 @@ k_ee_contacts(EdgeArgs A, const float4* __restrict__ y4, const float4* __restric
     const int v0 = A.edges_dy[2 * e], v1 = A.edges_dy[2 * e + 1];
     const float3 x0 = xyz(y4[v0]), x1 = xyz(y4[v1]);
     const float3 x01 = sub3(x0, x1);
+    if (A.gate) {                          // an edge the gate dropped (it was not walked): no pair can act
+      const float3 u = sub3(x1, x0);
+      if (ee_gated(A.gate, set, sqrtf(dot3(u, u)))) n = 0;
+    }
     const uint32_t base = set ? 6u * (uint32_t)A.ne * (uint32_t)A.cap : 0u;
     const uint32_t nslot = set ? 4u : 2u;
     for (int j = 0; j < n; ++j) {
@@ -254,11 +282,166 @@ k_vel_ee_contacts(EdgeArgs A, const float4* __restrict__ v4, const float4* __res
   if (over && status) atomicOr(status, kStatusRecOverflow);
 }
 
+// ---------------- the edge gate ----------------
+//
+// Type 8 needs (u x v).w >= 1e-4 in d_type_EE (u, v the two edges, w = x0 - x2),
+// and |(u x v).w| <= |u| |v| |w|.  For a candidate face f of edge e's query box
+// Q (y -+ pad), f's box (its vertices at the tree's positions X, -+ pad) overlaps
+// Q, so per axis |y0 - X2| <= |u| + ext(f) + 2 pad and |y2 - X2| <= D, hence
+//   |w| <= |u| + sqrt(3) (LX + 2 pad + D),   |v| <= LX + 2 sqrt(3) D,
+// with LX the longest edge at X and D = max |y - X|_inf over the vertices (for
+// the static mesh, whose tree and narrow phase both use xs: LS and D = 0).  When
+// that bound on the triple product, with allowances for rounding (2^-20 of the
+// largest coordinate per axis, a factor 1.001), stays below 1e-4, none of e's
+// pairs can act: the edge is not walked, and k_ee_contacts, evaluating the same
+// predicate, tests nothing for it (pair count 0).  NaN or infinite inputs never
+// pass the test.  The dynamic set is gated only when its tree's boxes come from
+// the current x (X = x).
+//
+// gate[] (float bits; maxima kept with integer atomics: non-negative floats order
+// as their bits, NaN counts as +inf): 0 LX, 1 max |x|, 2 D, 3 max |y|, 4 flags (1
+// static set gated, 2 dynamic set gated), 5 padding, 6 LS, 7 max |xs|.
+__device__ __forceinline__ void block_max2(float a, float b, uint32_t* ga, uint32_t* gb) {
+  __shared__ uint32_t sa[4], sb[4];
+  uint32_t ka = max_key(a), kb = max_key(b);
+  for (int o = 32; o > 0; o >>= 1) {
+    ka = max(ka, (uint32_t)__shfl_xor((int)ka, o, 64));
+    kb = max(kb, (uint32_t)__shfl_xor((int)kb, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sa[w] = ka;
+    sb[w] = kb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      ka = max(ka, sa[i]);
+      kb = max(kb, sb[i]);
+    }
+    if (ka > *ga) atomicMax(ga, ka);
+    if (kb > *gb) atomicMax(gb, kb);
+  }
+}
+
+// zero gate[z0 .. z1), set the flags and the padding, zero the live counts
+__global__ void k_gate_set(uint32_t* __restrict__ gate, int z0, int z1, uint32_t flags, float pad,
+                           int32_t* __restrict__ c0, int32_t* __restrict__ c1) {
+  const int i = threadIdx.x;
+  if (i >= z0 && i < z1) gate[i] = 0u;
+  if (i == 4) gate[4] = flags;
+  if (i == 5) gate[5] = __float_as_uint(pad);
+  if (i == 0 && c0) *c0 = 0;
+  if (i == 0 && c1) *c1 = 0;
+}
+
+// gate[slot] = longest edge of `edges` at p4, gate[slot + 1] = largest |coordinate|
+// of their end points (8 edges per thread; 256 threads)
+__global__ void __launch_bounds__(256)
+k_gate_edges(int ne, const int32_t* __restrict__ edges, const float4* __restrict__ p4, uint32_t* gate, int slot) {
+  float lm = 0.f, cm = 0.f;
+  const int base = (blockIdx.x * 256 + threadIdx.x) * 8;
+  for (int i = 0; i < 8; ++i) {
+    const int e = base + i;
+    if (e >= ne) break;
+    const float3 a = xyz(p4[edges[2 * e]]), b = xyz(p4[edges[2 * e + 1]]);
+    const float3 d = sub3(b, a);
+    const float l = sqrtf(dot3(d, d));
+    const float c = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))),
+                          fmaxf(fabsf(b.y), fabsf(b.z)));
+    lm = (l != l || lm != lm) ? l + lm : fmaxf(lm, l);            // a NaN sticks
+    cm = (c != c || cm != cm) ? c + cm : fmaxf(cm, c);
+  }
+  block_max2(lm, cm, gate + slot, gate + slot + 1);
+}
+
+// gate[2] = max |y - x|_inf, gate[3] = max |y| over the dynamic vertices (8 per
+// thread; 256 threads)
+__global__ void __launch_bounds__(256)
+k_gate_disp(int nv, const float4* __restrict__ x4, const float4* __restrict__ y4, uint32_t* gate) {
+  float dm = 0.f, cm = 0.f;
+  const int base = (blockIdx.x * 256 + threadIdx.x) * 8;
+  for (int i = 0; i < 8; ++i) {
+    const int v = base + i;
+    if (v >= nv) break;
+    const float4 x = x4[v], y = y4[v];
+    const float d = fmaxf(fmaxf(fabsf(y.x - x.x), fabsf(y.y - x.y)), fabsf(y.z - x.z));
+    const float c = fmaxf(fmaxf(fabsf(y.x), fabsf(y.y)), fabsf(y.z));
+    dm = (d != d || dm != dm) ? d + dm : fmaxf(dm, d);
+    cm = (c != c || cm != cm) ? c + cm : fmaxf(cm, c);
+  }
+  block_max2(dm, cm, gate + 2, gate + 3);
+}
+
+// the edges (taken in `order`) whose pairs could act, appended per gated set to
+// its list ([0] = count, then edge ids)
+__global__ void __launch_bounds__(256)
+k_ee_gate(int ne, const int32_t* __restrict__ order, const int32_t* __restrict__ edges, const float4* __restrict__ y4,
+          const uint32_t* __restrict__ gate, int32_t* __restrict__ live_st, int32_t* __restrict__ live_dy) {
+  // the bound grows with |u| <= LX + 2 sqrt(3) D: when the longest possible edge
+  // is gated in both sets (with an allowance for rounding), every edge is
+  {
+    const float lu_max = (__uint_as_float(gate[0]) + 3.4641016f * __uint_as_float(gate[2])) * 1.0001f +
+                         ldexpf(fmaxf(__uint_as_float(gate[1]), __uint_as_float(gate[3])), -20);
+    if ((!live_st || ee_gated(gate, 0, lu_max)) && (!live_dy || ee_gated(gate, 1, lu_max))) return;
+  }
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const bool active = t < ne;
+  const int e = active ? (order ? order[t] : t) : 0;
+  bool live0 = false, live1 = false;
+  if (active) {
+    const float3 u = sub3(xyz(y4[edges[2 * e + 1]]), xyz(y4[edges[2 * e]]));
+    const float lu = sqrtf(dot3(u, u));
+    live0 = !ee_gated(gate, 0, lu);
+    live1 = !ee_gated(gate, 1, lu);
+  }
+  const int lane = threadIdx.x & 63;
+  auto append = [&](bool need, int32_t* list) {
+    if (!list) return;
+    const unsigned long long m = __ballot(need);
+    if (!m) return;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&list[0], __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (need) list[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = e;
+  };
+  append(active && live0, live_st);
+  append(active && live1, live_dy);
+}
+
+int ee_gate_static(const slx_edge_sets* ee, int nes, const float4* xs4, hipStream_t st) {
+  SLX_LAUNCH("ee_gate", k_gate_set, dim3(1), dim3(64), 0, st, ee->gate, 6, 8, 0u, 0.f, (int32_t*)nullptr,
+             (int32_t*)nullptr);
+  if (nes > 0)
+    SLX_LAUNCH("ee_gate", k_gate_edges, dim3(blocks_for(nes, 256 * 8)), dim3(256), 0, st, nes, ee->edges_st, xs4,
+               ee->gate, 6);
+  return check_launch("ee_gate_static");
+}
+
+// live_st / live_dy: the lists of the gated sets (null: that set is not gated);
+// new_x: x changed since the last call (the substep's first pass)
+int ee_gate(const slx_edge_sets* ee, int nv, const float4* x4, const float4* y4, float pad, bool new_x,
+            int32_t* live_st, int32_t* live_dy, hipStream_t st) {
+  const uint32_t flags = (live_st ? 1u : 0u) | (live_dy ? 2u : 0u);
+  SLX_LAUNCH("ee_gate", k_gate_set, dim3(1), dim3(64), 0, st, ee->gate, new_x ? 0 : 2, 4, flags, pad, live_st,
+             live_dy);
+  if (!flags) return check_launch("ee_gate");
+  if (new_x)
+    SLX_LAUNCH("ee_gate", k_gate_edges, dim3(blocks_for(ee->ne, 256 * 8)), dim3(256), 0, st, ee->ne, ee->edges_dy,
+               x4, ee->gate, 0);
+  SLX_LAUNCH("ee_gate", k_gate_disp, dim3(blocks_for(nv, 256 * 8)), dim3(256), 0, st, nv, x4, y4, ee->gate);
+  SLX_LAUNCH("ee_gate", k_ee_gate, dim3(blocks_for(ee->ne, 256)), dim3(256), 0, st, ee->ne, ee->order,
+             ee->edges_dy, y4, ee->gate, live_st, live_dy);
+  return check_launch("ee_gate");
+}
+
 static EdgeArgs edge_args(const slx_edge_sets* ee) {
   EdgeArgs A;
   A.ne = ee->ne;
   A.cap = ee->cap;
   A.pcap = ee->pair_cap;
+  A.gate = ee->gate;
   A.edges_dy = ee->edges_dy;
   A.edges_st = ee->edges_st;
   A.fe_dy = ee->fe_dy;

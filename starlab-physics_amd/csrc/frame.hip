@@ -189,19 +189,34 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     ee_norec.pair_rec = nullptr;
     if (!vel) ee_rec = ee_norec;
   }
+  // the edge gate: triangle trees (their walks take a device list), static maxima once
+  const bool gate_st = ee && d->ee.gate && d->ee.live && nvs > 0 && d->st.mode == 1 && !batched(d->st) &&
+                       d->st.nf > 0 && d->ee.edges_st;
+  const bool gate_dy = ee && d->ee.gate && d->ee.live && d->dyn.mode == 1 && d->dyn.nf > 0;
+  int32_t* live_st = gate_st ? d->ee.live : nullptr;
+  int32_t* live_dy_list = gate_dy ? d->ee.live + 1 + (size_t)d->ee.ne : nullptr;
+  if (gate_st) {
+    rc = ee_gate_static(&d->ee, d->ee.nes, (const float4*)d->xs4, st);
+    if (rc) return rc;
+  }
   // edge boxes of the current y against a tree (XPBD.py:217-232); triangle trees
-  // take the two-tier walk when the caller gave redo lists (set 0 static, 1 dynamic)
+  // take the two-tier walk when the caller gave redo lists (set 0 static, 1 dynamic),
+  // over the gate's list when there is one
   auto ee_query = [&](const slx_tree& t, int set, const char* name, const float* y, const int32_t* c,
-                      const int32_t* n, hipStream_t qs) {
+                      const int32_t* n, hipStream_t qs, const int32_t* live) {
     int nl = t.nf <= 0 ? 0 : (t.mode == 0 ? kCells : t.nf);
     if (t.mode == 1 && d->ee.redo && nl > 0) {
       slx_cand_cache walk{};
       walk.redo = d->ee.redo + (size_t)set * (1 + (size_t)d->ee.ne);
       // the coherent order pays for its scattered list writes on the deep dynamic tree only
+      const int32_t* ord = live ? live + 1 : (set == 1 ? d->ee.order : nullptr);
       return traverse_cached(name, 0, d->ee.ne, y, pad, nl, t.nlo4, t.nhi4, t.quad4, &walk, nullptr, d->ee.cap,
-                             (int32_t*)c, (int32_t*)n, d->status, qs, set == 1 ? d->ee.order : nullptr,
-                             d->ee.edges_dy, true);
+                             (int32_t*)c, (int32_t*)n, d->status, qs, ord, d->ee.edges_dy, true, live);
     }
+    if (live && nl > 0)
+      return traverse_launch(name, 1, d->ee.ne, y, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4, t.quad4,
+                             t.fmin4, t.fmax4, t.box6, d->ee.cap, (int32_t*)c, (int32_t*)n, d->status, qs, nullptr,
+                             nullptr, live + 1, live, d->ee.edges_dy);
     return traverse_launch(name, t.mode, d->ee.ne, y, pad, nl, t.num, t.sorted_faces, t.nlo4,
                            t.nhi4, t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, d->ee.cap,
                            (int32_t*)c, (int32_t*)n, d->status, qs, nullptr, nullptr, nullptr, nullptr,
@@ -316,6 +331,13 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       // moves its own vertex), on their own stream beside the dynamic tree's
       // refit and queries
       const bool rec = vel && it == d->n_iters - 1;   // the velocity pass reads the last pass's pairs
+      // the edge gate of this pass: the dynamic set only when the dynamic tree's
+      // boxes come from the current x (refit this substep, or built this frame)
+      const int32_t* live_dy = (gate_dy && (refits || s == 0)) ? live_dy_list : nullptr;
+      if (ee && d->ee.gate && d->ee.live) {    // (also clears the gate's flags when neither set is gated)
+        rc = ee_gate(&d->ee, nv, (const float4*)d->x4, ycur, pad, it == 0, live_st, (int32_t*)live_dy, st);
+        if (rc) return rc;
+      }
       bool static_pending = false;
       if (nvs > 0) {
         if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
@@ -329,7 +351,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
                                  d->status, d->ws, d->ws_bytes, side->q, true, after, 1, nullptr);
         if (rc) return rc;
         if (ee) {
-          rc = ee_query(d->st, 0, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q);
+          rc = ee_query(d->st, 0, "traverse_ee_st", (const float*)ycur, d->ee.c_st, d->ee.n_st, side->q, live_st);
           if (rc) return rc;
         }
         if (hipEventRecord(side->qjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
@@ -347,7 +369,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         tv_fresh = true;
       }
       if (ee) {
-        rc = ee_query(d->dyn, 1, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st);
+        rc = ee_query(d->dyn, 1, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st, live_dy);
         if (rc) return rc;
       }
       // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369): the

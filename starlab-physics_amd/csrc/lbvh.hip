@@ -1454,12 +1454,13 @@ __global__ void k_zero_u32(uint32_t* __restrict__ a) {
 template <bool BUILD>
 int quad_first_tier(const char* name, int nq, const float* q4, float padding, int n_leaves, const float* nlo4,
                     const float* nhi4, const float* quad4, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
-                    hipStream_t st, const int32_t* order, const CacheOut& co, const int32_t* qedges) {
+                    hipStream_t st, const int32_t* order, const CacheOut& co, const int32_t* qedges,
+                    const int32_t* n_active) {
   const dim3 g(blocks_for(nq, kTravBS)), b(kTravBS);
 #define SLX_QUAD_TIER(D)                                                                                         \
   SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, qedges, n_leaves,    \
              (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
-             nullptr, order, nullptr, co)
+             nullptr, order, n_active, co)
   if (g_walk_stack <= 16) SLX_QUAD_TIER(16);
   else if (g_walk_stack <= 24) SLX_QUAD_TIER(24);
   else if (g_walk_stack <= 32) SLX_QUAD_TIER(32);
@@ -1471,9 +1472,10 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order, const int32_t* qedges, bool prezeroed) {
+                    const int32_t* order, const int32_t* qedges, bool prezeroed, const int32_t* n_active) {
   SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
   SLX_REQUIRE(!qedges || kind == 0, "slx_traverse_cached: edge queries are walked, not cached");
+  SLX_REQUIRE(!n_active || (kind == 0 && order), "slx_traverse_cached: a device query count needs a walk and a list");
   SLX_REQUIRE(c && c->redo && (kind == 0 || (c->cap >= 1 && c->leaves && c->count && c->q0)),
               "slx_traverse_cached: bad cache");
   SLX_REQUIRE(nq >= 0 && cap >= 0 && c->margin >= 0.f, "slx_traverse_cached: bad sizes");
@@ -1482,13 +1484,13 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   SLX_REQUIRE(n_leaves < (1 << 29), "slx_traverse_cached: tree too large for 30-bit stack words");
   if (n_leaves <= 0) return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4,
                                             nullptr, nullptr, nullptr, cap, cand, cand_n, status, st, nullptr,
-                                            nullptr, order, nullptr, qedges);
+                                            nullptr, order, n_active, qedges);
   SLX_REQUIRE(nlo4 && nhi4, "slx_traverse_cached: null tree");
   const bool wave = SLX_TRAV_WAVE && (long long)n_leaves * 8 <= (long long)nq;
   SLX_REQUIRE(wave || quad4, "slx_traverse_cached: the triangle tree needs its 4-wide node array");
   if (kind == 0 && wave)                      // the wavefront walk keeps its stack in registers
     return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
-                           nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order, nullptr, qedges);
+                           nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order, n_active, qedges);
   if (!prezeroed) SLX_LAUNCH("fill", k_zero_one, dim3(1), dim3(64), 0, st, c->redo);
   int rc = SLX_OK;
   if (kind == 1) {
@@ -1499,7 +1501,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                  cand_n, status, order, nullptr, co);
     else
       rc = quad_first_tier<true>("cache_build", nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n,
-                                 status, st, order, co, nullptr);
+                                 status, st, order, co, nullptr, nullptr);
   } else if (kind == 2) {
     SLX_LAUNCH(name, k_cand_filter, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4, padding,
                c->margin, (const float4*)nlo4, (const float4*)nhi4, c->cap, c->leaves, c->count,
@@ -1508,7 +1510,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
     CacheOut co{};
     co.redo = c->redo;
     rc = quad_first_tier<false>(name, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n, status, st,
-                                order, co, qedges);
+                                order, co, qedges, n_active);
   }
   if (rc) return rc;
   // the queries the first tier could not serve: a full walk over the list built on the device

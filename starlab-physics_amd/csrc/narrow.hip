@@ -394,9 +394,16 @@ extern "C" int slx_collision_pass_ee(int nv, int nvs, const int32_t* faces_dy, c
                                      const int32_t* n_tv_st, int32_t* pair_num, int pair_cap, int32_t* pair_cnt,
                                      float* pair_rec, const slx_edge_sets* ee, int rec_cap, int32_t* status,
                                      void* ws, size_t ws_bytes, slx_stream_t stream) {
+  slx_edge_sets plain;                       // the gate is slx_frame's: a standalone pass tests every edge
+  if (ee) {
+    plain = *ee;
+    plain.gate = nullptr;
+    plain.live = nullptr;
+  }
   return collision_pass_impl(nv, nvs, faces_dy, faces_st, y4, xs4, fixed, m_inv, dhat, k_col, cap, c_vt_st, n_vt_st,
-                             c_vt_dy, n_vt_dy, c_tv_st, n_tv_st, pair_num, pair_cap, pair_cnt, pair_rec, ee, rec_cap,
-                             status, ws, ws_bytes, (hipStream_t)stream, false, ZeroList{}, 0, nullptr);
+                             c_vt_dy, n_vt_dy, c_tv_st, n_tv_st, pair_num, pair_cap, pair_cnt, pair_rec,
+                             ee ? &plain : nullptr, rec_cap, status, ws, ws_bytes, (hipStream_t)stream, false,
+                             ZeroList{}, 0, nullptr);
 }
 
 extern "C" int slx_collision_pass(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,

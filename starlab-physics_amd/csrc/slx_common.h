@@ -125,7 +125,8 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false);
+                    const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false,
+                    const int32_t* n_active = nullptr);
 
 // Counters a kernel zeroes for the next pass (thread i < n of block 0 writes *p[i] = 0).
 struct ZeroList {
@@ -164,6 +165,13 @@ int ee_position_launch(int nv, const slx_edge_sets* ee, const float4* y4, const 
                        hipStream_t st);
 int ee_velocity_launch(int nv, const slx_edge_sets* ee, const float4* v4, const float4* vs4, const float* fixed,
                        const float* m_inv, const ContactWs& w, int bucket, int32_t* status, hipStream_t st);
+
+// the edge gate (edges.hip): the static maxima once per frame, then per pass the
+// dynamic maxima and the lists of edges whose pairs could act (null list: the set
+// is not gated)
+int ee_gate_static(const slx_edge_sets* ee, int nes, const float4* xs4, hipStream_t st);
+int ee_gate(const slx_edge_sets* ee, int nv, const float4* x4, const float4* y4, float pad, bool new_x,
+            int32_t* live_st, int32_t* live_dy, hipStream_t st);
 
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);

@@ -36,6 +36,10 @@ Extensions (all default to the reference's behaviour):
   ee_pair_cache_size (None)        edge pairs kept per edge and set for the
                                    velocity pass (None = cache_size, the
                                    reference's 40-wide fields)
+  edge_gate (True)                 skip the edges none of whose pairs could act
+                                   (an exact bound on the type-8 condition;
+                                   triangle trees); False walks and tests every
+                                   edge
   status()                         device status word (overflow flags)
 `step` is an alias of `forward`.
 """
@@ -92,6 +96,7 @@ class Solver:
         self._sew = None
         self.enable_edge_edge = False
         self.ee_pair_cache_size = None
+        self.edge_gate = True
         self.compliance_collision = 1e8          # XPBD.py:367
         self.records_per_vertex = 32
         self.method = "jacobi"                   # or "gs": graph-coloured Gauss-Seidel
@@ -230,7 +235,9 @@ class Solver:
                  n_st=torch.zeros(max(ne, 1), **i32) if s is not None else None,
                  c_dy=torch.full((max(cap * ne, 1),), -1, **i32), n_dy=torch.zeros(max(ne, 1), **i32),
                  pair_num=torch.zeros(max(2 * ne, 1), **i32), order=m.edge_order(),
-                 redo=torch.zeros(2 * (1 + ne), **i32) if self.bvh_mode == "tri" else None)
+                 redo=torch.zeros(2 * (1 + ne), **i32) if self.bvh_mode == "tri" else None,
+                 gate=torch.zeros(8, **i32), live=torch.zeros(2 * (1 + ne), **i32),
+                 nes=s.num_edges if s is not None else 0)
         if self.enable_velocity_update:
             e["pair_cnt"] = torch.zeros(max(2 * ne, 1), **i32)
             e["pair_rec"] = torch.zeros(max(16 * pcap * 2 * ne, 16), dtype=torch.float32, device=self.device)
@@ -317,7 +324,9 @@ class Solver:
                 d.ee = _abi.EdgeSets(e["ne"], P(m.edge_indices), P(s.edge_indices) if s is not None else None,
                                      P(e["fe_dy"]), P(e["fe_st"]), e["cap"], P(e["c_st"]), P(e["n_st"]),
                                      P(e["c_dy"]), P(e["n_dy"]), e["pcap"], P(e["pair_num"]), P(e.get("pair_cnt")),
-                                     P(e.get("pair_rec")), P(e["order"]), P(e["redo"]))
+                                     P(e.get("pair_rec")), P(e["order"]), P(e["redo"]),
+                                     P(e["gate"]) if self.edge_gate else None,
+                                     P(e["live"]) if self.edge_gate else None, e["nes"])
         if self.enable_sewing and self._sew is not None and self._sew["np"] > 0:
             sw = self._sew
             dt_sub = self.dt / n_substeps

@@ -338,3 +338,51 @@ def test_frames_with_edge_edge_shallow_walk(slx):
             np.testing.assert_array_equal(cloth.positions(), ref.x)
     finally:
         A.call("slx_set_walk_stack", 32)
+
+
+@pytest.mark.parametrize("velocity", [False, True])
+def test_frames_with_edge_gate_on_fine_cloth(slx, velocity):
+    """A 64 x 64 cloth (edges of 1.6-2.2 cm) falling on a sphere, padding 0.02: the
+    edge gate's bound |u| LY (|u| + sqrt(3) (LX + 2 pad + D)) stays below 1e-4
+    for every cloth edge while the cloth falls freely, so the dynamic edge set is
+    skipped whole; once the sphere pushes vertices, D (the pass's largest y - x)
+    lets part of the set through.  Every frame equals the oracle's, which runs
+    every edge-edge test."""
+    import torch
+    from oracle import oracle as orc
+    g = slx.scene.grid_cloth(64, size=1.0, height=0.35, jitter=1e-3, seed=0)
+    cloth = slx.Mesh(x=g.x, faces=g.faces)
+    ico = slx.scene.icosphere(3, 0.3)
+    obst = slx.Mesh(x=ico.x, faces=ico.faces, is_static=True)
+    sim = slx.Solver(cloth, obst, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3,
+                     g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+    sim.enable_collision_handling = True
+    sim.enable_edge_edge = True
+    sim.enable_velocity_update = velocity
+    sim.bvh_mode = "tri"
+    sim.refit_every_substep = True
+    sim.padding = 0.02
+    sel = torch.zeros(cloth.num_verts, device="cuda")
+    sel[0] = 1
+    sel[63] = 1
+    sim.set_fixed_vertices(sel)
+    st = dict(x=obst.positions(), faces=obst.fid_np, m_inv=obst.m_inv_np, edges=obst.eid_np,
+              fe=slx.topology.face_edges(obst.fid_np, obst.eid_np))
+    ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
+                          stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
+                          collision=True, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
+                          velocity=velocity, mu=sim.mu, edge_edge=True)
+    ne = cloth.num_edges
+    for frame in range(8):
+        print(f"[ee gate] velocity={velocity} frame={frame}", flush=True)
+        sim.forward(5)
+        ref.forward(5)
+        np.testing.assert_array_equal(cloth.positions(), ref.x)
+        live = sim._cd["ee"]["live"].cpu().numpy()
+        print(f"[ee gate] live edges: static set {live[0]}, dynamic set {live[1 + ne]} of {ne}", flush=True)
+        if frame < 2:                             # falling freely, not yet at the sphere
+            assert live[1 + ne] == 0, "every cloth edge is gated while the cloth falls"
+        assert 0 < live[0] <= ne                  # the sphere's edges are long: the static set runs
+        np.testing.assert_array_equal(sim.edge_pairs()["num"].cpu().numpy(),
+                                      np.concatenate([ref.last_ee["pn_st"], ref.last_ee["pn_dy"]]))
+    assert ref.last_ee["pn_dy"].sum() == 0
