@@ -186,3 +186,33 @@ def test_morton_order_is_a_permutation_with_locality(slx):
     row_spans = [np.ptp(g.x[i:i + 64, [0, 2]], axis=0).max() for i in range(0, len(p) - 63, 64)]
     assert np.median(spans) < 0.5 * np.median(row_spans)
     assert topo.morton_order(np.zeros((0, 3))).shape == (0,)
+
+
+def test_face_edges_and_edge_order(slx, rng):
+    """face_edges: every face's sides (v0v1, v1v2, v2v0) are found among the
+    sorted unique edges; each interior edge is a side of exactly two faces and a
+    boundary edge of one.  edge_order: a permutation of the edges, in
+    non-decreasing rank (in the given vertex order) of their first vertex, stable."""
+    n = 9
+    g = slx.scene.grid_cloth(n, size=1.0, height=0.0)
+    f = g.faces
+    e = slx.topology.edges_from_faces(f)
+    fe = slx.topology.face_edges(f, e)
+    for k, (a, b) in enumerate(((0, 1), (1, 2), (2, 0))):
+        lo = np.minimum(f[:, a], f[:, b])
+        hi = np.maximum(f[:, a], f[:, b])
+        np.testing.assert_array_equal(e[fe[:, k]], np.stack([lo, hi], axis=1))
+    uses = np.bincount(fe.ravel(), minlength=len(e))
+    boundary = 4 * (n - 1)
+    assert (uses == 1).sum() == boundary and (uses == 2).sum() == len(e) - boundary
+    vo = rng.permutation(n * n).astype(np.int32)
+    order = slx.topology.edge_order(e, vo)
+    np.testing.assert_array_equal(np.sort(order), np.arange(len(e)))
+    rank = np.empty(n * n, np.int64)
+    rank[vo] = np.arange(n * n)
+    r = rank[e[order, 0]]
+    assert np.all(np.diff(r) >= 0)
+    same = np.diff(r) == 0
+    assert np.all(np.diff(order)[same] > 0)          # stable within a first vertex
+    with pytest.raises(ValueError):
+        slx.topology.face_edges(f, e[::-1])          # edges must be sorted
