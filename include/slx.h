@@ -247,7 +247,8 @@ int slx_seg_build(int nv, int nf, int n_inst, const int32_t* faces, const float*
 typedef struct slx_cand_cache {
   int32_t cap;                           /* leaves stored per query */
   float margin;                          /* enlargement of the query box (length) */
-  int32_t* leaves;                       /* [cap * nq] leaf node ids, column-major */
+  int32_t* leaves;                       /* leaf node ids: int4 [(cap + 3) / 4][nq] (16-byte aligned,
+                                            4 ceil(cap/4) nq int32), four per processing slot and row */
   int32_t* count;                        /* [nq] leaves found (> cap or INT_MAX: incomplete) */
   float* q0;                             /* float4 [nq] query positions at the build */
   int32_t* redo;                         /* [nq + 1] queries walked in full ([0] = how many) */

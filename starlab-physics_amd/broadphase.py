@@ -41,7 +41,8 @@ class CandidateCache:
         """cap 0: no cache, only the redo list of the two-tier walk."""
         i32 = dict(dtype=torch.int32, device=device)
         self.nq, self.cap, self.margin = nq, cap, margin
-        self.leaves = torch.zeros(max(cap * nq, 1), **i32) if cap > 0 else None
+        # int4 rows of four leaves per query: [(cap + 3) // 4][nq]
+        self.leaves = torch.zeros(max(4 * ((cap + 3) // 4) * nq, 4), **i32) if cap > 0 else None
         self.count = torch.zeros(max(nq, 1), **i32) if cap > 0 else None
         self.q0 = torch.zeros((max(nq, 1), 4), dtype=torch.float32, device=device) if cap > 0 else None
         self.redo = torch.zeros(nq + 1, **i32)

@@ -1059,6 +1059,26 @@ struct CacheOut {
   int32_t* redo;           // [1 + nq]
 };
 
+// Stored leaves go out four at a time: a lane collects them in an int4 and writes
+// it to row (i / 4) of the int4 array [(cap + 3) / 4][nq] when it is full (and
+// the partial last one at the end), so a store covers 16 bytes, not 4.
+struct LeafPack {
+  int4 v = make_int4(0, 0, 0, 0);
+  __device__ __forceinline__ void put(int i, int node, int cap, int nq, int t, int32_t* leaves) {
+    if (i >= cap) return;
+    const int s = i & 3;
+    v.x = s == 0 ? node : v.x;
+    v.y = s == 1 ? node : v.y;
+    v.z = s == 2 ? node : v.z;
+    v.w = s == 3 ? node : v.w;
+    if (s == 3) reinterpret_cast<int4*>(leaves)[(size_t)(i >> 2) * nq + t] = v;
+  }
+  __device__ __forceinline__ void finish(int n, int cap, int nq, int t, int32_t* leaves) {
+    const int m = n < cap ? n : cap;
+    if (m & 3) reinterpret_cast<int4*>(leaves)[(size_t)(m >> 2) * nq + t] = v;
+  }
+};
+
 constexpr uint32_t kExactBit = 1u << 31;   // quad walk, cache build: with kEmit, the leaf overlaps the exact box
 constexpr uint32_t kCacheNode = 1u << 31;  // quad walk, cache build: without kEmit, a leaf node for the cache
 
@@ -1098,9 +1118,10 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
     ++cnt;
   };
   // a leaf reached by the walk: to the cache (BUILD) and, if it overlaps the exact box, to the list
+  LeafPack pack;
   auto leaf = [&](int node, const float4& lo, const float4& hi) {
     if (BUILD) {
-      if (ccnt < co.cap) co.leaves[(size_t)ccnt * nq + t] = node;
+      pack.put(ccnt, node, co.cap, nq, t, co.leaves);
       ++ccnt;
       if (overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
     } else {
@@ -1158,6 +1179,7 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
   if (!active) return;
   if (BUILD) {
     const bool over = flags & kStatusStackOverflow;
+    pack.finish(ccnt, co.cap, nq, t, co.leaves);
     co.count[t] = over ? INT_MAX : ccnt;
     if (over) {
       co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
@@ -1231,8 +1253,9 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
     else flags |= kStatusCandOverflow;
     ++cnt;
   };
+  LeafPack pack;
   auto keep = [&](int node) {
-    if (ccnt < co.cap) co.leaves[(size_t)ccnt * nq + t] = node;
+    pack.put(ccnt, node, co.cap, nq, t, co.leaves);
     ++ccnt;
   };
   int sp = 0;
@@ -1283,7 +1306,10 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
     }
     if (co.redo && (flags & kStatusStackOverflow)) break;   // walked again by the second tier
   }
-  if (BUILD) co.count[t] = (flags & kStatusStackOverflow) ? INT_MAX : ccnt;
+  if (BUILD) {
+    pack.finish(ccnt, co.cap, nq, t, co.leaves);
+    co.count[t] = (flags & kStatusStackOverflow) ? INT_MAX : ccnt;
+  }
   if (co.redo && (flags & kStatusStackOverflow)) {    // the exact list may be incomplete: walk it again
     co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
     return;
@@ -1394,12 +1420,18 @@ k_cand_filter(int nq, const float4* __restrict__ q4, float pad, float margin, co
   const float pp = pad * 1.0f;
   const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
   int cnt = 0;
-  for (int j = 0; j < n; ++j) {
-    const int L = cache[(size_t)j * nq + t];
-    const float4 lo = nlo[L], hi = nhi[L];
-    if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
-    if (cnt < cap) cand[(size_t)cnt * nq + q] = __float_as_int(lo.w);
-    ++cnt;
+  const int4* rows = reinterpret_cast<const int4*>(cache);
+  for (int j0 = 0; j0 < n; j0 += 4) {
+    const int4 r = rows[(size_t)(j0 >> 2) * nq + t];
+    const int Ls[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (j0 + k >= n) break;
+      const float4 lo = nlo[Ls[k]], hi = nhi[Ls[k]];
+      if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
+      if (cnt < cap) cand[(size_t)cnt * nq + q] = __float_as_int(lo.w);
+      ++cnt;
+    }
   }
   cand_n[q] = cnt < cap ? cnt : cap;
   if (cnt > cap && status) atomicOr(status, kStatusCandOverflow);

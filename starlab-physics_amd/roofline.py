@@ -9,8 +9,10 @@ candidates per query of a query set.
   predict                      40 V   (read x, v, f; write y)
   one Jacobi spring pass       12 (E+B) + 40 V   (ids + rest length; read y, x, f*w; write y)
   velocity / position update   52 V   (read x, y, f; write v, x)
-  refit (face boxes fused)     224 F + 16 V  (faces + leaf order 16 F, topology 16 F, x 16 V; write 2F-1
-                                              nodes 64 F and the 4-wide node array 128 F)
+  refit (face boxes fused)     160 F + 16 V  (faces + leaf order 16 F, topology 16 F, x 16 V; write 2F-1
+                                              nodes 64 F and the 4-wide lines of the even-depth
+                                              internal nodes, about half of them, 64 F; a batched
+                                              tree writes every line: 224 F + 16 V)
   traversal (nq queries, tree of Ft faces)   16 nq + 4 nq + 4 c nq + tree, tree = 128 (Ft - 1) + 32 for the
                                4-wide node array, 32 (2 Ft - 1) for the binary node arrays
   narrow phase                 16 c V + 24 V  (per query set of V queries)
@@ -65,8 +67,8 @@ def update_vx(V):
     return 52 * V
 
 
-def refit(V, F):
-    return 224 * F + 16 * V
+def refit(V, F, every_line=False):
+    return (224 if every_line else 160) * F + 16 * V
 
 
 def contact_apply(V):
