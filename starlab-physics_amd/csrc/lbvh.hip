@@ -624,30 +624,34 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
       if (up >= 0 && (up >> 1) < nl) s_cell[up >> 1][up & 1] = P;
     }
   }
-  if constexpr (!COUNTED) {
-    if (quad) {                            // quad lines of the top nodes, once every box is final
-      __syncthreads();
-      auto grand = [&](float4* d, int g) {
-        const float4 lo = nlo[g], hi = nhi[g];
-        d[0] = make_float4(lo.x, lo.y, lo.z, __int_as_float(g));
-        d[1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(g >= n - 1 ? __float_as_int(lo.w) : -1));
-      };
-      for (int i = threadIdx.x; i < n_top; i += kTopBS) {
-        const int p = S.top_node(i);
-        float4* qp = quad + 8 * (size_t)p;
-        const int c2[2] = {ca[p], cb[p]};
-        for (int h = 0; h < 2; ++h) {
-          const int c = c2[h];
-          if (c >= n - 1) {
-            grand(qp + 4 * h, c);
-            empty_slot(qp + 4 * h + 2);
-          } else {
-            grand(qp + 4 * h, __float_as_int(nlo[c].w));
-            grand(qp + 4 * h + 2, __float_as_int(nhi[c].w));
-          }
-        }
+}
+
+// The top nodes' 4-wide lines, once every box is final: 8 threads per line (one
+// float4 each), grid-stride over the top nodes (their count is on the device).
+__global__ void __launch_bounds__(256)
+k_top_lines(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+            const float4* __restrict__ nlo, const float4* __restrict__ nhi, float4* __restrict__ quad) {
+  const Sched S{sched, n};
+  const int n_top = S.n_top();
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < 8 * n_top; t += gridDim.x * 256) {
+    const int i = t >> 3, j = t & 7, slot = j >> 1, half = j & 1;
+    const int p = S.top_node(i);
+    const int c = slot < 2 ? ca[p] : cb[p];
+    float4 v;
+    if (c >= n - 1 && (slot & 1)) {                   // a leaf child's companion slot: empty
+      const float inf = __int_as_float(0x7f800000);
+      v = half ? make_float4(-inf, -inf, -inf, __int_as_float(-1)) : make_float4(inf, inf, inf, __int_as_float(-1));
+    } else {
+      const int g = c >= n - 1 ? c : __float_as_int((slot & 1) ? nhi[c].w : nlo[c].w);
+      const float4 lo = nlo[g];
+      if (half) {
+        const float4 hi = nhi[g];
+        v = make_float4(hi.x, hi.y, hi.z, __int_as_float(g >= n - 1 ? __float_as_int(lo.w) : -1));
+      } else {
+        v = make_float4(lo.x, lo.y, lo.z, __int_as_float(g));
       }
     }
+    quad[8 * (size_t)p + j] = v;
   }
 }
 
@@ -670,6 +674,8 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
   SLX_LAUNCH(name, k_refit_sub<LEAF>, dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb, parent, num, nlo, nhi);
   SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
              src.quad);
+  if (!COUNTED && src.quad)
+    SLX_LAUNCH("refit_top", k_top_lines, dim3(64), dim3(256), 0, st, n, sched, ca, cb, nlo, nhi, src.quad);
   return check_launch(name);
 }
 
