@@ -118,6 +118,16 @@ __device__ __forceinline__ bool ee_gated(const uint32_t* __restrict__ gate, int 
   return b * 1.001f < 1e-4f;                                       // false for NaN
 }
 
+// every edge of set `set` is gated: the bound grows with |u|, and |u| <= LX +
+// 2 sqrt(3) D for every dynamic edge at y (with an allowance for rounding), so
+// when that longest possible edge is gated, each edge is
+__device__ __forceinline__ bool ee_set_gated(const uint32_t* __restrict__ gate, int set) {
+  if (!gate || !(gate[4] & (set ? 2u : 1u))) return false;
+  const float lu_max = (__uint_as_float(gate[0]) + 3.4641016f * __uint_as_float(gate[2])) * 1.0001f +
+                       ldexpf(fmaxf(__uint_as_float(gate[1]), __uint_as_float(gate[3])), -20);
+  return ee_gated(gate, set, lu_max);
+}
+
 // A vertex's bucket records so far folded into its running sum (in key order,
 // as the apply kernels would add them) and count; the bucket is emptied.
 __global__ void __launch_bounds__(256)
@@ -155,7 +165,7 @@ k_ee_contacts(EdgeArgs A, const float4* __restrict__ y4, const float4* __restric
   const int32_t* C = set ? A.c_dy : A.c_st;
   int pairs = 0, stored = 0;
   bool over = false, pover = false;
-  if (C) {
+  if (C && !ee_set_gated(A.gate, set)) {   // a gated set: no pair of it can act
     const int32_t* N = set ? A.n_dy : A.n_st;
     const int32_t* fe = set ? A.fe_dy : A.fe_st;
     const int32_t* E2 = set ? A.edges_dy : A.edges_st;
@@ -336,13 +346,13 @@ __global__ void k_gate_set(uint32_t* __restrict__ gate, int z0, int z1, uint32_t
 }
 
 // gate[slot] = longest edge of `edges` at p4, gate[slot + 1] = largest |coordinate|
-// of their end points (8 edges per thread; 256 threads)
+// of their end points (2048 edges per workgroup of 256 threads)
 __global__ void __launch_bounds__(256)
 k_gate_edges(int ne, const int32_t* __restrict__ edges, const float4* __restrict__ p4, uint32_t* gate, int slot) {
   float lm = 0.f, cm = 0.f;
-  const int base = (blockIdx.x * 256 + threadIdx.x) * 8;
+  const int base = blockIdx.x * 2048 + threadIdx.x;
   for (int i = 0; i < 8; ++i) {
-    const int e = base + i;
+    const int e = base + i * 256;
     if (e >= ne) break;
     const float3 a = xyz(p4[edges[2 * e]]), b = xyz(p4[edges[2 * e + 1]]);
     const float3 d = sub3(b, a);
@@ -355,14 +365,14 @@ k_gate_edges(int ne, const int32_t* __restrict__ edges, const float4* __restrict
   block_max2(lm, cm, gate + slot, gate + slot + 1);
 }
 
-// gate[2] = max |y - x|_inf, gate[3] = max |y| over the dynamic vertices (8 per
-// thread; 256 threads)
+// gate[2] = max |y - x|_inf, gate[3] = max |y| over the dynamic vertices (2048 per
+// workgroup of 256 threads)
 __global__ void __launch_bounds__(256)
 k_gate_disp(int nv, const float4* __restrict__ x4, const float4* __restrict__ y4, uint32_t* gate) {
   float dm = 0.f, cm = 0.f;
-  const int base = (blockIdx.x * 256 + threadIdx.x) * 8;
+  const int base = blockIdx.x * 2048 + threadIdx.x;
   for (int i = 0; i < 8; ++i) {
-    const int v = base + i;
+    const int v = base + i * 256;
     if (v >= nv) break;
     const float4 x = x4[v], y = y4[v];
     const float d = fmaxf(fmaxf(fabsf(y.x - x.x), fabsf(y.y - x.y)), fabsf(y.z - x.z));
@@ -378,13 +388,7 @@ k_gate_disp(int nv, const float4* __restrict__ x4, const float4* __restrict__ y4
 __global__ void __launch_bounds__(256)
 k_ee_gate(int ne, const int32_t* __restrict__ order, const int32_t* __restrict__ edges, const float4* __restrict__ y4,
           const uint32_t* __restrict__ gate, int32_t* __restrict__ live_st, int32_t* __restrict__ live_dy) {
-  // the bound grows with |u| <= LX + 2 sqrt(3) D: when the longest possible edge
-  // is gated in both sets (with an allowance for rounding), every edge is
-  {
-    const float lu_max = (__uint_as_float(gate[0]) + 3.4641016f * __uint_as_float(gate[2])) * 1.0001f +
-                         ldexpf(fmaxf(__uint_as_float(gate[1]), __uint_as_float(gate[3])), -20);
-    if ((!live_st || ee_gated(gate, 0, lu_max)) && (!live_dy || ee_gated(gate, 1, lu_max))) return;
-  }
+  if ((!live_st || ee_set_gated(gate, 0)) && (!live_dy || ee_set_gated(gate, 1))) return;
   const int t = blockIdx.x * 256 + threadIdx.x;
   const bool active = t < ne;
   const int e = active ? (order ? order[t] : t) : 0;
