@@ -248,6 +248,10 @@ __global__ void k_fill_i32(int n, int32_t* __restrict__ p, int32_t v) {
 constexpr int kSubMax = 2048;   // leaves of one subtree (LDS of k_refit_sub)
 constexpr int kSubBS = 1024;
 constexpr int kSubGrid = 2048;
+// subtrees of at most 1024 leaves run in a half-size workgroup (77 KB of LDS, so
+// two fit on a CU and one's loads overlap the other's walk)
+constexpr int kSubHalf = 1024;
+constexpr int kSubHalfBS = 512;
 constexpr int kTopMax = 2048;   // top nodes with LDS state (LDS of k_refit_top)
 constexpr int kTopBS = 1024;
 int g_refit_sub = kSubMax;
@@ -472,22 +476,22 @@ __device__ __forceinline__ NodeBox leaf_box(int k, int n, const LeafSrc& s, int3
 // parents and the subtree's internal nodes' children and parents (all contiguous
 // index ranges) into LDS; phase 2 walks up from every leaf, the second child to
 // arrive at a node refitting it, touching global memory only to store boxes.
-template <int LEAF>
-__global__ void __launch_bounds__(kSubBS)
+template <int LEAF, int SUBMAX, int BS>
+__global__ void __launch_bounds__(BS)
 k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t* __restrict__ ca,
             const int32_t* __restrict__ cb, const int32_t* __restrict__ parent, int32_t* num, float4* nlo,
             float4* nhi) {
   constexpr bool COUNTED = LEAF == kLeafCells;
-  __shared__ NodeBox s_leaf[kSubMax];
-  __shared__ NodeBox s_int[kSubMax];
-  __shared__ int s_flag[kSubMax];
-  __shared__ int s_ca[kSubMax], s_cb[kSubMax], s_par[kSubMax], s_lface[kSubMax];
+  __shared__ NodeBox s_leaf[SUBMAX];
+  __shared__ NodeBox s_int[SUBMAX];
+  __shared__ int s_flag[SUBMAX];
+  __shared__ int s_ca[SUBMAX], s_cb[SUBMAX], s_par[SUBMAX], s_lface[SUBMAX];
   const Sched S{sched, n};
   const int n_sub = S.n_sub();
   for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
     const int4 e = S.sub(si);
     const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
-    for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+    for (int k = threadIdx.x; k < cnt; k += BS) {
       s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
       if constexpr (!COUNTED) s_lface[k] = src.sorted[first + k];
       s_flag[k] = 0;
@@ -498,7 +502,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
       }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+    for (int k = threadIdx.x; k < cnt; k += BS) {
       int node = n - 1 + first + k;
       if (node == root) continue;
       int p = parent[node];
@@ -517,12 +521,12 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     __syncthreads();
     // the subtree's internal nodes (both children arrived: flag 2) to global memory,
     // coalesced: node boxes one node per thread, quad lines 8 threads per line
-    for (int k = threadIdx.x; k < cnt; k += kSubBS)
+    for (int k = threadIdx.x; k < cnt; k += BS)
       if (s_flag[k] == 2) store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
     if constexpr (!COUNTED) {
       if (src.quad && S.even_lines()) {      // odd-depth nodes' lines are never read: flag 2 -> 1
         const int rp = S.sub_parity(si);
-        for (int k = threadIdx.x; k < cnt; k += kSubBS) {
+        for (int k = threadIdx.x; k < cnt; k += BS) {
           if (s_flag[k] != 2) continue;
           int x = first + k, d = rp;
           while (x != root) {
@@ -537,7 +541,7 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
         auto box_of = [&](int g) -> const NodeBox& {
           return g >= n - 1 ? s_leaf[g - (n - 1) - first] : s_int[g - first];
         };
-        for (int t = threadIdx.x; t < 8 * cnt; t += kSubBS) {
+        for (int t = threadIdx.x; t < 8 * cnt; t += BS) {
           const int k = t >> 3, j = t & 7;
           if (s_flag[k] != 2) continue;
           const int slot = j >> 1, half = j & 1;
@@ -670,8 +674,15 @@ template <int LEAF>
 int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, const int32_t* ca, const int32_t* cb,
           const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st) {
   constexpr bool COUNTED = LEAF == kLeafCells;
-  const int grid = n < kSubGrid ? n : kSubGrid;
-  SLX_LAUNCH(name, k_refit_sub<LEAF>, dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb, parent, num, nlo, nhi);
+  if (g_refit_sub <= kSubHalf) {
+    const int grid = n < 2 * kSubGrid ? n : 2 * kSubGrid;
+    SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubHalf, kSubHalfBS>), dim3(grid), dim3(kSubHalfBS), 0, st, n, sched, src,
+               ca, cb, parent, num, nlo, nhi);
+  } else {
+    const int grid = n < kSubGrid ? n : kSubGrid;
+    SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubMax, kSubBS>), dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb,
+               parent, num, nlo, nhi);
+  }
   SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
              src.quad);
   if (!COUNTED && src.quad)
