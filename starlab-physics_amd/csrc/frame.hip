@@ -39,7 +39,7 @@ bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
 // the dynamic-tree queries.
 struct SideStream {
   hipStream_t s = nullptr, q = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr, sub = nullptr;
 };
 SideStream g_side[64];
 
@@ -53,7 +53,8 @@ int side_stream(SideStream** out) {
         hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ss.qfork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.sub, hipEventDisableTiming) != hipSuccess)
       return fail(SLX_EHIP, "slx_frame: side stream creation failed");
   }
   *out = &ss;
@@ -284,9 +285,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       // fork: the refit reads x as the previous substep left it
       if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: fork failed");
-      rc = slx_tri_refit(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
-                         d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4,
-                         (slx_stream_t)side->s);
+      rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
+                          d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub);
       if (rc) return rc;
       if (dyn_cached) {                      // how far the refit boxes' vertices moved since the caches
         rc = tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, !snapped, side->s);
@@ -342,9 +342,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (nvs > 0) {
         if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
-        // and after the refit: beside it their workgroups hold the CUs whose
-        // whole LDS each refit workgroup needs (refit 333 -> 203 us, frame -4.7%)
-        if (refit_pending && hipStreamWaitEvent(side->q, side->join, 0) != hipSuccess)
+        // and after the refit's subtree kernel: beside it their workgroups hold
+        // the CUs whose whole LDS each refit workgroup needs (refit 333 -> 203
+        // us, frame -4.7%); its top part is one workgroup
+        if (refit_pending && hipStreamWaitEvent(side->q, side->sub, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
                    n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);

@@ -670,9 +670,12 @@ int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_
   return check_launch("refit_schedule");
 }
 
+// after_sub (nullable) is recorded once the subtrees are done: the top part that
+// follows is one workgroup, so other streams may take the GPU from there
 template <int LEAF>
 int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, const int32_t* ca, const int32_t* cb,
-          const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st) {
+          const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st,
+          hipEvent_t after_sub = nullptr) {
   constexpr bool COUNTED = LEAF == kLeafCells;
   if (g_refit_sub <= kSubHalf) {
     const int grid = n < 2 * kSubGrid ? n : 2 * kSubGrid;
@@ -683,6 +686,7 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
     SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubMax, kSubBS>), dim3(grid), dim3(kSubBS), 0, st, n, sched, src, ca, cb,
                parent, num, nlo, nhi);
   }
+  if (after_sub && hipEventRecord(after_sub, st) != hipSuccess) return fail(SLX_EHIP, "refit: event record failed");
   SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
              src.quad);
   if (!COUNTED && src.quad)
@@ -934,6 +938,13 @@ extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, floa
                              const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b,
                              const int32_t* parent, const int32_t* sched, float* nlo4, float* nhi4, float* quad4,
                              slx_stream_t stream) {
+  return tri_refit_impl(nf, faces, x4, padding, sorted_faces, child_a, child_b, parent, sched, nlo4, nhi4, quad4,
+                        S(stream), nullptr);
+}
+
+int slx::tri_refit_impl(int nf, const int32_t* faces, const float* x4, float padding,
+                        const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b, const int32_t* parent, const int32_t* sched,
+                   float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub) {
   SLX_REQUIRE(nf >= 0, "slx_tri_refit: nf < 0");
   if (nf == 0) return SLX_OK;
   SLX_REQUIRE(faces && x4 && sorted_faces && child_a && child_b && parent && sched && nlo4 && nhi4,
@@ -945,7 +956,7 @@ extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, floa
   src.sorted = sorted_faces;
   src.quad = (float4*)quad4;
   return refit<kLeafVertices>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
-                              (float4*)nhi4, S(stream));
+                              (float4*)nhi4, st, after_sub);
 }
 
 extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6, int32_t* codes,
