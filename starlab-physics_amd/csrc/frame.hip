@@ -39,7 +39,7 @@ bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
 // the dynamic-tree queries.
 struct SideStream {
   hipStream_t s = nullptr, q = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr, sub = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr, sub = nullptr, tvjoin = nullptr;
 };
 SideStream g_side[64];
 
@@ -54,7 +54,8 @@ int side_stream(SideStream** out) {
         hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ss.qfork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.sub, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ss.sub, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.tvjoin, hipEventDisableTiming) != hipSuccess)
       return fail(SLX_EHIP, "slx_frame: side stream creation failed");
   }
   *out = &ss;
@@ -270,7 +271,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     if (rc) return rc;
   }
   for (int s = 0; s < d->n_substeps; ++s) {
-    bool refit_pending = false, snapped = false;
+    bool refit_pending = false, snapped = false, disp_pending = false;
     if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
       // caches rebuilt in this substep, against x as it is now (the refit below
       // builds the dynamic tree's boxes from this x)
@@ -288,10 +289,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
                           d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub);
       if (rc) return rc;
-      if (dyn_cached) {                      // how far the refit boxes' vertices moved since the caches
-        rc = tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, !snapped, side->s);
-        if (rc) return rc;
-      }
+      disp_pending = dyn_cached;             // measured on the main stream after the first spring pass
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
       refit_pending = true;
       tv_fresh = false;
@@ -338,7 +336,12 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         rc = ee_gate(&d->ee, nv, (const float4*)d->x4, ycur, pad, it == 0, live_st, (int32_t*)live_dy, st);
         if (rc) return rc;
       }
-      bool static_pending = false;
+      if (disp_pending) {                    // how far the refit boxes' vertices moved since the caches
+        rc = tree_displacement(nv, d->x4, d->dyn.x_snap, d->dyn.disp, !snapped, st);
+        if (rc) return rc;
+        disp_pending = false;
+      }
+      bool static_pending = false, tv_side = false;
       if (nvs > 0) {
         if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
@@ -350,6 +353,15 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
                    n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
         if (rc) return rc;
+        if (!tv_fresh) {                     // obstacle vertices vs the dynamic tree, beside its cache build
+          if (refit_pending && hipStreamWaitEvent(side->q, side->join, 0) != hipSuccess)
+            return fail(SLX_EHIP, "slx_frame: fork failed");
+          rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
+                     d->status, (slx_stream_t)side->q, &d->cache[2], reuse_t, &built[2], d->dyn.disp);
+          if (rc) return rc;
+          if (hipEventRecord(side->tvjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
+          tv_fresh = tv_side = true;
+        }
         rc = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                                  d->k_col, cap, c0, n0, c1, n1, c2, n2, pn, rec ? d->pair_cap : 0,
                                  rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr, nullptr, d->rec_cap,
@@ -367,12 +379,8 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
                  n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp);
       if (rc) return rc;
-      if (nvs > 0 && !tv_fresh) {
-        rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
-                   d->status, stream, &d->cache[2], reuse_t, &built[2], d->dyn.disp);
-        if (rc) return rc;
-        tv_fresh = true;
-      }
+      if (tv_side && hipStreamWaitEvent(st, side->tvjoin, 0) != hipSuccess)
+        return fail(SLX_EHIP, "slx_frame: join failed");
       if (ee) {
         rc = ee_query(d->dyn, 1, "traverse_ee_dy", (const float*)ycur, d->ee.c_dy, d->ee.n_dy, st, live_dy);
         if (rc) return rc;
