@@ -623,10 +623,22 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
         B = load_node<COUNTED>(r, num, nlo, nhi);
       }
       const NodeBox P = unite<COUNTED>(A, B);
-      store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
       up = lds ? s_up[slot] : S.top_up(slot);
-      if (up >= 0 && (up >> 1) < nl) s_cell[up >> 1][up & 1] = P;
+      const bool up_lds = up >= 0 && (up >> 1) < nl;
+      // a node whose parent reads the node arrays stores now; the others are
+      // stored after the walk (a store here would make every level's fence wait
+      // for it)
+      if (!lds || !up_lds) store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
+      if (up_lds) s_cell[up >> 1][up & 1] = P;
     }
+  }
+  __syncthreads();
+  // the deferred nodes: both child cells are final, and uniting them again gives
+  // the same box
+  for (int i = threadIdx.x; i < nl; i += kTopBS) {
+    const int up = s_up[i];
+    if (up >= 0 && (up >> 1) < nl)
+      store_node<COUNTED>(s_node[i], unite<COUNTED>(s_cell[i][0], s_cell[i][1]), s_l[i], s_r[i], num, nlo, nhi);
   }
 }
 
