@@ -45,12 +45,14 @@ constexpr int kTravBS = 128;     // traversal block size
 
 // computeAABB_faces, meshTaichiWrapper.py:350-361: componentwise min/max of the
 // three corners, minus/plus the padding
-__device__ __forceinline__ void face_box(const int32_t* __restrict__ faces, const float4* __restrict__ x4, int f,
-                                         float pad, float3& lo, float3& hi) {
-  float3 a = xyz(x4[faces[3 * f]]), b = xyz(x4[faces[3 * f + 1]]), c = xyz(x4[faces[3 * f + 2]]);
+__device__ __forceinline__ void corner_box(float3 a, float3 b, float3 c, float pad, float3& lo, float3& hi) {
   const float p = pad * 1.0f;
   lo = make_float3(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p, fminf(fminf(a.z, b.z), c.z) - p);
   hi = make_float3(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p, fmaxf(fmaxf(a.z, b.z), c.z) + p);
+}
+__device__ __forceinline__ void face_box(const int32_t* __restrict__ faces, const float4* __restrict__ x4, int f,
+                                         float pad, float3& lo, float3& hi) {
+  corner_box(xyz(x4[faces[3 * f]]), xyz(x4[faces[3 * f + 1]]), xyz(x4[faces[3 * f + 2]]), pad, lo, hi);
 }
 
 __global__ void k_face_aabbs(int nf, const int32_t* __restrict__ faces, const float4* __restrict__ x4,
@@ -491,9 +493,36 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
   for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
     const int4 e = S.sub(si);
     const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
+    if constexpr (LEAF == kLeafVertices) {
+      // two leaves per thread, every load issued before any is used (each leaf is
+      // three dependent gathers: face id, corner ids, corner positions)
+      for (int k = threadIdx.x; k < cnt; k += 2 * BS) {
+        const int k2 = k + BS;
+        const bool two = k2 < cnt;
+        const int f1 = src.sorted[first + k], f2 = two ? src.sorted[first + k2] : f1;
+        const int a1 = src.faces[3 * f1], b1 = src.faces[3 * f1 + 1], c1 = src.faces[3 * f1 + 2];
+        const int a2 = src.faces[3 * f2], b2 = src.faces[3 * f2 + 1], c2 = src.faces[3 * f2 + 2];
+        const float4 pa1 = src.x4[a1], pb1 = src.x4[b1], pc1 = src.x4[c1];
+        const float4 pa2 = src.x4[a2], pb2 = src.x4[b2], pc2 = src.x4[c2];
+        NodeBox B1, B2;
+        corner_box(xyz(pa1), xyz(pb1), xyz(pc1), src.pad, B1.lo, B1.hi);
+        corner_box(xyz(pa2), xyz(pb2), xyz(pc2), src.pad, B2.lo, B2.hi);
+        B1.num = B2.num = 1;
+        s_leaf[k] = B1;
+        s_lface[k] = f1;
+        store_node<false>(n - 1 + first + k, B1, f1, f1, num, nlo, nhi);
+        if (two) {
+          s_leaf[k2] = B2;
+          s_lface[k2] = f2;
+          store_node<false>(n - 1 + first + k2, B2, f2, f2, num, nlo, nhi);
+        }
+      }
+    }
     for (int k = threadIdx.x; k < cnt; k += BS) {
-      s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
-      if constexpr (!COUNTED) s_lface[k] = src.sorted[first + k];
+      if constexpr (LEAF != kLeafVertices) {
+        s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
+        if constexpr (!COUNTED) s_lface[k] = src.sorted[first + k];
+      }
       s_flag[k] = 0;
       if (first + k < n - 1) {          // internal node first+k (those outside the subtree are never read)
         s_ca[k] = ca[first + k];
