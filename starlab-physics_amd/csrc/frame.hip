@@ -271,7 +271,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     if (rc) return rc;
   }
   for (int s = 0; s < d->n_substeps; ++s) {
-    bool refit_pending = false, snapped = false, disp_pending = false;
+    bool refit_pending = false, snapped = false, disp_pending = false, v_done = false;
     if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
       // caches rebuilt in this substep, against x as it is now (the refit below
       // builds the dynamic tree's boxes from this x)
@@ -387,13 +387,17 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       }
       // narrow phase + projection + update_dx (XPBD.py:236-281, 365-369): the
       // other two sets here, then (after the static stream's join) the edge sets
-      // and the apply
+      // and the apply; the substep's last apply also computes the velocity when
+      // nothing moves y after it
+      const bool fuse_v = vel && !sew && it == d->n_iters - 1;
+      const VelFuse vfuse{(const float4*)d->x4, (float4*)d->v4, d->damping, d->dt};
+      v_done = v_done || fuse_v;
       rc = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                                d->k_col, cap, nvs > 0 ? c0 : nullptr, n0, c1, n1, nvs > 0 ? c2 : nullptr, n2, pn,
                                rec ? d->pair_cap : 0, rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr,
                                ee ? (rec ? &ee_rec : &ee_norec) : nullptr, d->rec_cap, d->status, d->ws,
                                d->ws_bytes, st, true, after, static_pending ? 2 : 0,
-                               static_pending ? side->qjoin : nullptr);
+                               static_pending ? side->qjoin : nullptr, fuse_v ? &vfuse : nullptr);
       if (rc) return rc;
       if (sew) {                             // sewing constraints (XPBD.py:371-372, there commented out)
         rc = slx_sewing_pass(d->n_sewn, d->sew_verts, d->sew_off, d->sew_list, d->sew_pairs, (float*)ycur, d->fixed,
@@ -404,8 +408,10 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
     rc = join_refit();                       // (n_iters == 0: nothing queried the tree)
     if (rc) return rc;
     if (vel) {
-      rc = slx_compute_velocity(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
-      if (rc) return rc;
+      if (!v_done) {
+        rc = slx_compute_velocity(nv, d->x4, d->v4, (const float*)ycur, d->fixed, d->damping, d->dt, stream);
+        if (rc) return rc;
+      }
       rc = velocity_pass_impl(nv, nvs, d->faces, d->faces_st, d->v4, d->vs4, d->fixed, d->m_inv, d->m_inv_st,
                               d->mu, d->pair_cap, coll ? d->pair_cnt : nullptr, d->pair_rec,
                               ee ? &ee_rec : nullptr, d->x4, d->dt, d->rec_cap > 0 ? d->rec_cap : 1, d->status,

@@ -282,10 +282,12 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
 // update_dx over the contacts: y += f * (dx / nc) (XPBD.py:383-387); the bucket
 // is summed in increasing serial order.  The bucket's counter is left at zero for
 // the next pass, and so are the counters in `after`.
+// VEL: also compute_velocity from the final y (the substep's last pass).
+template <bool VEL>
 __global__ void __launch_bounds__(256)
 k_contact_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __restrict__ brec,
                 const float4* __restrict__ self_acc, const float* __restrict__ fixed, float4* __restrict__ y4,
-                ZeroList after) {
+                ZeroList after, VelFuse vf) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && (int)threadIdx.x < after.n) *after.p[threadIdx.x] = 0;
   if (v >= nv) return;
@@ -294,11 +296,19 @@ k_contact_apply(int nv, int bucket, int32_t* __restrict__ bcnt, const float4* __
   if (n != 0) bcnt[v] = 0;
   if (n > bucket) n = bucket;
   const float nc = sa.w + (float)n;
-  if (!(nc > 0.f)) return;
-  const float3 dx = bucket_sum(make_float3(sa.x, sa.y, sa.z), brec + (size_t)v * bucket, n);
+  if (!VEL && !(nc > 0.f)) return;
   const float f = fixed[v];
-  const float4 y = y4[v];
-  y4[v] = make_float4(y.x + f * (dx.x / nc), y.y + f * (dx.y / nc), y.z + f * (dx.z / nc), y.w);
+  float4 y = y4[v];
+  if (nc > 0.f) {
+    const float3 dx = bucket_sum(make_float3(sa.x, sa.y, sa.z), brec + (size_t)v * bucket, n);
+    y = make_float4(y.x + f * (dx.x / nc), y.y + f * (dx.y / nc), y.z + f * (dx.z / nc), y.w);
+    y4[v] = y;
+  }
+  if (VEL) {                                // compute_velocity on the final y
+    const float4 x = vf.x4[v];
+    const float af = (1.0f - vf.damping) * f;
+    vf.v4[v] = make_float4((af * (y.x - x.x)) / vf.dt, (af * (y.y - x.y)) / vf.dt, (af * (y.z - x.z)) / vf.dt, f);
+  }
 }
 
 __global__ void k_zero_i32(int n, int32_t* __restrict__ a) {
@@ -335,7 +345,7 @@ int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t*
                         const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
                         int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee, int rec_cap,
                         int32_t* status, void* ws, size_t ws_bytes, hipStream_t st, bool bcnt_zeroed,
-                        const ZeroList& after, int set0, hipEvent_t before_apply) {
+                        const ZeroList& after, int set0, hipEvent_t before_apply, const VelFuse* vel) {
   SLX_REQUIRE(nv >= 0 && nvs >= 0 && cap >= 0 && rec_cap >= 1, "slx_collision_pass: bad sizes");
   SLX_REQUIRE(set0 >= 0 && set0 <= 2, "slx_collision_pass: bad first-set mode");
   if (ee) {
@@ -381,8 +391,12 @@ int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t*
                                 status, st);
     if (rc) return rc;
   }
-  SLX_LAUNCH("contact_apply", k_contact_apply, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap, w.bcnt,
-             w.brec, w.self_acc, fixed, (float4*)y4, after);
+  if (vel)
+    SLX_LAUNCH("contact_apply", k_contact_apply<true>, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap,
+               w.bcnt, w.brec, w.self_acc, fixed, (float4*)y4, after, *vel);
+  else
+    SLX_LAUNCH("contact_apply", k_contact_apply<false>, dim3(blocks_for(nv, 256)), dim3(256), 0, st, nv, rec_cap,
+               w.bcnt, w.brec, w.self_acc, fixed, (float4*)y4, after, VelFuse{});
   return check_launch("slx_collision_pass");
 }
 }  // namespace slx

@@ -139,13 +139,21 @@ struct ZeroList {
 // apply kernel leaves them so); `after` -- counters the apply kernel zeroes for the
 // next pass (the walks' redo lists); set0 / before_apply: the first query set
 // launched on its own (1) or left to such a launch (2, joined by before_apply).
+// compute_velocity fused into the apply of a substep's last pass (its arithmetic,
+// on the final y): v = (1 - damping) * fixed * (y - x) / dt, v.w = fixed
+struct VelFuse {
+  const float4* x4;
+  float4* v4;
+  float damping, dt;
+};
 int collision_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* y4,
                         const float* xs4, const float* fixed, const float* m_inv, float dhat, float k_col, int cap,
                         const int32_t* c_vt_st, const int32_t* n_vt_st, const int32_t* c_vt_dy,
                         const int32_t* n_vt_dy, const int32_t* c_tv_st, const int32_t* n_tv_st, int32_t* pair_num,
                         int pair_cap, int32_t* pair_cnt, float* pair_rec, const slx_edge_sets* ee, int rec_cap,
                         int32_t* status, void* ws, size_t ws_bytes, hipStream_t st, bool bcnt_zeroed,
-                        const ZeroList& after, int set0 = 0, hipEvent_t before_apply = nullptr);
+                        const ZeroList& after, int set0 = 0, hipEvent_t before_apply = nullptr,
+                        const VelFuse* vel = nullptr);
 int velocity_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* faces_st, float* v4,
                        const float* vs4, const float* fixed, const float* m_inv, const float* m_inv_st, float mu,
                        int pair_cap, const int32_t* pair_cnt, const float* pair_rec, const slx_edge_sets* ee,
