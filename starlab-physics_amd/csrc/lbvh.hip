@@ -1159,9 +1159,12 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
                     const float4* __restrict__ nhi, int cap, int32_t* __restrict__ cand,
                     int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
                     const int32_t* __restrict__ n_active, CacheOut co) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  // a grid-stride loop (launches sized by n_active get a capped grid); the step is
+  // per workgroup, so every lane of a wavefront takes the same iterations
+  const int limit = n_active ? min(nq, *n_active) : nq;
+  for (int base = blockIdx.x * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
   const int lane = threadIdx.x & 63;
-  const bool active = t < nq && (!n_active || t < *n_active);
+  const bool active = t < limit;
   const int q = active ? (order ? order[t] : t) : 0;
   float3 a, b, aw, bw;
   if (active) {
@@ -1258,6 +1261,7 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
   }
   cand_n[q] = cnt < cap ? cnt : cap;
   if (flags && status) atomicOr(status, flags);
+  }(base + (int)threadIdx.x);
 }
 
 }  // namespace slx
@@ -1296,8 +1300,9 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
                 const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order,
                 const int32_t* __restrict__ n_active, CacheOut co) {
   __shared__ uint32_t s_stack[S * kTravBS];
-  const int t = blockIdx.x * kTravBS + threadIdx.x;
-  if (t >= nq || (n_active && t >= *n_active)) return;
+  // a grid-stride loop (launches sized by n_active get a capped grid)
+  const int limit = n_active ? min(nq, *n_active) : nq;
+  for (int t0 = blockIdx.x * kTravBS + threadIdx.x; t0 < limit; t0 += gridDim.x * kTravBS) [&](const int t) {
   const int q = order ? order[t] : t;
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   float3 a, b;
@@ -1385,11 +1390,21 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
   }
   cand_n[q] = cnt < cap ? cnt : cap;
   if (flags && status) atomicOr(status, flags);
+  }(t0);
 }
 
 }  // namespace slx
 
 namespace slx {
+// A launch whose query count is on the device (n_active) gets at most this many
+// workgroups and loops over the queries: with few or none active it ends in
+// microseconds instead of dispatching a workgroup per possible query.
+constexpr int kActiveGrid = 2048;
+static inline int active_grid(int nq, int bs, const int32_t* n_active) {
+  const int g = blocks_for(nq, bs);
+  return n_active && g > kActiveGrid ? kActiveGrid : g;
+}
+
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
                     const int32_t* num, const int32_t* sorted_faces, const float* nlo4, const float* nhi4,
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
@@ -1417,7 +1432,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   // roots.  The candidate lists are the same either way.
   else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq) {
     if (quad4 && SLX_TRAV_QUAD)
-      SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
+      SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(active_grid(nq, kTravBS, n_active)), dim3(kTravBS), 0, st, nq,
                  (const float4*)q4, padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
                  (const float4*)quad4, cap, cand, cand_n, status, roots, q_inst, order, n_active, CacheOut{});
     else
@@ -1426,7 +1441,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                  q_inst, order, n_active);
   }
   else
-    SLX_LAUNCH(name, k_traverse_tri_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4,
+    SLX_LAUNCH(name, k_traverse_tri_wave<false>, dim3(active_grid(nq, 256, n_active)), dim3(256), 0, st, nq, (const float4*)q4,
                padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, order,
                n_active, CacheOut{});
   return check_launch("slx_traverse");
@@ -1557,7 +1572,7 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
                     const float* nhi4, const float* quad4, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
                     hipStream_t st, const int32_t* order, const CacheOut& co, const int32_t* qedges,
                     const int32_t* n_active) {
-  const dim3 g(blocks_for(nq, kTravBS)), b(kTravBS);
+  const dim3 g(active_grid(nq, kTravBS, n_active)), b(kTravBS);
 #define SLX_QUAD_TIER(D)                                                                                         \
   SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, qedges, n_leaves,    \
              (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
