@@ -268,6 +268,11 @@ int slx_traverse_cached(int kind, int nq, const float* q4, float padding, int n_
                         int32_t* cand_n, int32_t* status, const int32_t* order, slx_stream_t stream);
 /* Stack depth of the first-tier walk: 16, 24, 32 (default) or 64 entries. */
 int slx_set_walk_stack(int depth);
+/* Workgroups (1..65535, default 2048) of a walk whose query count is on the
+ * device (the second tier, the edge gate's live lists): such a walk loops over
+ * its queries, so one with few or none ends at once.  Results do not depend on
+ * it (a tuning and test knob, no counterpart in the reference). */
+int slx_set_active_grid(int blocks);
 /* x_snap = x (float4 [nv]); disp = 0. */
 int slx_tree_snapshot(int nv, const float* x4, float* x_snap, uint32_t* disp, slx_stream_t stream);
 /* disp = max over vertices of |x - x_snap|_inf (float bits; infinity for NaN). */

@@ -57,6 +57,7 @@ SIGNATURES = {
                            PTR, PTR, PTR, STREAM],
     "slx_traverse_cached": [I32, I32, PTR, F32, I32, PTR, PTR, PTR, PTR, PTR, I32, PTR, PTR, PTR, PTR, STREAM],
     "slx_set_walk_stack": [I32],
+    "slx_set_active_grid": [I32],
     "slx_tree_snapshot": [I32, PTR, PTR, PTR, STREAM],
     "slx_tree_displacement": [I32, PTR, PTR, PTR, STREAM],
     "slx_collision_ws_bytes": [I32, I32, I32],

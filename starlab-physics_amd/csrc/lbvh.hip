@@ -1400,9 +1400,10 @@ namespace slx {
 // workgroups and loops over the queries: with few or none active it ends in
 // microseconds instead of dispatching a workgroup per possible query.
 constexpr int kActiveGrid = 2048;
+int g_active_grid = kActiveGrid;
 static inline int active_grid(int nq, int bs, const int32_t* n_active) {
   const int g = blocks_for(nq, bs);
-  return n_active && g > kActiveGrid ? kActiveGrid : g;
+  return n_active && g > g_active_grid ? g_active_grid : g;
 }
 
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
@@ -1643,6 +1644,12 @@ extern "C" int slx_traverse_cached(int kind, int nq, const float* q4, float padd
                                    const int32_t* order, slx_stream_t stream) {
   return traverse_cached("traverse_cached", kind, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cache, disp, cap,
                          cand, cand_n, status, S(stream), order, nullptr);
+}
+
+extern "C" int slx_set_active_grid(int blocks) {
+  SLX_REQUIRE(blocks >= 1 && blocks <= 65535, "slx_set_active_grid: 1..65535 workgroups");
+  g_active_grid = blocks;
+  return SLX_OK;
 }
 
 extern "C" int slx_set_walk_stack(int depth) {

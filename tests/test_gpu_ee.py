@@ -340,6 +340,22 @@ def test_frames_with_edge_edge_shallow_walk(slx):
         A.call("slx_set_walk_stack", 32)
 
 
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_device_count_walks_loop(slx, orc, blocks):
+    """Walks whose query count is on the device (the second tier's redo lists,
+    the edge gate's live lists) run on a capped grid and loop over their
+    queries; with one or three workgroups every such walk takes many
+    iterations, and the lists and the frames stay bit-exact."""
+    A = abi()
+    print(f"[active grid] blocks={blocks}", flush=True)
+    A.call("slx_set_active_grid", blocks)
+    try:
+        test_edge_traversal_two_tier_bit_exact(slx, orc, 16)
+        test_frames_with_edge_edge_shallow_walk(slx)
+    finally:
+        A.call("slx_set_active_grid", 2048)
+
+
 @pytest.mark.parametrize("velocity", [False, True])
 def test_frames_with_edge_gate_on_fine_cloth(slx, velocity):
     """A 64 x 64 cloth (edges of 1.6-2.2 cm) falling on a sphere, padding 0.02: the
