@@ -272,18 +272,9 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   }
   for (int s = 0; s < d->n_substeps; ++s) {
     bool refit_pending = false, snapped = false, disp_pending = false, v_done = false;
-    if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
-      // caches rebuilt in this substep, against x as it is now (the refit below
-      // builds the dynamic tree's boxes from this x)
-      built[0] = built[1] = built[2] = false;
-      if (dyn_cached) {
-        rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);   // (zeroes disp)
-        if (rc) return rc;
-        snapped = true;
-      }
-    }
     if (refits && s > 0) {
-      // fork: the refit reads x as the previous substep left it
+      // fork: the refit reads x as the previous substep left it (ahead of the
+      // snapshot below, which it does not need)
       if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: fork failed");
       rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
@@ -293,6 +284,16 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
       refit_pending = true;
       tv_fresh = false;
+    }
+    if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
+      // caches rebuilt in this substep, against x as it is now (the refit above
+      // builds the dynamic tree's boxes from this x)
+      built[0] = built[1] = built[2] = false;
+      if (dyn_cached) {
+        rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);   // (zeroes disp)
+        if (rc) return rc;
+        snapped = true;
+      }
     }
     auto join_refit = [&]() -> int {
       if (!refit_pending) return SLX_OK;
