@@ -73,3 +73,18 @@ def test_solver_fails_loudly_without_gpu(slx):
     m = slx.Mesh(x=g.x, faces=g.faces, device=None)
     with pytest.raises(slx.SlxError):
         slx.Solver(m, None, 1e-4, 1e6, 1e3, (0, -9.8, 0), 1 / 60)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark runs on one GPU: --gpus 2 exits with status 2 and says why,
+    before any GPU work."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2, r.stderr[-2000:]
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
