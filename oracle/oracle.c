@@ -340,6 +340,9 @@ int orc_traverse(int mode, int nq, const float* qmin, const float* qmax, int nl,
                  const float* nmax, const float* fmin, const float* fmax, const float* cs,
                  const float* origin, int cap, int32_t* cand, int32_t* cand_n) {
   int flagged = 0;
+  /* queries are independent (each writes its own list and count): the loop
+     runs on OpenMP threads with results identical to the serial loop */
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : flagged)
   for (int q = 0; q < nq; ++q) {
     const float *a = qmin + 3 * q, *b = qmax + 3 * q;
     int cnt = 0, over = 0;

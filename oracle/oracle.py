@@ -377,7 +377,7 @@ def greedy_coloring(nv, edges, max_colors=64):
 
 
 def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st, record=False,
-                   ee=None):
+                   ee=None, rec_bufs=None):
     """Position-level collision pass; with record=True the result also holds the
     pair records of the three query sets ("pairs": per set a dict of num [n],
     fid/dtype [n, cap], g [n, cap, 4, 3], schur [n, cap]), the reference's
@@ -388,7 +388,12 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     None, pcap (pair records kept per edge, default 3 cap).  The result's "ee"
     holds per set the contacting pair count pn_* (uncapped) and the records
     pi_* (other edge, type), pg_*, ps_* (with record=True), ready for
-    velocity_pass."""
+    velocity_pass.
+
+    rec_bufs: a list the caller keeps between passes; the record arrays are
+    allocated into it once and reused (records past a query's count then hold
+    an earlier pass's values, which nothing reads), instead of ~60 bytes x cap
+    of fresh zeroed memory per query and pass."""
     ct, s = _real(y.dtype)
     nv = len(fixed)
     nvs = 0 if xs is None else len(xs)
@@ -399,7 +404,13 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     for n in (nv, nv, nvs):
         n = max(n, 1)
         rec.append((np.zeros((n, cap, 2), np.int32), np.zeros((n, cap, 4, 3), y.dtype),
-                    np.zeros((n, cap), y.dtype)) if record else (None, None, None))
+                    np.zeros((n, cap), y.dtype)) if record and rec_bufs is None else (None, None, None))
+    if record and rec_bufs is not None:
+        shapes = [((max(n, 1), cap, 2), np.int32) for n in (nv, nv, nvs)]
+        if not rec_bufs or any(b[0].shape != s[0] or b[1].dtype != y.dtype for b, s in zip(rec_bufs, shapes)):
+            rec_bufs[:] = [(np.zeros((max(n, 1), cap, 2), np.int32), np.zeros((max(n, 1), cap, 4, 3), y.dtype),
+                            np.zeros((max(n, 1), cap), y.dtype)) for n in (nv, nv, nvs)]
+        rec = list(rec_bufs)
 
     def cn(t):
         return (None, None) if t is None else (np.ascontiguousarray(t[0], np.int32), np.ascontiguousarray(t[1], np.int32))
@@ -520,6 +531,7 @@ class OracleCloth:
         self.collision, self.bvh, self.cap = collision, bvh, cap
         self.refit_every_substep = refit_every_substep
         self.stats = dict(overflow=0)
+        self._rec_bufs = []
 
     def _trees(self):
         xf = self.x.astype(np.float32)
@@ -591,7 +603,7 @@ class OracleCloth:
                     res = collision_pass(self.m, None if self.st is None else self.st["faces"], self.y, xs,
                                          self.fixed, self.m_inv, R(self.dhat), R(1e8), self.cap,
                                          cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"),
-                                         record=self.velocity, ee=ee)
+                                         record=self.velocity, ee=ee, rec_bufs=self._rec_bufs)
                     self.last_pairs = res.get("pairs")
                     self.last_ee = res.get("ee")
                 if self.sewing is not None and len(self.sewing):

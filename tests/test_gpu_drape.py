@@ -1,0 +1,62 @@
+"""GPU parity of the benchmarked workload itself: the cloth draped over the
+sinusoid terrain exactly as bench.build_scene sets it up (terrain 142 x 142,
+thickness 0.5 mm, padding 0.25 mm, triangle LBVH rebuilt per frame and refit
+per substep, candidate cache with its margin and displacement bound, the
+velocity-level contact and friction pass with mu 0.8, three streams), frame
+by frame against the FP32 oracle's serial restatement of Solver.forward
+(XPBD.py:443-469 with collision_constraints_x.py:5-310 and
+collision_constraints_v.py:5-570).  Bit-exact: x and v equal, status 0, and
+no candidate list overflows in the oracle either."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def drape(n):
+    import bench
+    slx = importlib.import_module("starlab-physics_amd")
+    from oracle import oracle as orc
+    cloth, terr, sim = bench.build_scene(slx, n, True)
+    sim.enable_velocity_update = True          # what the benchmark runs (friction keeps the drape at rest)
+    sim.candidate_cache = True
+    st = dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
+    ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
+                          stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
+                          collision=True, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
+                          velocity=True, mu=sim.mu)
+    return cloth, sim, ref
+
+
+def check_frames(n, frames, substeps=20, iters=2):
+    cloth, sim, ref = drape(n)
+    pairs = 0
+    for f in range(frames):
+        print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters}", flush=True)
+        sim.clear_status()
+        sim.forward(substeps, iters)
+        ref.forward(substeps, iters)
+        assert sim.status() == 0, (f, sim.status())
+        assert ref.stats["overflow"] == 0, f
+        pairs += int(sim.pairs()["cnt"].sum().item())
+        np.testing.assert_array_equal(cloth.positions(), ref.x, err_msg=f"x, frame {f}")
+        np.testing.assert_array_equal(cloth.v4[:, :3].cpu().numpy(), ref.v, err_msg=f"v, frame {f}")
+    assert pairs > 0, "the drape must rest in contact"
+    return cloth, sim
+
+
+@pytest.mark.timeout(900)
+def test_drape_256_ten_frames_bit_exact():
+    check_frames(256, 10)
+
+
+@pytest.mark.timeout(900)
+def test_drape_1024_one_frame_bit_exact():
+    """The benchmark's own size: 1,048,576 vertices, one frame of 20 x 2."""
+    cloth, sim = check_frames(1024, 1)
+    cands = sim.candidates()
+    # the contact-rich state the benchmark measures: terrain and self candidates
+    assert float(cands["vt_st"][1].float().mean().item()) > 1.0
+    assert float(cands["vt_dy"][1].float().mean().item()) > 1.0
