@@ -49,7 +49,9 @@ def parse():
     p.add_argument("--edge-edge", action="store_true",
                    help="turn the edge-edge constraints on (enable_edge_edge; commented out in the reference)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--cpu-seconds", type=float, default=5.0,
+                   help="CPU baseline: whole frames per leg until this many seconds have passed (at least one)")
+    p.add_argument("--no-drift", action="store_true", help="skip the FP64 drift measurement")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed frames, copy a 256 MiB buffer 4 times (a known-size kernel that "
                         "calibrates hardware byte counters)")
@@ -101,34 +103,74 @@ def build_scene(slx, n, collision):
     return cloth, terr, sim
 
 
-def cpu_baseline(cloth, terr, sim, iters, seconds):
-    """The serial FP32 oracle (1 core) on the same scene: whole substeps
-    (predict, `iters` x [spring pass, broad phase, collision pass],
-    velocity/position update, the velocity pass when enabled), repeated for
-    about `seconds`."""
+def _oracle_scene(cloth, terr, sim, dtype=np.float32, parallel=False, velocity=None):
     from oracle import oracle as orc
     st = None if terr is None else dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np,
                                         edges=terr.eid_np, fe=terr.host_dict()["fe"])
-    ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
-                          stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt / 20, padding=sim.padding,
-                          collision=terr is not None, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
-                          velocity=sim.enable_velocity_update, mu=sim.mu, edge_edge=sim.enable_edge_edge)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        ref.forward(1, iters)       # one substep of dt/20 (tree rebuilt per call: a frame of 1 substep)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
+    return orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
+                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
+                           collision=terr is not None, bvh="tri", cap=sim.cache_size, refit_every_substep=True,
+                           velocity=sim.enable_velocity_update if velocity is None else velocity, mu=sim.mu,
+                           edge_edge=sim.enable_edge_edge, dtype=dtype, parallel=parallel)
+
+
+def cpu_baseline(cloth, terr, sim, substeps, iters, seconds):
+    """The oracle on the GPU box's host cores, on the same scene and the same
+    schedule as the GPU (whole frames: the tree rebuilt once per frame and
+    refit every substep), from the state the timed GPU frames ended in.  Two
+    legs: all the cores the process may use, with the element loops on OpenMP
+    threads and atomic scatter (the way a parallel CPU backend runs the
+    reference's kernels), and one core running the serial restatement.  Each
+    leg runs whole frames until `seconds` have passed (at least one)."""
+    from oracle import oracle as orc
+    cores = len(os.sched_getaffinity(0))
+    legs = {}
+    for name, threads, par in (("all_cores", cores, True), ("one_core", 1, False)):
+        orc.set_threads(threads)
+        ref = _oracle_scene(cloth, terr, sim, parallel=par)
+        frames = 0
+        t0 = time.perf_counter()
+        while True:
+            ref.forward(substeps, iters)
+            frames += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+        legs[name] = dict(value=cloth.num_verts * substeps * frames / el, threads=threads, frames=frames,
+                          seconds=round(el, 2))
+    orc.set_threads(cores)
     what = "with terrain + self collision" if terr is not None else "collision off"
     if sim.enable_velocity_update:
         what += ", velocity pass"
     if sim.enable_edge_edge:
         what += ", edge-edge constraints"
-    return dict(value=cloth.num_verts * reps / el, unit="vertex-substeps/s", cores=1, kind="port",
-                sample=f"{reps} substeps x {iters} Jacobi iterations of the {cloth.num_verts}-vertex cloth ({what}) "
-                       f"on the serial FP32 oracle, {el:.1f} s")
+    a = legs["all_cores"]
+    return dict(value=a["value"], unit="vertex-substeps/s", cores=cores, kind="port",
+                sample=f"{a['frames']} frame(s) of {substeps} substeps x {iters} Jacobi iterations of the "
+                       f"{cloth.num_verts}-vertex cloth ({what}; tree rebuilt per frame, refit per substep) on the "
+                       f"FP32 oracle, element loops on {cores} OpenMP threads with atomic scatter, {a['seconds']} s; "
+                       "CPU restatement, not Taichi",
+                one_core=dict(value=legs["one_core"]["value"], cores=1, frames=legs["one_core"]["frames"],
+                              seconds=legs["one_core"]["seconds"], sample="the serial FP32 oracle, same frames"))
+
+
+def fp64_drift(slx, substeps, iters):
+    """max |x_gpu - x_fp64| after one frame, against the FP64 oracle (the GPU's
+    FP32 result equals the FP32 oracle bit for bit): the collision-free solve of
+    the 1024^2 cloth, and the draped 256^2 cloth with collision and friction."""
+    import torch
+    out = {}
+    for key, n, coll in (("solve_1024_collision_off", 1024, False), ("drape_256_collision_friction", 256, True)):
+        cloth, terr, sim = build_scene(slx, n, coll)
+        sim.enable_velocity_update = coll
+        ref = _oracle_scene(cloth, terr, sim, dtype=np.float64)
+        sim.forward(substeps, iters)
+        torch.cuda.synchronize()
+        ref.forward(substeps, iters)
+        d = np.abs(cloth.positions().astype(np.float64) - ref.x)
+        out[key] = dict(max_abs_m=float(d.max()), p999_abs_m=float(np.percentile(d, 99.9)),
+                        extent_m=SCENE["size"], within_1e5_extent=bool(d.max() <= 1e-5 * SCENE["size"]))
+    return out
 
 
 def velocity_counts(sim, V):
@@ -235,6 +277,7 @@ def main():
     tot_ms, cnt = dom_timed
     avg_s = tot_ms / cnt / 1e3
     nbytes = rl.kernel_bytes(dom, counts)
+    impl_bytes = rl.kernel_impl_bytes(dom, counts)
     achieved = nbytes / avg_s / 1e9 if nbytes else None
     traffic = None
     # measured HBM bytes per launch of the dominant kernel (two rocprofv3 counter passes,
@@ -276,9 +319,12 @@ def main():
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": (achieved / 8000.0) if achieved else None, "traffic": traffic, "kernel": dom,
-                     "algorithmic_bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6},
+                     "algorithmic_bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
+                     "implementation_bytes_per_launch": impl_bytes,
+                     "implementation_frac": (impl_bytes / avg_s / rl.HBM_PEAK_BYTES_PER_S) if impl_bytes else None},
         "step_roofline_frac": (rl.step_bytes(counts, a.iters, a.substeps, collision) * a.steps / el)
                               / rl.HBM_PEAK_BYTES_PER_S,
+        "step_algorithmic_bytes": rl.step_bytes(counts, a.iters, a.substeps, collision),
         "status": status,
         "mean_candidates": cbar,
         "mean_cached_leaves": counts.get("cache", {}),
@@ -296,7 +342,9 @@ def main():
         torch.cuda.synchronize()
         out["copy_probe_bytes"] = 2 * src.numel() * 4
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cloth, terr, sim, a.iters, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cloth, terr, sim, a.substeps, a.iters, a.cpu_seconds)
+    if not a.no_drift and a.n == 1024:
+        out["fp64_drift"] = fp64_drift(slx, a.substeps, a.iters)
     print(json.dumps(out), flush=True)
 
 

@@ -8,6 +8,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static inline float sqrt_f(float a) { return sqrtf(a); }
 static inline double sqrt_d(double a) { return sqrt(a); }
@@ -24,6 +27,33 @@ static inline double fabs_d(double a) { return fabs(a); }
 #include "oracle_real.h"
 #undef R
 #undef S
+/* the OpenMP instantiation (suffix p, float): element loops on threads,
+   atomic scatter -- a timing baseline only, never a checker */
+static inline float sqrt_p(float a) { return sqrtf(a); }
+static inline float fabs_p(float a) { return fabsf(a); }
+#define ORC_PAR 1
+#define R float
+#define S p
+#include "oracle_real.h"
+#undef R
+#undef S
+#undef ORC_PAR
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 /* ------------------------------ LBVH -------------------------------------- */
 

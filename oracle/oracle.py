@@ -40,6 +40,16 @@ def lib():
     return _LIB
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (the traversal, and the
+    element loops of OracleCloth(parallel=True))."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
+
+
 def P(a):
     """Pointer of a contiguous numpy array (None stays NULL)."""
     if a is None:
@@ -48,22 +58,27 @@ def P(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _real(dtype):
+def _real(dtype, par=False):
+    """(ctypes real, function suffix); par: the OpenMP instantiation (float32
+    only, atomic scatter in no fixed order -- timing baseline, not a checker)."""
+    if par:
+        assert np.dtype(dtype) == np.float32, "the OpenMP instantiation is float32"
+        return C.c_float, "p"
     return (C.c_float, "f") if np.dtype(dtype) == np.float32 else (C.c_double, "d")
 
 
 # ------------------------------------------------------------------ primitives
 
-def compute_y(x, v, fixed, dt, g):
-    ct, s = _real(x.dtype)
+def compute_y(x, v, fixed, dt, g, par=False):
+    ct, s = _real(x.dtype, par)
     y = np.empty_like(x)
     gg = np.asarray(g, x.dtype)
     getattr(lib(), "orc_compute_y_" + s)(len(fixed), P(x), P(v), P(fixed), P(y), ct(dt), P(gg))
     return y
 
 
-def spring_pass(mesh, x, y, fixed, m_inv, k_s, k_b):
-    ct, s = _real(x.dtype)
+def spring_pass(mesh, x, y, fixed, m_inv, k_s, k_b, par=False):
+    ct, s = _real(x.dtype, par)
     dt = x.dtype
     dx = np.zeros_like(x)
     nc = np.zeros(len(fixed), dt)
@@ -103,8 +118,8 @@ def gs_pass(mesh, x, y, fixed, m_inv, k_s, k_b, order, offsets):
     return y
 
 
-def update_vx(x, v, y, fixed, damping, dt):
-    ct, s = _real(x.dtype)
+def update_vx(x, v, y, fixed, damping, dt, par=False):
+    ct, s = _real(x.dtype, par)
     getattr(lib(), "orc_update_vx_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
 
 
@@ -118,13 +133,13 @@ def sewing_pass(y, pairs, fixed, m_inv, k):
     return y
 
 
-def compute_velocity(x, v, y, fixed, damping, dt):
-    ct, s = _real(x.dtype)
+def compute_velocity(x, v, y, fixed, damping, dt, par=False):
+    ct, s = _real(x.dtype, par)
     getattr(lib(), "orc_compute_velocity_" + s)(len(fixed), P(x), P(v), P(y), P(fixed), ct(damping), ct(dt))
 
 
-def update_x(x, v, dt):
-    ct, s = _real(x.dtype)
+def update_x(x, v, dt, par=False):
+    ct, s = _real(x.dtype, par)
     getattr(lib(), "orc_update_x_" + s)(len(x), P(x), P(v), ct(dt))
 
 
@@ -377,7 +392,7 @@ def greedy_coloring(nv, edges, max_colors=64):
 
 
 def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_st, vt_dy, tv_st, record=False,
-                   ee=None, rec_bufs=None):
+                   ee=None, rec_bufs=None, par=False):
     """Position-level collision pass; with record=True the result also holds the
     pair records of the three query sets ("pairs": per set a dict of num [n],
     fid/dtype [n, cap], g [n, cap, 4, 3], schur [n, cap]), the reference's
@@ -394,7 +409,7 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     allocated into it once and reused (records past a query's count then hold
     an earlier pass's values, which nothing reads), instead of ~60 bytes x cap
     of fresh zeroed memory per query and pass."""
-    ct, s = _real(y.dtype)
+    ct, s = _real(y.dtype, par)
     nv = len(fixed)
     nvs = 0 if xs is None else len(xs)
     dx = np.zeros_like(y)
@@ -443,7 +458,7 @@ def collision_pass(mesh_dy, faces_st, y, xs, fixed, m_inv, dhat, k_col, cap, vt_
     return out
 
 
-def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, cap, nvs=None, ee=None):
+def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, cap, nvs=None, ee=None, par=False):
     """solve_constraints_v (XPBD.py:374-381, 283-311; collision_constraints_v.py
     5-570; update_dv 397-404) over pair records from collision_pass(record=True);
     v is updated in place.  pairs None: no collision, update_dv alone.
@@ -452,7 +467,7 @@ def velocity_pass(mesh_dy, faces_st, v, vs, fixed, m_inv, m_inv_st, mu, pairs, c
     ee: collision_pass(record=True, ee=...)["ee"] -- the edge-edge pairs follow
     the vertex-triangle ones (XPBD.py:313-330; collision_constraints_v.py
     573-878)."""
-    ct, s = _real(v.dtype)
+    ct, s = _real(v.dtype, par)
     nv = len(fixed)
     if nvs is None:
         if vs is not None:
@@ -498,11 +513,14 @@ class OracleCloth:
     def __init__(self, mesh_dy, mesh_st=None, *, dhat, stiffness_stretch, stiffness_bending,
                  g, dt, dtype=np.float32, damping=0.001, padding=0.05, collision=False,
                  bvh="cell", cap=40, refit_every_substep=False, method="jacobi", velocity=False, mu=0.8,
-                 sewing=None, sewing_stiffness=None, edge_edge=False, ee_pcap=None):
+                 sewing=None, sewing_stiffness=None, edge_edge=False, ee_pcap=None, parallel=False):
         """edge_edge: the edge-edge constraints of XPBD.py:217-232, 266-281,
         313-330 (mesh_dy and mesh_st then also carry edges [E, 2] and fe [F, 3],
         the edge ids of each face's sides); ee_pcap: edge pairs kept per edge for
-        the velocity pass (default cap)."""
+        the velocity pass (default cap).  parallel: run the element loops on
+        OpenMP threads with atomic scatter (float32; a timing baseline, not a
+        checker: sums are then in no fixed order)."""
+        self.par = bool(parallel)
         self.method = method
         self.edge_edge, self.ee_pcap = edge_edge, (cap if ee_pcap is None else ee_pcap)
         self.velocity, self.mu = velocity, mu
@@ -581,7 +599,7 @@ class OracleCloth:
         for _ in range(n_substeps):
             if self.collision and self.bvh == "tri" and self.refit_every_substep:
                 self.tree_dy.refit(self.x.astype(np.float32))
-            self.y = compute_y(self.x, self.v, self.fixed, dts, g)
+            self.y = compute_y(self.x, self.v, self.fixed, dts, g, self.par)
             if self.method == "gs" and not hasattr(self, "_colors"):
                 _, order, off = greedy_coloring(len(self.fixed), self.m["edges"])
                 self._colors = (order, off)
@@ -589,7 +607,7 @@ class OracleCloth:
                 if self.method == "gs":
                     gs_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b, *self._colors)
                 else:
-                    spring_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b)
+                    spring_pass(self.m, self.x, self.y, self.fixed, self.m_inv, k_s, k_b, self.par)
                 if self.collision:
                     cands = self._broadphase()
                     xs = None if self.st is None else np.asarray(self.st["x"], self.dt_)
@@ -603,22 +621,22 @@ class OracleCloth:
                     res = collision_pass(self.m, None if self.st is None else self.st["faces"], self.y, xs,
                                          self.fixed, self.m_inv, R(self.dhat), R(1e8), self.cap,
                                          cands.get("vt_st"), cands.get("vt_dy"), cands.get("tv_st"),
-                                         record=self.velocity, ee=ee, rec_bufs=self._rec_bufs)
+                                         record=self.velocity, ee=ee, rec_bufs=self._rec_bufs, par=self.par)
                     self.last_pairs = res.get("pairs")
                     self.last_ee = res.get("ee")
                 if self.sewing is not None and len(self.sewing):
                     ksew = 0.5 * self.kb if self.sewing_stiffness is None else self.sewing_stiffness
                     sewing_pass(self.y, self.sewing, self.fixed, self.m_inv, R(ksew * dt_sub * dt_sub))
             if not self.velocity:
-                update_vx(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
+                update_vx(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts, self.par)
                 continue
             # compute_velocity, solve_constraints_v, update_x (XPBD.py:465-469)
-            compute_velocity(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts)
+            compute_velocity(self.x, self.v, self.y, self.fixed, R(np.float32(self.damping)), dts, self.par)
             st = self.st
             vs = None if st is None else np.ascontiguousarray(st.get("v", np.zeros_like(st["x"])), self.dt_)
             mst = None if st is None or "m_inv" not in st else np.asarray(st["m_inv"], self.dt_)
             velocity_pass(self.m, None if st is None else st["faces"], self.v, vs, self.fixed, self.m_inv, mst,
                           R(np.float32(self.mu)), self.last_pairs if self.collision else None, self.cap,
-                          ee=self.last_ee if self.collision and self.edge_edge else None)
-            update_x(self.x, self.v, dts)
+                          ee=self.last_ee if self.collision and self.edge_edge else None, par=self.par)
+            update_x(self.x, self.v, dts, self.par)
         return self.x

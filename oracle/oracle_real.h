@@ -23,8 +23,28 @@ static inline void CAT(cross3_, S)(const R* a, const R* b, R* o) {
 #define SUB CAT(sub3_, S)
 #define CROSS CAT(cross3_, S)
 
+/* ACC(lvalue, value): the scatter of a delta into dx / dv / nc.  The serial
+   instantiations add in loop order (lvalue = lvalue + value, the reference's
+   serial order); the OpenMP instantiation (ORC_PAR, suffix p) runs the
+   element loops marked PFOR on threads and adds with an atomic, as a
+   parallel CPU backend does -- its sums are then in no fixed order. */
+#undef ACC
+#undef PFOR
+#ifdef ORC_PAR
+#define ACC(lv, val)                      \
+  do {                                    \
+    R acc_v_ = (val);                     \
+    _Pragma("omp atomic") (lv) += acc_v_; \
+  } while (0)
+#define PFOR _Pragma("omp parallel for schedule(dynamic, 512)")
+#else
+#define ACC(lv, val) ((lv) = (lv) + (val))
+#define PFOR
+#endif
+
 /* XPBD.py:115-118: y = x + f*v*dt + g*dt*dt, evaluated left to right. */
 void FN(compute_y)(int nv, const R* x, const R* v, const R* fixed, R* y, R dt, const R* g) {
+  PFOR
   for (int i = 0; i < nv; ++i) {
     for (int c = 0; c < 3; ++c) {
       R t = (fixed[i] * v[3 * i + c]) * dt;
@@ -48,15 +68,16 @@ static inline void CAT(spring_one_, S)(int v0, int v1, R l0, const R* p, R k, co
   R schur = (fw0 + fw1) * DOT(n, n);
   R ld = (k * C) / ((k * schur) + (R)1);
   for (int c = 0; c < 3; ++c) {
-    dx[3 * v0 + c] = dx[3 * v0 + c] - (fw0 * ld) * n[c];
-    dx[3 * v1 + c] = dx[3 * v1 + c] + (fw1 * ld) * n[c];
+    ACC(dx[3 * v0 + c], -((fw0 * ld) * n[c]));
+    ACC(dx[3 * v1 + c], (fw1 * ld) * n[c]);
   }
-  nc[v0] = nc[v0] + (R)1;
-  nc[v1] = nc[v1] + (R)1;
+  ACC(nc[v0], (R)1);
+  ACC(nc[v1], (R)1);
 }
 
 /* XPBD.py:383-387 update_dx: y += f * (dx / nc) where nc > 0. */
 static void CAT(update_dx_, S)(int nv, R* y, const R* fixed, const R* dx, const R* nc) {
+  PFOR
   for (int i = 0; i < nv; ++i) {
     if (nc[i] > (R)0) {
       for (int c = 0; c < 3; ++c) y[3 * i + c] = y[3 * i + c] + fixed[i] * (dx[3 * i + c] / nc[i]);
@@ -69,10 +90,14 @@ static void CAT(update_dx_, S)(int nv, R* y, const R* fixed, const R* dx, const 
 void FN(spring_pass)(int nv, int ne, int nb, const int* edges, const R* l0, const int* bend,
                      const R* bl0, const R* x, R* y, const R* fixed, const R* m_inv, R k_s, R k_b,
                      R* dx, R* nc) {
+  PFOR
   for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  PFOR
   for (int i = 0; i < nv; ++i) nc[i] = 0;
+  PFOR
   for (int i = 0; i < ne; ++i)
     CAT(spring_one_, S)(edges[2 * i], edges[2 * i + 1], l0[i], y, k_s, fixed, m_inv, dx, nc);
+  PFOR
   for (int i = 0; i < nb; ++i)
     CAT(spring_one_, S)(bend[2 * i], bend[2 * i + 1], bl0[i], x, k_b, fixed, m_inv, dx, nc);
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
@@ -162,6 +187,7 @@ void FN(gs_pass)(int nv, int nb, const int* bend, const R* bl0, const R* x, R* y
    v = (1 - damping) * f * (y - x) / dt;  x += dt * v. */
 void FN(update_vx)(int nv, R* x, R* v, const R* y, const R* fixed, R damping, R dt) {
   R a = (R)1 - damping;
+  PFOR
   for (int i = 0; i < nv; ++i) {
     for (int c = 0; c < 3; ++c) {
       R vv = ((a * fixed[i]) * (y[3 * i + c] - x[3 * i + c])) / dt;
@@ -320,8 +346,8 @@ static void CAT(vt_st_, S)(int vd, int fs, const int* faces_st, const R* y, cons
     R w = m_inv[vd];
     R schur = w * DOT(g, g);
     R ld = (k * (dhat - d)) / ((k * schur) + (R)1);
-    for (int c = 0; c < 3; ++c) dx[3 * vd + c] = dx[3 * vd + c] + (w * ld) * g[c];
-    nc[vd] = nc[vd] + (R)1;
+    for (int c = 0; c < 3; ++c) ACC(dx[3 * vd + c], (w * ld) * g[c]);
+    ACC(nc[vd], (R)1);
     CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fs, dtype, g, schur);
   }
 }
@@ -353,10 +379,10 @@ static void CAT(tv_st_, S)(int fd, int vs, const int* faces_dy, R* y, const R* x
       if (!(m & (1u << s))) continue;
       int v = vid[s];
       R a = (fixed[v] * m_inv[v]) * ld;
-      for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s + c];
+      for (int c = 0; c < 3; ++c) ACC(dx[3 * v + c], a * g[3 * s + c]);
     }
     for (int s = 1; s < 4; ++s)
-      if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+      if (m & (1u << s)) ACC(nc[vid[s]], (R)1);
     CAT(record_, S)(vs, cap, pnum, pi, pg, ps, fd, dtype, g, schur);
     return;
   }
@@ -393,10 +419,10 @@ static void CAT(vt_dy_, S)(int vd, int fd, const int* faces_dy, R* y, const R* f
       if (!(m & (1u << s))) continue;
       int v = vid[s];
       R a = (dtype == 0) ? m_inv[v] * ld : (fixed[v] * m_inv[v]) * ld;
-      for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s + c];
+      for (int c = 0; c < 3; ++c) ACC(dx[3 * v + c], a * g[3 * s + c]);
     }
     for (int s = 0; s < 4; ++s)
-      if (m & (1u << s)) nc[vid[s]] = nc[vid[s]] + (R)1;
+      if (m & (1u << s)) ACC(nc[vid[s]], (R)1);
     CAT(record_, S)(vd, cap, pnum, pi, pg, ps, fd, dtype, g, schur);
     return;
   }
@@ -412,17 +438,22 @@ static void CAT(vt_loops_, S)(int nv, int nvs, const int* faces_dy, const int* f
                               const int* vt_dy_n, const int* tv_st_c, const int* tv_st_n, R* dx, R* nc,
                               int* pn_vt_st, int* pn_vt_dy, int* pn_tv_st, int* pi0, R* pg0, R* ps0,
                               int* pi1, R* pg1, R* ps1, int* pi2, R* pg2, R* ps2) {
+  PFOR
   for (int i = 0; i < 3 * nv; ++i) dx[i] = 0;
+  PFOR
   for (int i = 0; i < nv; ++i) { nc[i] = 0; pn_vt_st[i] = 0; pn_vt_dy[i] = 0; }
   for (int i = 0; i < nvs; ++i) pn_tv_st[i] = 0;
-  if (vt_st_c)
+  if (vt_st_c) {
+    PFOR
     for (int v = 0; v < nv; ++v) {
       int n = vt_st_n[v] < cap ? vt_st_n[v] : cap;
       for (int j = 0; j < n; ++j)
         CAT(vt_st_, S)(v, vt_st_c[v * cap + j], faces_st, y, xs, m_inv, dhat, k_col, dx, nc, cap, pn_vt_st,
                        pi0, pg0, ps0);
     }
-  if (vt_dy_c)
+  }
+  if (vt_dy_c) {
+    PFOR
     for (int v = 0; v < nv; ++v) {
       int n = vt_dy_n[v] < cap ? vt_dy_n[v] : cap;
       for (int j = 0; j < n; ++j) {
@@ -431,13 +462,16 @@ static void CAT(vt_loops_, S)(int nv, int nvs, const int* faces_dy, const int* f
         CAT(vt_dy_, S)(v, f, faces_dy, y, fixed, m_inv, dhat, dx, nc, cap, pn_vt_dy, pi1, pg1, ps1);
       }
     }
-  if (tv_st_c)
+  }
+  if (tv_st_c) {
+    PFOR
     for (int v = 0; v < nvs; ++v) {
       int n = tv_st_n[v] < cap ? tv_st_n[v] : cap;
       for (int j = 0; j < n; ++j)
         CAT(tv_st_, S)(tv_st_c[v * cap + j], v, faces_dy, y, xs, fixed, m_inv, dhat, k_col, dx, nc, cap, pn_tv_st,
                        pi2, pg2, ps2);
     }
+  }
 }
 
 /* XPBD.py:236-264 + update_dx.  pi_k/pg_k/ps_k (nullable) receive the pair
@@ -566,9 +600,9 @@ static int CAT(ee_pair_, S)(int st, const int* vid, const R* p0, const R* p1, co
   for (int s_ = 0; s_ < nmov; ++s_) {
     int v = vid[s_];
     R a = (fixed[v] * m_inv[v]) * ld;
-    for (int c = 0; c < 3; ++c) dx[3 * v + c] = dx[3 * v + c] + a * g[3 * s_ + c];
+    for (int c = 0; c < 3; ++c) ACC(dx[3 * v + c], a * g[3 * s_ + c]);
   }
-  for (int s_ = 0; s_ < nmov; ++s_) nc[vid[s_]] = nc[vid[s_]] + (R)1;
+  for (int s_ = 0; s_ < nmov; ++s_) ACC(nc[vid[s_]], (R)1);
   *schur_out = schur;
   return 1;
 }
@@ -672,12 +706,12 @@ void FN(sewing_pass)(int nv, int np, const int* pairs, R* y, const R* fixed, con
       R schur = fw1 + fw2;
       R ld = (k * C) / ((k * schur) + (R)1);
       for (int c = 0; c < 3; ++c) {
-        dx[3 * v1 + c] = dx[3 * v1 + c] - (fw1 * ld) * n[c];
-        dx[3 * v2 + c] = dx[3 * v2 + c] + (fw2 * ld) * n[c];
+        ACC(dx[3 * v1 + c], -((fw1 * ld) * n[c]));
+        ACC(dx[3 * v2 + c], (fw2 * ld) * n[c]);
       }
     }
-    nc[v1] = nc[v1] + (R)1;
-    nc[v2] = nc[v2] + (R)1;
+    ACC(nc[v1], (R)1);
+    ACC(nc[v2], (R)1);
   }
   CAT(update_dx_, S)(nv, y, fixed, dx, nc);
 }
@@ -687,20 +721,22 @@ void FN(sewing_pass)(int nv, int np, const int* pairs, R* y, const R* fixed, con
 /* XPBD.py:340-343: v = (1 - damping) * f * (y - x) / dt. */
 void FN(compute_velocity)(int nv, const R* x, R* v, const R* y, const R* fixed, R damping, R dt) {
   R a = (R)1 - damping;
+  PFOR
   for (int i = 0; i < nv; ++i)
     for (int c = 0; c < 3; ++c) v[3 * i + c] = ((a * fixed[i]) * (y[3 * i + c] - x[3 * i + c])) / dt;
 }
 
 /* XPBD.py:332-337: x += dt * v. */
 void FN(update_x)(int nv, R* x, const R* v, R dt) {
+  PFOR
   for (int i = 0; i < 3 * nv; ++i) x[i] = x[i] + dt * v[i];
 }
 
 static inline R CAT(norm3_, S)(const R* a) { return SQRT(DOT(a, a)); }
 
 static inline void CAT(vadd_, S)(R* dv, R* nc, int v, const R* d) {
-  for (int c = 0; c < 3; ++c) dv[3 * v + c] = dv[3 * v + c] + d[c];
-  nc[v] = nc[v] + (R)1;
+  for (int c = 0; c < 3; ++c) ACC(dv[3 * v + c], d[c]);
+  ACC(nc[v], (R)1);
 }
 
 /* collision_constraints_v.py:5-134 __vt_st: dynamic vertex v0 against static
@@ -898,33 +934,42 @@ static void CAT(vel_vt_loops_, S)(int nv, int nvs, const int* faces_dy, const in
                                   const int* pn0, const int* pi0, const R* pg0, const R* ps0,
                                   const int* pn1, const int* pi1, const R* pg1, const R* ps1,
                                   const int* pn2, const int* pi2, const R* pg2, const R* ps2, R* dv, R* nc) {
+  PFOR
   for (int i = 0; i < 3 * nv; ++i) dv[i] = 0;
+  PFOR
   for (int i = 0; i < nv; ++i) nc[i] = 0;
-  if (pn0)
+  if (pn0) {
+    PFOR
     for (int q = 0; q < nv; ++q)
       for (int j = 0; j < pn0[q]; ++j) {
         size_t at = (size_t)q * (size_t)cap + (size_t)j;
         CAT(vel_vt_st_, S)(q, pi0[2 * at], pi0[2 * at + 1], pg0 + 12 * at, ps0[at], faces_st, v, vs, m_inv,
                            m_inv_st, nvs, mu, dv, nc);
       }
-  if (pn1)
+  }
+  if (pn1) {
+    PFOR
     for (int q = 0; q < nv; ++q)
       for (int j = 0; j < pn1[q]; ++j) {
         size_t at = (size_t)q * (size_t)cap + (size_t)j;
         CAT(vel_vt_dy_, S)(q, pi1[2 * at], pi1[2 * at + 1], pg1 + 12 * at, ps1[at], faces_dy, v, fixed, m_inv,
                            mu, dv, nc);
       }
-  if (pn2)
+  }
+  if (pn2) {
+    PFOR
     for (int q = 0; q < nvs; ++q)
       for (int j = 0; j < pn2[q]; ++j) {
         size_t at = (size_t)q * (size_t)cap + (size_t)j;
         CAT(vel_tv_st_, S)(q, pi2[2 * at], pi2[2 * at + 1], pg2 + 12 * at, ps2[at], faces_dy, v, vs, fixed,
                            m_inv, mu, dv, nc);
       }
+  }
 }
 
 /* update_dv (XPBD.py:397-404): dv /= nc where nc > 0, v += f dv where m_inv > 0. */
 static void CAT(update_dv_, S)(int nv, R* v, const R* fixed, const R* m_inv, R* dv, const R* nc) {
+  PFOR
   for (int i = 0; i < nv; ++i) {
     if (nc[i] > (R)0)
       for (int c = 0; c < 3; ++c) dv[3 * i + c] = dv[3 * i + c] / nc[i];
