@@ -121,6 +121,18 @@ size_t slx_radix_sort_ws_bytes(int n_max);
 int slx_radix_sort_pairs(int n_max, const int32_t* n_dev, uint32_t* keys, int32_t* vals,
                          int key_bits, void* ws, size_t ws_bytes, slx_stream_t stream);
 
+/* slx_radix_sort_pairs without a workspace argument and without any allocation
+ * (no hipMalloc, hipFree or allocating library call).  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Its scratch
+ * is a static device array of 18.9 MB per GPU, held until the process ends,
+ * which limits n_max to 2,097,152 pairs (larger n_max: SLX_EINVAL with a
+ * message).  Every call on one GPU shares that GPU's scratch, so calls that
+ * overlap on one GPU (on different streams, or from different threads) race;
+ * calls in one stream's order do not. */
+int slx_radix_sort_pairs_noalloc(int n_max, const int32_t* n_dev, uint32_t* keys, int32_t* vals,
+                                 int key_bits, slx_stream_t stream);
+
 /* ---------------- broad phase (framework/collision/lbvh_cell.py) ---------------- */
 
 /* computeAABB_faces, meshTaichiWrapper.py:350-361: face boxes of x4 -+ padding,

@@ -37,6 +37,7 @@ SIGNATURES = {
     "slx_exclusive_scan": [I32, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
     "slx_radix_sort_ws_bytes": [I32],
     "slx_radix_sort_pairs": [I32, PTR, PTR, PTR, I32, PTR, C.c_size_t, STREAM],
+    "slx_radix_sort_pairs_noalloc": [I32, PTR, PTR, PTR, I32, STREAM],
     "slx_face_aabbs": [I32, PTR, PTR, F32, PTR, PTR, STREAM],
     "slx_mesh_aabb_ws_bytes": [],
     "slx_mesh_aabb": [I32, PTR, F32, PTR, PTR, C.c_size_t, STREAM],

@@ -12,6 +12,8 @@
 // Sizes may live on the device (n_dev != nullptr): kernels are launched for
 // n_max elements and read the actual count from n_dev, so a data-dependent
 // sort needs no host synchronisation.
+#include <mutex>
+
 #include "slx_common.h"
 
 namespace slx {
@@ -263,6 +265,47 @@ int radix_sort_pairs(int n_max, const int* n_dev, uint32_t* keys, int32_t* vals,
   return check_launch("radix_sort_pairs");
 }
 
+// ---- radix sort without allocation ----
+// The double buffer, digit histograms and scan tile sums of a sort of up to
+// kNoAllocPairs pairs, for callers that cannot allocate (code under graph
+// capture, serving loops).  One copy per GPU, held until the process ends
+// (18.9 MB).  Host code never passes the array's name to a kernel: it is a
+// host-side stand-in, and each device's copy is reached through the address
+// hipGetSymbolAddress returns on that device.
+constexpr int kNoAllocPairs = 1 << 21;
+constexpr size_t noalloc_scratch_bytes() {
+  size_t n = (size_t)kNoAllocPairs;
+  size_t tiles = (n + kRsTile - 1) / kRsTile;
+  size_t hist = (size_t)kRadix * tiles;
+  size_t scan_tiles = (hist + kScanTile - 1) / kScanTile;
+  return 2 * n * 4 + 2 * hist * 4 + sizeof(int) * (scan_tiles + 64) + 256;
+}
+constexpr size_t kNoAllocScratchBytes = noalloc_scratch_bytes();
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch`, looked up once per device
+int scratch_on_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(SLX_EHIP, "slx_radix_sort_pairs_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SLX_EHIP, "slx_radix_sort_pairs_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SLX_EHIP, "slx_radix_sort_pairs_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SLX_OK;
+}
+
 }  // namespace slx
 
 using namespace slx;
@@ -283,4 +326,19 @@ extern "C" int slx_radix_sort_pairs(int n_max, const int32_t* n_dev, uint32_t* k
   SLX_REQUIRE(n_max >= 0 && key_bits >= 0 && key_bits <= 32, "slx_radix_sort_pairs: bad sizes");
   SLX_REQUIRE(n_max == 0 || (keys && vals && ws), "slx_radix_sort_pairs: null pointer");
   return radix_sort_pairs(n_max, n_dev, keys, vals, key_bits, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int slx_radix_sort_pairs_noalloc(int n_max, const int32_t* n_dev, uint32_t* keys, int32_t* vals,
+                                            int key_bits, slx_stream_t stream) {
+  SLX_REQUIRE(n_max >= 0 && key_bits >= 0 && key_bits <= 32, "slx_radix_sort_pairs_noalloc: bad sizes");
+  SLX_REQUIRE(n_max <= kNoAllocPairs, "slx_radix_sort_pairs_noalloc: n_max %d exceeds the scratch limit of %d pairs",
+              n_max, kNoAllocPairs);
+  SLX_REQUIRE(n_max == 0 || (keys && vals), "slx_radix_sort_pairs_noalloc: null pointer");
+  if (n_max == 0 || key_bits == 0) return SLX_OK;
+  const size_t need = radix_sort_ws_bytes(n_max);
+  if (need > kNoAllocScratchBytes) return fail(SLX_EWS, "slx_radix_sort_pairs_noalloc: scratch too small");
+  void* ws = nullptr;
+  int rc = scratch_on_device(&ws);
+  if (rc) return rc;
+  return radix_sort_pairs(n_max, n_dev, keys, vals, key_bits, ws, kNoAllocScratchBytes, (hipStream_t)stream);
 }
