@@ -43,6 +43,8 @@ def parse():
                    help="candidate cache margin as a multiple of the padding (default: the solver's)")
     p.add_argument("--cache-every", type=int, default=None, help="substeps between candidate cache rebuilds")
     p.add_argument("--walk-stack", type=int, default=None, help="first-tier triangle walk stack depth (16/24/32/64)")
+    p.add_argument("--walk-wave", type=int, default=None,
+                   help="1: cloth-vertex walks of the cloth tree as one shared walk per wavefront; 0: one per lane")
     p.add_argument("--no-friction", action="store_true",
                    help="turn the velocity-level contact and friction pass off (enable_velocity_update False); "
                         "the frictionless drape slides into the terrain's valleys and bunches up within ~10 frames")
@@ -55,9 +57,11 @@ def parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed frames, copy a 256 MiB buffer 4 times (a known-size kernel that "
                         "calibrates hardware byte counters)")
-    p.add_argument("--workload", default="cloth", choices=["cloth", "batched", "learned"],
+    p.add_argument("--workload", default="cloth", choices=["cloth", "batched", "learned", "sphere64", "layers"],
                    help="cloth: the 1M-vertex cloth (default); batched: 256 uneven instances in one "
-                        "captured graph; learned: time to residual, plain vs learned warm start")
+                        "captured graph; learned: time to residual, plain vs learned warm start; sphere64: "
+                        "the 64x64 cloth on an icosphere with the cell LBVH, 120 frames; layers: four stacked "
+                        "256x256 layers on an icosphere, tree rebuilt every substep, Jacobi and Gauss-Seidel")
     p.add_argument("--instances", type=int, default=256)
     p.add_argument("--learned-side", type=int, default=256, help="cloth side of the learned workload")
     p.add_argument("--learned-samples", type=int, default=200000, help="training samples of the learned workload")
@@ -202,6 +206,10 @@ def main():
         return bench_batched(a)
     if a.workload == "learned":
         return bench_learned(a)
+    if a.workload == "sphere64":
+        return bench_sphere64(a)
+    if a.workload == "layers":
+        return bench_layers(a)
     import torch
     slx = importlib.import_module("starlab-physics_amd")
     rl = importlib.import_module("starlab-physics_amd.roofline")
@@ -215,7 +223,9 @@ def main():
     sim.candidate_cache = not a.no_cache
     sim.enable_edge_edge = collision and a.edge_edge
     if a.walk_stack is not None:
-        abi.call("slx_set_walk_stack", a.walk_stack)
+        sim.walk_stack = a.walk_stack
+    if a.walk_wave is not None:
+        sim.walk_wave = bool(a.walk_wave)
     if a.cache_margin is not None:
         sim.cache_margin = a.cache_margin * sim.padding
     if a.cache_every is not None:
@@ -284,7 +294,7 @@ def main():
     # calibrated on a copy kernel), valid for the default workload
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
     default_run = (collision and a.velocity and not a.no_cache and a.cache_margin is None and a.cache_every is None
-                   and a.walk_stack is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
+                   and a.walk_stack is None and a.walk_wave is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
                    and not a.edge_edge)
     if default_run and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
@@ -385,6 +395,133 @@ def bench_batched(a):
         "autotune": {"best": list(best), "frame_ms": {f"{k[0]}x{k[1]}": round(v * 1e3, 3) for k, v in tune.items()}},
         "status": bs.sim.status(), "setup_s": setup,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _line(a, metric_value, V, substeps_total, el, workload, extra):
+    out = {"metric": "vertex-substeps/s", "value": metric_value, "unit": "vertex-substeps/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene)",
+           "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def bench_sphere64(a):
+    """The reference's CPU-runnable case: a 64 x 64 cloth (1 m square at y = 0.5,
+    jitter 1e-3, corners (0, 0) and (63, 0) pinned) dropped on an icosphere
+    (subdivision 4, radius 0.3 m), the reference's 32^3 cell LBVH rebuilt every
+    frame, thickness 0.01 m (dHat = 1e-4 on squared distance), padding 0.05,
+    20 substeps x 1 Jacobi iteration per frame, 120 frames (all timed; --steps
+    is ignored)."""
+    import torch
+    slx = importlib.import_module("starlab-physics_amd")
+    n, frames = 64, 120
+
+    def make():
+        g = slx.scene.grid_cloth(n, size=1.0, height=0.5, jitter=1e-3, seed=0)
+        cloth = slx.Mesh(x=g.x, faces=g.faces)
+        ico = slx.scene.icosphere(4, 0.3)
+        obst = slx.Mesh(x=ico.x, faces=ico.faces, is_static=True)
+        sim = slx.Solver(cloth, obst, dHat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3,
+                         g=(0.0, -9.8, 0.0), dt=1.0 / 60.0)
+        sim.enable_collision_handling = True
+        sim.bvh_mode = "cell"
+        sim.padding = 0.05
+        sel = torch.zeros(cloth.num_verts, device="cuda")
+        sel[0] = 1
+        sel[(n - 1) * n] = 1
+        sim.set_fixed_vertices(sel)
+        return cloth, obst, sim
+
+    cloth, obst, sim = make()              # warm-up run (allocations, first launches)
+    for _ in range(a.warmup):
+        sim.forward(20, 1)
+    cloth, obst, sim = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.forward(20, 1)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    V = cloth.num_verts
+    a.steps = frames
+    out = _line(a, V * 20 * frames / el, V, 20 * frames, el,
+                f"{n}x{n} cloth ({V} vertices) on an icosphere (2562 vertices, 5120 triangles), cell LBVH, "
+                f"thickness 0.01 m, 20 substeps x 1 iteration, {frames} frames",
+                {"status": sim.status(), "frames": frames})
+    if not a.no_cpu_baseline:
+        from oracle import oracle as orc
+        orc.set_threads(1)
+        c2, o2, s2 = make()
+        ref = orc.OracleCloth(c2.host_dict(), dict(x=o2.positions(), faces=o2.fid_np, m_inv=o2.m_inv_np),
+                              dhat=1e-4, stiffness_stretch=1e6, stiffness_bending=1e3, g=(0.0, -9.8, 0.0),
+                              dt=1.0 / 60.0, padding=0.05, collision=True, bvh="cell", cap=40)
+        k = 0
+        t0 = time.perf_counter()
+        while k < frames:
+            ref.forward(20, 1)
+            k += 1
+            if time.perf_counter() - t0 >= a.cpu_seconds:
+                break
+        cel = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=V * 20 * k / cel, unit="vertex-substeps/s", cores=1, kind="port",
+                                   sample=f"the first {k} frames of the same scene on the serial FP32 oracle, "
+                                          f"{cel:.1f} s; CPU restatement, not Taichi")
+    print(json.dumps(out), flush=True)
+
+
+def bench_layers(a):
+    """The self-collision stack: four 256 x 256 layers 0.02 m apart (262,144
+    vertices, 520,200 triangles) on an icosphere (subdivision 4, r 0.3 m),
+    self and obstacle vertex-triangle collision at thickness 5 mm (dHat 2.5e-5
+    on squared distance, padding 2.5 mm), the triangle LBVH rebuilt every
+    substep (a frame of one substep per call), 10 substeps of 1/600 s per step,
+    one iteration, Jacobi and the greedy-coloured Gauss-Seidel variant."""
+    import torch
+    slx = importlib.import_module("starlab-physics_amd")
+    n, layers = 256, 4
+
+    def make(method):
+        g = slx.scene.stacked_layers(n, layers, 0.02, size=1.0, height=0.302, jitter=1e-3, seed=0)
+        cloth = slx.Mesh(x=g.x, faces=g.faces)
+        ico = slx.scene.icosphere(4, 0.3)
+        obst = slx.Mesh(x=ico.x, faces=ico.faces, is_static=True)
+        sim = slx.Solver(cloth, obst, dHat=0.005 ** 2, stiffness_stretch=1e6, stiffness_bending=1e3,
+                         g=(0.0, -9.8, 0.0), dt=1.0 / 600.0)
+        sim.method = method
+        sim.enable_collision_handling = True
+        sim.bvh_mode = "tri"
+        sim.padding = 0.0025
+        sim.records_per_vertex = 128
+        sel = torch.zeros(cloth.num_verts, device="cuda")
+        for k in range(layers):
+            sel[k * n * n] = 1
+            sel[k * n * n + n - 1] = 1
+        sim.set_fixed_vertices(sel)
+        return cloth, sim
+
+    res = {}
+    for method in ("jacobi", "gs"):
+        cloth, sim = make(method)
+        for _ in range(a.warmup * 10):
+            sim.forward(1, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps * 10):
+            sim.forward(1, 1)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[method] = dict(value=cloth.num_verts * 10 * a.steps / el, ms_per_step=el / a.steps * 1e3,
+                           status=sim.status())
+        if method == "gs":
+            res[method]["colors"] = sim.coloring()["n_colors"]
+    V = layers * n * n
+    out = _line(a, res["jacobi"]["value"], V, 10 * a.steps, res["jacobi"]["ms_per_step"] * a.steps / 1e3,
+                f"{layers} stacked {n}x{n} layers ({V} vertices) on an icosphere, self + obstacle collision, "
+                "thickness 5 mm, triangle LBVH rebuilt every substep, 10 substeps x 1 iteration per step, "
+                "Jacobi (value) and Gauss-Seidel",
+                {"jacobi": res["jacobi"], "gauss_seidel": res["gs"]})
     print(json.dumps(out), flush=True)
 
 

@@ -98,7 +98,8 @@ int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, float* v4_al
 
 /* Launch shape of every spring-pass kernel (no reference counterpart: set by the
  * autotuner): threads per block {64, 128, 256} x vertices per thread {1, 2, 4}.
- * Process-wide; results do not depend on it. */
+ * The library default (a frame descriptor may override it for its call);
+ * results do not depend on it. */
 int slx_set_spring_launch(int block, int vpt);
 
 /* (f, m_inv) -> x4.w = y4.w = f*m_inv, v4.w = f (set_fixed_vertices,
@@ -549,6 +550,8 @@ typedef struct slx_tree {
   uint32_t* disp;            /* [1] */
 } slx_tree;
 
+typedef struct slx_frame_ctx slx_frame_ctx;   /* opaque */
+
 typedef struct slx_frame_desc {
   /* dynamic mesh */
   int32_t nv;
@@ -618,10 +621,30 @@ typedef struct slx_frame_desc {
      (ee.pair_cnt / ee.pair_rec), and the velocity pass applies them.  Single
      meshes only (no batched trees). */
   slx_edge_sets ee;
+  /* per-solver execution state (no reference counterpart).  ctx: the side
+     streams and events this frame forks onto (slx_frame_ctx_create); null uses
+     one shared set per device, so two solvers that leave it null must not run
+     frames concurrently.  Launch knobs for this call only, 0 = the library
+     default (slx_set_spring_launch / slx_set_walk_stack / slx_set_active_grid /
+     slx_set_refit_shape); results never depend on them. */
+  slx_frame_ctx* ctx;
+  int32_t spring_block;                  /* 64, 128 or 256 */
+  int32_t spring_vpt;                    /* 1, 2 or 4 */
+  int32_t walk_stack;                    /* 16, 24, 32 or 64 */
+  int32_t active_grid;                   /* 1..65535 */
+  int32_t refit_sub;                     /* 1..2048 */
+  int32_t refit_top;                     /* 1..2048 */
+  int32_t walk_wave;                     /* cloth-vertex walks of the cloth tree: 1 = one shared walk per
+                                            wavefront over the 4-wide nodes, -1 = one walk per lane */
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);
 int slx_frame(const slx_frame_desc* d, slx_stream_t stream);
+
+/* A solver's own side streams and events (default priority, created on the
+ * current device), so that frames of different solvers never share them. */
+int slx_frame_ctx_create(slx_frame_ctx** out);
+int slx_frame_ctx_destroy(slx_frame_ctx* ctx);
 
 #ifdef __cplusplus
 }

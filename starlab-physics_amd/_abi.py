@@ -75,6 +75,8 @@ SIGNATURES = {
                              PTR, PTR, C.c_size_t, STREAM],
     "slx_frame_ws_bytes": [PTR],
     "slx_frame": [PTR, STREAM],
+    "slx_frame_ctx_create": [PTR],
+    "slx_frame_ctx_destroy": [PTR],
     "slx_seg_build_ws_bytes": [I32, I32],
     "slx_seg_build": [I32, I32, I32, PTR, PTR, F32, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR,
                       PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, PTR, C.c_size_t, STREAM],
@@ -137,7 +139,9 @@ class FrameDesc(C.Structure):
                 ("ws_bytes", C.c_size_t), ("velocity", I32), ("mu", F32), ("vs4", PTR), ("m_inv_st", PTR),
                 ("pair_cap", I32), ("pair_cnt", PTR), ("pair_rec", PTR), ("cache", CandCache * 3), ("cache_every", I32),
                 ("n_sewn", I32), ("sew_verts", PTR), ("sew_off", PTR), ("sew_list", PTR), ("sew_pairs", PTR),
-                ("k_sew", F32), ("sew_ws", PTR), ("ee", EdgeSets)]
+                ("k_sew", F32), ("sew_ws", PTR), ("ee", EdgeSets), ("ctx", PTR), ("spring_block", I32),
+                ("spring_vpt", I32), ("walk_stack", I32), ("active_grid", I32), ("refit_sub", I32),
+                ("refit_top", I32), ("walk_wave", I32)]
 
 
 class SlxError(RuntimeError):
@@ -172,6 +176,21 @@ def call(name: str, *args) -> None:
     rc = getattr(lib(), name)(*args)
     if rc != 0:
         raise SlxError(f"{name} failed ({rc}): {last_error()}")
+
+
+STATUS_BITS = {1: "candidate list overflow (contacts past the capacity dropped)",
+               2: "traversal stack overflow",
+               4: "face centre outside the 32^3 cell grid",
+               8: "contact records per vertex exceeded (deltas dropped)",
+               16: "contacting pairs per query exceeded (dropped)"}
+
+
+def status_flags(st: int) -> str:
+    """The SLX_STATUS_* bits of a status word, named."""
+    names = [v for b, v in STATUS_BITS.items() if st & b]
+    if st & ~sum(STATUS_BITS):
+        names.append(f"unknown bits {st & ~sum(STATUS_BITS):#x}")
+    return "; ".join(names)
 
 
 def ptr(t) -> int:

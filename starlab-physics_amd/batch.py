@@ -10,8 +10,8 @@ one subtree per instance of a single LBVH built with one segmented sort
 
 A frame is captured once into a HIP graph (torch.cuda.CUDAGraph) and replayed,
 which removes every per-kernel launch from the host loop.  `autotune` picks the
-spring kernels' launch shape (threads per block x vertices per thread) by timing
-replays of the captured frame.
+spring kernels' launch shape (threads per block x vertices per thread) for this
+scene's solver by timing replays of the captured frame.
 """
 from __future__ import annotations
 
@@ -19,7 +19,6 @@ import time
 
 import numpy as np
 
-from . import _abi
 from . import scene as _scene
 from .mesh import Mesh
 from .solver import Solver
@@ -107,7 +106,7 @@ class BatchedScene:
         state = [t.clone() for t in (self.cloth.x4, self.cloth.v4)]
         for bs in blocks:
             for vpt in vpts:
-                _abi.call("slx_set_spring_launch", bs, vpt)
+                self.sim.spring_launch = (bs, vpt)
                 self.cloth.x4.copy_(state[0])
                 self.cloth.v4.copy_(state[1])
                 self.capture(n_substeps, n_iters)
@@ -118,7 +117,7 @@ class BatchedScene:
                 torch.cuda.synchronize()
                 results[(bs, vpt)] = (time.perf_counter() - t0) / reps
         best = min(results, key=results.get)
-        _abi.call("slx_set_spring_launch", *best)
+        self.sim.spring_launch = best
         self.cloth.x4.copy_(state[0])
         self.cloth.v4.copy_(state[1])
         self.capture(n_substeps, n_iters)

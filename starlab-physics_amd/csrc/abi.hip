@@ -28,6 +28,16 @@ int check_launch(const char* what) {
   return SLX_OK;
 }
 
+// ---- launch knobs ----
+static Knobs g_knobs = {256, 1, 32, 2048, 2048, 2048, 0};
+static thread_local const Knobs* t_knobs = nullptr;
+
+Knobs& knob_defaults() { return g_knobs; }
+const Knobs& knobs() { return t_knobs ? *t_knobs : g_knobs; }
+
+KnobScope::KnobScope(const Knobs& k) : k_(k), prev_(t_knobs) { t_knobs = &k_; }
+KnobScope::~KnobScope() { t_knobs = prev_; }
+
 // ---- kernel timer: hipEvent pairs recorded on the launch stream ----
 struct TimerRec {
   const char* name;

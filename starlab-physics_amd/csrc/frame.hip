@@ -18,6 +18,8 @@
 // Every size the kernels need is either static or device-resident, so the
 // frame enqueues work without any host synchronisation and can be captured in
 // a HIP graph.
+#include <mutex>
+
 #include "slx_common.h"
 
 extern "C" {
@@ -33,30 +35,70 @@ using namespace slx;
 
 bool batched(const slx_tree& t) { return t.mode == 1 && t.n_inst > 1; }
 
-// Two extra streams per device (fork/join with events, also under graph capture;
-// default priority): s for the per-substep refit, which depends only on x and so
-// runs beside the spring pass; q for the static-tree queries, which run beside
-// the dynamic-tree queries.
-struct SideStream {
+}  // namespace
+
+// Two extra streams (fork/join with events, also under graph capture; default
+// priority): s for the per-substep refit, which depends only on x and so runs
+// beside the spring pass; q for the static-tree queries, which run beside the
+// dynamic-tree queries.  Each solver owns a set (slx_frame_ctx); frames whose
+// descriptor carries none share one set per device.
+struct slx_frame_ctx {
+  int device = -1;
   hipStream_t s = nullptr, q = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, qfork = nullptr, qjoin = nullptr, sub = nullptr, tvjoin = nullptr;
 };
-SideStream g_side[64];
 
-int side_stream(SideStream** out) {
+namespace {
+
+using SideStream = slx_frame_ctx;
+
+void destroy_side(SideStream& ss) {
+  for (hipEvent_t* e : {&ss.fork, &ss.join, &ss.qfork, &ss.qjoin, &ss.sub, &ss.tvjoin})
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
+  for (hipStream_t* q : {&ss.s, &ss.q})
+    if (*q) {
+      (void)hipStreamDestroy(*q);
+      *q = nullptr;
+    }
+}
+
+int create_side(SideStream& ss, int dev) {
+  ss.device = dev;
+  if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ss.q, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.qfork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.sub, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ss.tvjoin, hipEventDisableTiming) != hipSuccess) {
+    destroy_side(ss);
+    return fail(SLX_EHIP, "slx_frame: side stream creation failed");
+  }
+  return SLX_OK;
+}
+
+SideStream g_side[64];
+std::mutex g_side_mu;
+
+// the descriptor's context (on the current device), else the device's shared set
+int side_stream(slx_frame_ctx* ctx, SideStream** out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(SLX_EHIP, "slx_frame: no device");
+  if (ctx) {
+    SLX_REQUIRE(ctx->device == dev, "slx_frame: the context belongs to device %d, the current device is %d",
+                ctx->device, dev);
+    *out = ctx;
+    return SLX_OK;
+  }
+  std::lock_guard<std::mutex> lock(g_side_mu);
   SideStream& ss = g_side[dev];
   if (!ss.s) {
-    if (hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ss.q, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.qfork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.qjoin, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.sub, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ss.tvjoin, hipEventDisableTiming) != hipSuccess)
-      return fail(SLX_EHIP, "slx_frame: side stream creation failed");
+    int rc = create_side(ss, dev);
+    if (rc) return rc;
   }
   *out = &ss;
   return SLX_OK;
@@ -136,8 +178,56 @@ extern "C" size_t slx_frame_ws_bytes(const slx_frame_desc* d) {
   return w;
 }
 
+extern "C" int slx_frame_ctx_create(slx_frame_ctx** out) {
+  SLX_REQUIRE(out, "slx_frame_ctx_create: null pointer");
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(SLX_EHIP, "slx_frame_ctx_create: no device");
+  slx_frame_ctx* c = new slx_frame_ctx();
+  int rc = create_side(*c, dev);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return SLX_OK;
+}
+
+extern "C" int slx_frame_ctx_destroy(slx_frame_ctx* ctx) {
+  if (!ctx) return SLX_OK;
+  destroy_side(*ctx);
+  delete ctx;
+  return SLX_OK;
+}
+
+static int frame_impl(const slx_frame_desc* d, slx_stream_t stream);
+
 extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(d, "slx_frame: null descriptor");
+  SLX_REQUIRE(d->spring_block == 0 || d->spring_block == 64 || d->spring_block == 128 || d->spring_block == 256,
+              "slx_frame: spring_block must be 0, 64, 128 or 256");
+  SLX_REQUIRE(d->spring_vpt == 0 || d->spring_vpt == 1 || d->spring_vpt == 2 || d->spring_vpt == 4,
+              "slx_frame: spring_vpt must be 0, 1, 2 or 4");
+  SLX_REQUIRE(d->walk_stack == 0 || d->walk_stack == 16 || d->walk_stack == 24 || d->walk_stack == 32 ||
+                  d->walk_stack == 64,
+              "slx_frame: walk_stack must be 0, 16, 24, 32 or 64");
+  SLX_REQUIRE(d->active_grid >= 0 && d->active_grid <= 65535, "slx_frame: active_grid outside 0..65535");
+  SLX_REQUIRE(d->refit_sub >= 0 && d->refit_sub <= 2048 && d->refit_top >= 0 && d->refit_top <= 2048,
+              "slx_frame: refit_sub / refit_top outside 0..2048");
+  SLX_REQUIRE(d->walk_wave >= -1 && d->walk_wave <= 1, "slx_frame: walk_wave must be -1, 0 or 1");
+  Knobs k = knobs();
+  if (d->spring_block) k.spring_block = d->spring_block;
+  if (d->spring_vpt) k.spring_vpt = d->spring_vpt;
+  if (d->walk_stack) k.walk_stack = d->walk_stack;
+  if (d->active_grid) k.active_grid = d->active_grid;
+  if (d->refit_sub) k.refit_sub = d->refit_sub;
+  if (d->refit_top) k.refit_top = d->refit_top;
+  if (d->walk_wave) k.walk_wave = d->walk_wave > 0 ? 1 : 0;
+  KnobScope scope(k);
+  return frame_impl(d, stream);
+}
+
+static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
   SLX_REQUIRE(d->nv >= 0 && d->nvs >= 0 && d->n_substeps >= 0 && d->n_iters >= 1, "slx_frame: bad sizes");
   if (d->nv == 0 || d->n_substeps == 0) return SLX_OK;
   SLX_REQUIRE(d->x4 && d->v4 && d->y4 && d->y4_alt && d->fixed && d->m_inv && d->adj_slice_off &&
@@ -257,7 +347,7 @@ extern "C" int slx_frame(const slx_frame_desc* d, slx_stream_t stream) {
   // them at zero for the next pass, so no pass needs a zeroing launch of its own.
   ZeroList after{};
   if (coll) {
-    rc = side_stream(&side);
+    rc = side_stream(d->ctx, &side);
     if (rc) return rc;
     auto add = [&](int32_t* p) {
       if (p) after.p[after.n++] = p;

@@ -37,8 +37,7 @@ int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* to
 constexpr int kStack = SLX_TRAV_STACK;   // traversal stack depth (entries), twice the reference's 32
 // Shallow stacks of the first-tier triangle walk (LDS per block = depth x 128 x 4
 // bytes, which bounds the waves in flight); queries that overflow one are walked
-// again with kStack entries.
-int g_walk_stack = 32;
+// again with kStack entries (depth: knobs().walk_stack).
 constexpr int kTravBS = 128;     // traversal block size
 
 // ----------------------------------------------------------------- AABBs
@@ -256,8 +255,6 @@ constexpr int kSubHalf = 1024;
 constexpr int kSubHalfBS = 512;
 constexpr int kTopMax = 2048;   // top nodes with LDS state (LDS of k_refit_top)
 constexpr int kTopBS = 1024;
-int g_refit_sub = kSubMax;
-int g_refit_top = kTopMax;
 
 struct Sched {
   const int32_t* p;
@@ -704,7 +701,8 @@ size_t refit_sched_ints(int n) { return 8 + 9 * (size_t)(n > 0 ? n : 1); }
 
 int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_t* rl, const int32_t* rr,
                    int32_t* sched, hipStream_t st, bool even_lines) {
-  SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, g_refit_sub, g_refit_top, even_lines ? 1 : 0);
+  SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, knobs().refit_sub, knobs().refit_top,
+             even_lines ? 1 : 0);
   SLX_LAUNCH("sched", k_sched_mark, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, st, n, parent, rl, rr, sched);
   SLX_LAUNCH("sched", k_sched_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cb, parent, sched);
   if (even_lines) SLX_LAUNCH("sched", k_sched_parity, dim3(1), dim3(1024), 0, st, n, parent, sched);
@@ -718,7 +716,7 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
           const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st,
           hipEvent_t after_sub = nullptr) {
   constexpr bool COUNTED = LEAF == kLeafCells;
-  if (g_refit_sub <= kSubHalf) {
+  if (knobs().refit_sub <= kSubHalf) {
     const int grid = n < 2 * kSubGrid ? n : 2 * kSubGrid;
     SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubHalf, kSubHalfBS>), dim3(grid), dim3(kSubHalfBS), 0, st, n, sched, src,
                ca, cb, parent, num, nlo, nhi);
@@ -893,8 +891,8 @@ extern "C" size_t slx_refit_sched_ints(int n_leaves) { return refit_sched_ints(n
 extern "C" int slx_set_refit_shape(int sub_leaves, int top_lds) {
   SLX_REQUIRE(sub_leaves >= 1 && sub_leaves <= kSubMax, "slx_set_refit_shape: sub_leaves outside [1, 2048]");
   SLX_REQUIRE(top_lds >= 0 && top_lds <= kTopMax, "slx_set_refit_shape: top_lds outside [0, 2048]");
-  g_refit_sub = sub_leaves;
-  g_refit_top = top_lds;
+  knob_defaults().refit_sub = sub_leaves;
+  knob_defaults().refit_top = top_lds;
   return SLX_OK;
 }
 
@@ -1268,6 +1266,142 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
 
 namespace slx {
 
+// Wavefront walk over the 4-wide node array, for fine trees whose queries come in
+// a coherent order (neighbouring vertices in a wavefront): the 64 lanes share one
+// depth-first walk over the union of their boxes, with scalar line loads and the
+// stack one entry per lane register.  A popped line's four grandchild slots are
+// tested by every lane against its own box.  Slots are visited in the order 3, 2,
+// 1, 0 (the reference's right-first order, as in k_traverse_quad): the leaf slots
+// that come before the line's last internal slot in that order are reported at
+// once, the internal slots and the leaves after them are pushed in slot order (a
+// pushed leaf, tagged kCacheNode, is reported when popped, its box reloaded).
+// Each lane reports only the leaves its own box overlaps, in the shared walk's
+// order, which restricted to them is its own walk's order: every lane's list is
+// the one the per-lane walk gives.  BUILD, co and n_active as for k_traverse_quad;
+// a stack overflow (shared) sends every active lane of the wavefront to co.redo.
+template <bool BUILD>
+__global__ void __launch_bounds__(256)
+k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe, int nl,
+                     const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                     const float4* __restrict__ quad, int cap, int32_t* __restrict__ cand,
+                     int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
+                     const int32_t* __restrict__ n_active, CacheOut co) {
+  const int limit = n_active ? min(nq, *n_active) : nq;
+  for (int base = blockIdx.x * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
+  const int lane = threadIdx.x & 63;
+  const bool active = t < limit;
+  const int q = active ? (order ? order[t] : t) : 0;
+  float3 a, b, aw, bw;
+  if (active) {
+    query_box(q4, qe, q, pad, a, b);
+    aw = a;
+    bw = b;
+    if (BUILD) {                             // point queries only
+      const float3 p = xyz(q4[q]);
+      const float pw = co.pad_walk;
+      aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
+      bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
+      co.q0[t] = make_float4(p.x, p.y, p.z, 0.f);
+    }
+  } else {                                   // an empty box: overlaps nothing
+    const float inf = __int_as_float(0x7f800000);
+    a = aw = make_float3(inf, inf, inf);
+    b = bw = make_float3(-inf, -inf, -inf);
+  }
+  int cnt = 0, ccnt = 0, flags = 0;
+  auto emit = [&](int f) {
+    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
+    else flags |= kStatusCandOverflow;
+    ++cnt;
+  };
+  // a leaf (node id, face id, box) whose box this lane's walk box overlaps: to the
+  // cache (BUILD) and, if it overlaps the exact box, to the list.  (A line slot
+  // holds the node id in lo.w and the face in hi.w; the binary node arrays hold
+  // a leaf's face in lo.w.)
+  LeafPack pack;
+  auto leaf = [&](int node, int face, const float4& lo, const float4& hi) {
+    if (BUILD) {
+      pack.put(ccnt, node, co.cap, nq, t, co.leaves);
+      ++ccnt;
+      if (overlap(a, b, xyz(lo), xyz(hi))) emit(face);
+    } else {
+      emit(face);
+    }
+  };
+  uint32_t sreg = 0;                         // this lane's stack slot
+  int sp = 0;                                // wave-uniform depth
+  auto push = [&](uint32_t v) {
+    if (sp < kWave) {
+      if (lane == sp) sreg = v;
+      ++sp;
+    } else {
+      flags |= kStatusStackOverflow;
+    }
+  };
+  if (nl > 0) {
+    const float4 lo = nlo[0], hi = nhi[0];
+    const bool ov = overlap(aw, bw, xyz(lo), xyz(hi));
+    if (__ballot(ov)) {
+      if (nl == 1) {
+        if (ov) leaf(0, __float_as_int(lo.w), lo, hi);
+      } else {
+        push(0u);
+      }
+    }
+  }
+  while (__builtin_amdgcn_readfirstlane(sp) > 0) {
+    sp = __builtin_amdgcn_readfirstlane(sp) - 1;
+    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)sreg, sp);
+    if (e & kCacheNode) {                    // a leaf pushed below an internal slot
+      const int L = (int)(e & kLow30);
+      const float4 lo = nlo[L], hi = nhi[L];
+      if (overlap(aw, bw, xyz(lo), xyz(hi))) leaf(L, __float_as_int(lo.w), lo, hi);
+      continue;
+    }
+    const float4* qn = quad + 8 * (size_t)e;
+    float4 lo[4], hi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      lo[k] = qn[2 * k];
+      hi[k] = qn[2 * k + 1];
+    }
+    bool ov[4], any[4];
+    int face[4], node[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ov[k] = overlap(aw, bw, xyz(lo[k]), xyz(hi[k]));
+      any[k] = __ballot(ov[k]) != 0ull;
+      face[k] = __float_as_int(hi[k].w);
+      node[k] = __float_as_int(lo[k].w);
+    }
+    // the last internal slot some lane overlaps, in visiting order 3, 2, 1, 0
+    int kstar = -1;
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+      if (kstar < 0 && any[k] && face[k] < 0) kstar = k;
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+      if (k > kstar && ov[k] && face[k] >= 0) leaf(node[k], face[k], lo[k], hi[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k <= kstar && any[k]) push(face[k] >= 0 ? ((uint32_t)node[k] | kCacheNode) : (uint32_t)node[k]);
+    if (co.redo && (flags & kStatusStackOverflow)) break;
+  }
+  if (!active) return;
+  if (co.redo && (flags & kStatusStackOverflow)) {     // walked again by the second tier
+    if (BUILD) co.count[t] = INT_MAX;
+    co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
+    return;
+  }
+  if (BUILD) {
+    pack.finish(ccnt, co.cap, nq, t, co.leaves);
+    co.count[t] = ccnt;
+  }
+  cand_n[q] = cnt < cap ? cnt : cap;
+  if (flags && status) atomicOr(status, flags);
+  }(base + (int)threadIdx.x);
+}
+
 // Triangle-tree walk over the 4-wide node array: a popped node's four grandchild
 // slots (one 128-byte line) are tested together, so one dependent load covers two
 // tree levels.  Overlapping slots are pushed in slot order (a leaf as an emit
@@ -1396,14 +1530,14 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
 }  // namespace slx
 
 namespace slx {
-// A launch whose query count is on the device (n_active) gets at most this many
-// workgroups and loops over the queries: with few or none active it ends in
-// microseconds instead of dispatching a workgroup per possible query.
-constexpr int kActiveGrid = 2048;
-int g_active_grid = kActiveGrid;
+// A launch whose query count is on the device (n_active) gets at most
+// knobs().active_grid workgroups (default 2048) and loops over the queries: with
+// few or none active it ends in microseconds instead of dispatching a workgroup
+// per possible query.
 static inline int active_grid(int nq, int bs, const int32_t* n_active) {
   const int g = blocks_for(nq, bs);
-  return n_active && g > g_active_grid ? g_active_grid : g;
+  const int cap = knobs().active_grid;
+  return n_active && g > cap ? cap : g;
 }
 
 int traverse_launch(const char* name, int mode, int nq, const float* q4, float padding, int n_leaves,
@@ -1578,9 +1712,10 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
   SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, qedges, n_leaves,    \
              (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
              nullptr, order, n_active, co)
-  if (g_walk_stack <= 16) SLX_QUAD_TIER(16);
-  else if (g_walk_stack <= 24) SLX_QUAD_TIER(24);
-  else if (g_walk_stack <= 32) SLX_QUAD_TIER(32);
+  const int depth = knobs().walk_stack;
+  if (depth <= 16) SLX_QUAD_TIER(16);
+  else if (depth <= 24) SLX_QUAD_TIER(24);
+  else if (depth <= 32) SLX_QUAD_TIER(32);
   else SLX_QUAD_TIER(kStack);
 #undef SLX_QUAD_TIER
   return SLX_OK;
@@ -1609,6 +1744,11 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
     return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
                            nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order, n_active, qedges);
   if (!prezeroed) SLX_LAUNCH("fill", k_zero_one, dim3(1), dim3(64), 0, st, c->redo);
+  // dense point queries of a fine tree (about as many queries as leaves, in a
+  // coherent order): the 4-wide wavefront walk when knobs().walk_wave is on (off
+  // by default: on the 1M-vertex drape its cache build took 1018 us against the
+  // per-lane walk's 366 us -- the shared walk is a long chain of dependent loads)
+  const bool quad_wave = !wave && knobs().walk_wave && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
   int rc = SLX_OK;
   if (kind == 1) {
     const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
@@ -1616,6 +1756,10 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
       SLX_LAUNCH("cache_build_wave", k_traverse_tri_wave<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
                  (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand,
                  cand_n, status, order, nullptr, co);
+    else if (quad_wave)
+      SLX_LAUNCH("cache_build", k_traverse_quad_wave<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
+                 (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
+                 (const float4*)quad4, cap, cand, cand_n, status, order, nullptr, co);
     else
       rc = quad_first_tier<true>("cache_build", nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n,
                                  status, st, order, co, nullptr, nullptr);
@@ -1626,8 +1770,13 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   } else {
     CacheOut co{};
     co.redo = c->redo;
-    rc = quad_first_tier<false>(name, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n, status, st,
-                                order, co, qedges, n_active);
+    if (quad_wave)
+      SLX_LAUNCH(name, k_traverse_quad_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
+                 (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
+                 (const float4*)quad4, cap, cand, cand_n, status, order, nullptr, co);
+    else
+      rc = quad_first_tier<false>(name, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n, status,
+                                  st, order, co, qedges, n_active);
   }
   if (rc) return rc;
   // the queries the first tier could not serve: a full walk over the list built on the device
@@ -1648,13 +1797,13 @@ extern "C" int slx_traverse_cached(int kind, int nq, const float* q4, float padd
 
 extern "C" int slx_set_active_grid(int blocks) {
   SLX_REQUIRE(blocks >= 1 && blocks <= 65535, "slx_set_active_grid: 1..65535 workgroups");
-  g_active_grid = blocks;
+  knob_defaults().active_grid = blocks;
   return SLX_OK;
 }
 
 extern "C" int slx_set_walk_stack(int depth) {
   SLX_REQUIRE(depth == 16 || depth == 24 || depth == 32 || depth == 64, "slx_set_walk_stack: 16, 24, 32 or 64");
-  g_walk_stack = depth;
+  knob_defaults().walk_stack = depth;
   return SLX_OK;
 }
 

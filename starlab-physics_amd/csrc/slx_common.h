@@ -185,6 +185,24 @@ int ee_gate_static(const slx_edge_sets* ee, int nes, const float4* xs4, hipStrea
 int ee_gate(const slx_edge_sets* ee, int nv, const float4* x4, const float4* y4, float pad, bool new_x,
             int32_t* live_st, int32_t* live_dy, hipStream_t st);
 
+// Launch knobs (results never depend on them): the library defaults, set by
+// the slx_set_* entry points, and the values an slx_frame call takes from its
+// descriptor for that call on the calling thread (KnobScope).
+struct Knobs {
+  int spring_block, spring_vpt, walk_stack, active_grid, refit_sub, refit_top, walk_wave;
+};
+Knobs& knob_defaults();
+const Knobs& knobs();
+class KnobScope {
+ public:
+  explicit KnobScope(const Knobs& k);
+  ~KnobScope();
+
+ private:
+  Knobs k_;
+  const Knobs* prev_;
+};
+
 // optional per-kernel event timer (slx_timer_enable); no-ops when disabled
 void timer_begin(const char* name, hipStream_t s, int* slot);
 void timer_end(int slot, hipStream_t s);

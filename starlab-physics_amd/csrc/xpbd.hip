@@ -188,8 +188,6 @@ k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
   }
 }
 
-// launch shape of the spring kernels (autotuned by the host: block x vertices/thread)
-static int g_spring_block = 256, g_spring_vpt = 1;
 
 int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4* v4, const float* fixed,
                   const float4* y_in, float4* y_out, float4* x_out, float4* v_out, const int32_t* slice_off,
@@ -197,7 +195,8 @@ int spring_launch(bool pred, bool final, int nv, const float4* x4, const float4*
                   float gy, float gz, float damping, hipStream_t st) {
   if (nv <= 0) return SLX_OK;
   const float3 gdt2 = make_float3((gx * dt) * dt, (gy * dt) * dt, (gz * dt) * dt);
-  const int bs = g_spring_block, vpt = g_spring_vpt;
+  // launch shape (autotuned by the host: block x vertices/thread)
+  const int bs = knobs().spring_block, vpt = knobs().spring_vpt;
   const dim3 grid(blocks_for(nv, bs * vpt)), blk(bs);
   const int2* E = (const int2*)entries;
 #define SLX_SPRING(NAME, P, F, V)                                                                              \
@@ -318,7 +317,7 @@ extern "C" int slx_substeps_plain(int nv, float* x4, float* v4, float* x4_alt, f
 extern "C" int slx_set_spring_launch(int block, int vpt) {
   SLX_REQUIRE(block == 64 || block == 128 || block == 256, "slx_set_spring_launch: block must be 64, 128 or 256");
   SLX_REQUIRE(vpt == 1 || vpt == 2 || vpt == 4, "slx_set_spring_launch: vertices per thread must be 1, 2 or 4");
-  slx::g_spring_block = block;
-  slx::g_spring_vpt = vpt;
+  slx::knob_defaults().spring_block = block;
+  slx::knob_defaults().spring_vpt = vpt;
   return SLX_OK;
 }

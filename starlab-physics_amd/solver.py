@@ -41,6 +41,18 @@ Extensions (all default to the reference's behaviour):
                                    triangle trees); False walks and tests every
                                    edge
   status()                         device status word (overflow flags)
+  static_rebuild_every_frame (True)  rebuild the static tree at every forward as
+                                   the reference does (False: build it once and
+                                   again only when the padding changes)
+  check_status (False)             read the status word after every forward
+                                   and raise SlxError naming the flags set
+  spring_launch (None)             (threads per block, vertices per thread) of
+                                   the spring kernels, this solver only (None =
+                                   the library default); walk_stack,
+                                   active_grid, refit_shape likewise for the
+                                   broad phase (results never depend on them)
+Each Solver owns its side streams and events (slx_frame_ctx), so solvers in
+different threads never share them.
 `step` is an alias of `forward`.
 """
 from __future__ import annotations
@@ -109,6 +121,14 @@ class Solver:
         self._status = torch.zeros(1, dtype=torch.int32, device=dev)
         self._cd = None            # collision buffers, allocated on first use
         self._ws = None
+        self.check_status = False
+        self.static_rebuild_every_frame = True
+        self.spring_launch = None
+        self.walk_stack = None
+        self.active_grid = None
+        self.refit_shape = None
+        self.walk_wave = None
+        self._ctx = None           # side streams + events of this solver
 
     # ------------------------------------------------------------ reference API
     def reset(self):
@@ -138,6 +158,25 @@ class Solver:
         n_iters = self.n_iters if n_iters is None else n_iters
         d = self._desc(n_substeps, n_iters)
         _abi.call("slx_frame", _abi.C.byref(d), _abi.stream_handle())
+        if self.check_status:
+            st = self.status()
+            if st:
+                raise _abi.SlxError(f"slx_frame status {st:#x}: {_abi.status_flags(st)}")
+
+    def __del__(self):
+        ctx, self._ctx = getattr(self, "_ctx", None), None
+        if ctx:
+            try:
+                _abi.lib().slx_frame_ctx_destroy(ctx)
+            except Exception:
+                pass
+
+    def _frame_ctx(self):
+        if self._ctx is None:
+            h = _abi.C.c_void_p()
+            _abi.call("slx_frame_ctx_create", _abi.C.byref(h))
+            self._ctx = h.value
+        return self._ctx
 
     step = forward
 
@@ -188,7 +227,7 @@ class Solver:
         cd["cand_n"] = torch.zeros(2 * nv + nvs + 1, **i32)
         cd["pair_num"] = torch.zeros(2 * nv + nvs + 1, **i32)
         cd["rec_cap"] = int(self.records_per_vertex)
-        cd["static_built"] = False
+        cd["static_built"] = None          # padding the static tree was built with
         if s is not None:
             s.xs4 = s.x4
         self._cd = cd
@@ -288,6 +327,17 @@ class Solver:
         d.collision = 1 if self.enable_collision_handling else 0
         d.refit_every_substep = 1 if self.refit_every_substep else 0
         d.status = P(self._status)
+        d.ctx = self._frame_ctx()
+        if self.spring_launch is not None:
+            d.spring_block, d.spring_vpt = (int(v) for v in self.spring_launch)
+        if self.walk_stack is not None:
+            d.walk_stack = int(self.walk_stack)
+        if self.active_grid is not None:
+            d.active_grid = int(self.active_grid)
+        if self.refit_shape is not None:
+            d.refit_sub, d.refit_top = (int(v) for v in self.refit_shape)
+        if self.walk_wave is not None:
+            d.walk_wave = 1 if self.walk_wave else -1
         if self.method not in ("jacobi", "gs"):
             raise ValueError("method must be 'jacobi' or 'gs'")
         if self.method == "gs":
@@ -302,9 +352,13 @@ class Solver:
             if s is not None:
                 d.xs4, d.faces_st = P(s.x4), P(s.face_indices)
                 d.st = cd["st"].struct()
-                # the static mesh never moves: its tree is built once and reused
-                d.rebuild_static = 0 if cd["static_built"] else 1
-                cd["static_built"] = True
+                # the reference rebuilds the static tree every frame with the current
+                # padding (XPBD.py:454-458); with static_rebuild_every_frame False it
+                # is built once and rebuilt only when the padding changes
+                pad = _f32(self.padding)
+                rebuild = self.static_rebuild_every_frame or cd["static_built"] != pad
+                d.rebuild_static = 1 if rebuild else 0
+                cd["static_built"] = pad
             d.dyn = cd["dyn"].struct()
             d.cap, d.rec_cap = self.cache_size, cd["rec_cap"]
             d.cand, d.cand_n, d.pair_num = P(cd["cand"]), P(cd["cand_n"]), P(cd["pair_num"])

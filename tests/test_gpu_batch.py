@@ -82,22 +82,48 @@ def test_graph_replay_equals_eager(slx):
 
 
 def test_launch_shape_does_not_change_results(slx):
+    """Per-solver launch shapes (spring kernels, walk stack, active grid, refit
+    shape), and the library default set process-wide, give the same frames."""
     A = __import__("importlib").import_module("starlab-physics_amd._abi")
     outs = []
-    for shape in ((256, 1), (64, 4), (128, 2)):
-        A.call("slx_set_spring_launch", *shape)
+    for shape, walk, grid, refit in (((256, 1), None, None, None), ((64, 4), 16, 3, (1024, 512)),
+                                     ((128, 2), 64, 1, (2048, 2048))):
         _, _, _, bs = small_batch(slx, seed=7)
+        bs.sim.spring_launch, bs.sim.walk_stack, bs.sim.active_grid, bs.sim.refit_shape = shape, walk, grid, refit
         for _ in range(3):
             bs.forward(10, 2)
         outs.append(bs.cloth.positions())
+    A.call("slx_set_spring_launch", 64, 2)
+    _, _, _, bs = small_batch(slx, seed=7)
+    for _ in range(3):
+        bs.forward(10, 2)
+    outs.append(bs.cloth.positions())
     A.call("slx_set_spring_launch", 256, 1)
-    np.testing.assert_array_equal(outs[0], outs[1])
-    np.testing.assert_array_equal(outs[0], outs[2])
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0], o)
+
+
+def test_two_solvers_keep_their_own_launch_shapes(slx):
+    """Two scenes with different launch shapes interleaved frame by frame equal
+    the same scenes run one after the other."""
+    runs = []
+    for interleave in (True, False):
+        _, _, _, a = small_batch(slx, seed=11)
+        _, _, _, b = small_batch(slx, seed=11)
+        a.sim.spring_launch, b.sim.spring_launch = (64, 4), (256, 2)
+        a.sim.walk_stack, b.sim.walk_stack = 16, 64
+        order = [a, b] * 3 if interleave else [a] * 3 + [b] * 3
+        for s in order:
+            s.forward(10, 2)
+        runs.append((a.cloth.positions(), b.cloth.positions()))
+        assert a.sim._ctx != b.sim._ctx
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    np.testing.assert_array_equal(runs[0][0], runs[0][1])
 
 
 def test_autotune_picks_a_valid_shape(slx):
     batch, cloths, spheres, bs = small_batch(slx, seed=9)
     best, res = bs.autotune(5, 1, reps=2)
     assert best in res and len(res) == 9
-    A = __import__("importlib").import_module("starlab-physics_amd._abi")
-    A.call("slx_set_spring_launch", 256, 1)
+    assert bs.sim.spring_launch == best

@@ -15,13 +15,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def drape(n):
+def drape(n, walk_wave=None):
     import bench
     slx = importlib.import_module("starlab-physics_amd")
     from oracle import oracle as orc
     cloth, terr, sim = bench.build_scene(slx, n, True)
     sim.enable_velocity_update = True          # what the benchmark runs (friction keeps the drape at rest)
     sim.candidate_cache = True
+    sim.walk_wave = walk_wave
     st = dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
     ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
@@ -30,11 +31,11 @@ def drape(n):
     return cloth, sim, ref
 
 
-def check_frames(n, frames, substeps=20, iters=2):
-    cloth, sim, ref = drape(n)
+def check_frames(n, frames, substeps=20, iters=2, walk_wave=None):
+    cloth, sim, ref = drape(n, walk_wave)
     pairs = 0
     for f in range(frames):
-        print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters}", flush=True)
+        print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters} walk_wave={walk_wave}", flush=True)
         sim.clear_status()
         sim.forward(substeps, iters)
         ref.forward(substeps, iters)
@@ -48,8 +49,11 @@ def check_frames(n, frames, substeps=20, iters=2):
 
 
 @pytest.mark.timeout(900)
-def test_drape_256_ten_frames_bit_exact():
-    check_frames(256, 10)
+@pytest.mark.parametrize("walk_wave", [False, True])
+def test_drape_256_ten_frames_bit_exact(walk_wave):
+    """Both walks of the cloth tree: one per lane, and one shared walk per
+    wavefront (k_traverse_quad_wave)."""
+    check_frames(256, 10, walk_wave=walk_wave)
 
 
 @pytest.mark.timeout(900)
