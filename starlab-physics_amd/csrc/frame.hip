@@ -342,6 +342,11 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
   bool tv_fresh = false;   // set-2 candidates depend only on the static x and the dynamic tree
   SideStream* side = nullptr;
   const bool refits = coll && d->dyn.mode == 1 && d->refit_every_substep && d->dyn.nf > 0 && d->n_substeps > 1;
+  // every walk of a single dynamic tree reads its 4-wide lines (and leaves), unless
+  // a query set is dense enough next to it for the binary wavefront walk (the
+  // obstacle vertices with at least 8 per dynamic face): then the refit keeps the
+  // binary node boxes too
+  const bool lines_only = d->dyn.quad4 && !batched(d->dyn) && (long long)d->dyn.nf * 8 > (long long)nvs;
   // The buckets' counters and the walks' redo counters are zeroed once here (the
   // tree builds above reuse the workspace); from then on every apply kernel leaves
   // them at zero for the next pass, so no pass needs a zeroing launch of its own.
@@ -368,7 +373,8 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: fork failed");
       rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
-                          d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub);
+                          d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub,
+                          lines_only);
       if (rc) return rc;
       disp_pending = dyn_cached;             // measured on the main stream after the first spring pass
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");

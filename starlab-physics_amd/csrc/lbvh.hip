@@ -427,6 +427,9 @@ struct LeafSrc {
   int32_t* leaf_l;
   int32_t* leaf_r;
   float4* quad;             // triangle trees: 4-wide node array (nullable)
+  int lines_only;           // triangle trees walked only through their 4-wide lines: of the
+                            // internal nodes inside a subtree, store only the subtree root and
+                            // its children (what the top part and the top lines read)
 };
 
 // Leaf k: its box, written to the node arrays as well.  Cell leaves follow
@@ -548,7 +551,8 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
     // the subtree's internal nodes (both children arrived: flag 2) to global memory,
     // coalesced: node boxes one node per thread, quad lines 8 threads per line
     for (int k = threadIdx.x; k < cnt; k += BS)
-      if (s_flag[k] == 2) store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
+      if (s_flag[k] == 2 && (!src.lines_only || first + k == root || s_par[k] == root))
+        store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
     if constexpr (!COUNTED) {
       if (src.quad && S.even_lines()) {      // odd-depth nodes' lines are never read: flag 2 -> 1
         const int rp = S.sub_parity(si);
@@ -983,7 +987,8 @@ extern "C" int slx_tri_refit(int nf, const int32_t* faces, const float* x4, floa
 
 int slx::tri_refit_impl(int nf, const int32_t* faces, const float* x4, float padding,
                         const int32_t* sorted_faces, const int32_t* child_a, const int32_t* child_b, const int32_t* parent, const int32_t* sched,
-                   float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub) {
+                   float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub,
+                   bool lines_only) {
   SLX_REQUIRE(nf >= 0, "slx_tri_refit: nf < 0");
   if (nf == 0) return SLX_OK;
   SLX_REQUIRE(faces && x4 && sorted_faces && child_a && child_b && parent && sched && nlo4 && nhi4,
@@ -994,6 +999,7 @@ int slx::tri_refit_impl(int nf, const int32_t* faces, const float* x4, float pad
   src.pad = padding;
   src.sorted = sorted_faces;
   src.quad = (float4*)quad4;
+  src.lines_only = lines_only && quad4 ? 1 : 0;
   return refit<kLeafVertices>("tri_refit", nf, sched, src, child_a, child_b, parent, nullptr, (float4*)nlo4,
                               (float4*)nhi4, st, after_sub);
 }

@@ -161,10 +161,14 @@ int velocity_pass_impl(int nv, int nvs, const int32_t* faces_dy, const int32_t* 
                        bool bcnt_zeroed);
 // zero the buckets' counters (workspace ws) and the listed counters in one launch
 int frame_zero(int nv, int rec_cap, void* ws, const ZeroList& zl, hipStream_t st);
-// slx_tri_refit, recording after_sub (nullable) once the subtree kernel is queued
+// slx_tri_refit, recording after_sub (nullable) once the subtree kernel is queued.
+// lines_only: the tree is read only through its 4-wide lines, its leaves, its top
+// nodes and its subtree roots with their children (single trees whose walks all
+// use the lines): the other internal nodes' binary boxes are left as they were.
 int tri_refit_impl(int nf, const int32_t* faces, const float* x4, float padding, const int32_t* sorted_faces,
                    const int32_t* child_a, const int32_t* child_b, const int32_t* parent, const int32_t* sched,
-                   float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub);
+                   float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub,
+                   bool lines_only = false);
 // slx_tree_displacement; zero = false keeps disp as it is before the max (a
 // snapshot taken in the same stream order has zeroed it)
 int tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp, bool zero, hipStream_t st);
