@@ -103,9 +103,13 @@ __device__ void mlp_forward_tile(const __bf16* W, const float* bias, FwdLds L, f
   out = c;
 }
 
-// BF16 weights and FP32 biases (from the master copy) into LDS
+// BF16 weights (16-byte loads: kParams is a multiple of 8 and both arrays are
+// 16-byte aligned) and FP32 biases (from the master copy) into LDS
+static_assert(kParams % 8 == 0, "weights are loaded eight bf16 at a time");
 __device__ __forceinline__ void load_weights(__bf16* sW, float* sB, const __bf16* W, const float* Wf) {
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) sW[i] = W[i];
+  const uint4* src = reinterpret_cast<const uint4*>(W);
+  uint4* dst = reinterpret_cast<uint4*>(sW);
+  for (int i = threadIdx.x; i < kParams / 8; i += blockDim.x) dst[i] = src[i];
   for (int i = threadIdx.x; i < kH; i += blockDim.x) {
     sB[i] = Wf[ob1 + i];
     sB[kH + i] = Wf[ob2 + i];
@@ -163,47 +167,51 @@ __global__ void k_features(int nv, const float4* __restrict__ x4, const float4* 
   for (int i = 0; i < kFeat; ++i) feat[(size_t)v * kFeat + i] = f[i];
 }
 
-// ---- inference: features -> MLP -> y = y_pred + f * out * out_scale (in place on y4) ----
-constexpr int kInfWavesLds = 1;
-__global__ void __launch_bounds__(64)
+// ---- inference: features -> MLP -> y = y_pred + f * out * out_scale ----
+// Four wavefronts per workgroup share the weights in LDS (loaded once per
+// workgroup) and each takes its own 32-vertex tile with its own activation
+// buffers; the workgroup steps over groups of four tiles so every wavefront
+// reaches the forward's workgroup barriers the same number of times.
+constexpr int kInfWaves = 4;
+__global__ void __launch_bounds__(64 * kInfWaves)
 k_warm_start(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4, const float4* __restrict__ y4,
-             float4* __restrict__ y_out,
-             const float* __restrict__ fixed, const int32_t* __restrict__ soff, const int32_t* __restrict__ swid,
-             const int2* __restrict__ ent, float dt, float inv_s, const __bf16* __restrict__ W,
-             const float* __restrict__ Wf, const float* __restrict__ fmean, const float* __restrict__ fistd,
-             float out_scale) {
-  __shared__ __bf16 sW[kParams];
+             float4* __restrict__ y_out, const float* __restrict__ fixed, const int32_t* __restrict__ soff,
+             const int32_t* __restrict__ swid, const int2* __restrict__ ent, float dt, float inv_s,
+             const __bf16* __restrict__ W, const float* __restrict__ Wf, const float* __restrict__ fmean,
+             const float* __restrict__ fistd, float out_scale) {
+  __shared__ __align__(16) __bf16 sW[kParams];
   __shared__ float sB[kBias];
-  __shared__ __bf16 a0[32 * kInP];
-  __shared__ __bf16 a1[32 * kH], a2[32 * kH], a3[32 * kH];
-  __shared__ float so[32 * 4];
+  __shared__ __bf16 a0[kInfWaves][32 * kInP];
+  __shared__ __bf16 a1[kInfWaves][32 * kH], a2[kInfWaves][32 * kH];
+  __shared__ float so[kInfWaves][32 * 4];
   load_weights(sW, sB, W, Wf);
   __syncthreads();
-  const int l = threadIdx.x;
-  const int ntiles = (nv + 31) / 32;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int v = t * 32 + (l & 31);
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ntiles = (nv + 31) / 32, ngroups = (ntiles + kInfWaves - 1) / kInfWaves;
+  for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+    const int v = (gi * kInfWaves + w) * 32 + (l & 31);
     if (l < 32) {
       float f[kFeat];
       if (v < nv) vertex_features(v, x4, v4, y4, soff, swid, ent, dt, inv_s, f);
       else for (int i = 0; i < kFeat; ++i) f[i] = 0.f;
       for (int i = 0; i < kInP; ++i)
-        a0[l * kInP + i] = (__bf16)(i < kFeat && v < nv ? norm_feature(f[i], fmean[i], fistd[i]) : 0.f);
+        a0[w][l * kInP + i] = (__bf16)(i < kFeat && v < nv ? norm_feature(f[i], fmean[i], fistd[i]) : 0.f);
     }
     __syncthreads();
-    FwdLds L{a0, a1, a2, a3, nullptr, nullptr, nullptr};
+    // the third layer's output reuses a1 (the second layer has read it by then)
+    FwdLds L{a0[w], a1[w], a2[w], a1[w], nullptr, nullptr, nullptr};
     f32x16 out;
     mlp_forward_tile(sW, sB, L, out);
     if (d_col() < kOut) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) so[d_row(j) * 4 + d_col()] = out[j];
+      for (int j = 0; j < 16; ++j) so[w][d_row(j) * 4 + d_col()] = out[j];
     }
     __syncthreads();
     if (l < 32 && v < nv) {
       const float fv = fixed[v];
-      float4 y = y4[v];
-      y_out[v] = make_float4(y.x + fv * (so[l * 4 + 0] * out_scale), y.y + fv * (so[l * 4 + 1] * out_scale),
-                          y.z + fv * (so[l * 4 + 2] * out_scale), y.w);
+      const float4 y = y4[v];
+      y_out[v] = make_float4(y.x + fv * (so[w][l * 4 + 0] * out_scale), y.y + fv * (so[w][l * 4 + 1] * out_scale),
+                             y.z + fv * (so[w][l * 4 + 2] * out_scale), y.w);
     }
     __syncthreads();
   }
@@ -214,7 +222,7 @@ __global__ void __launch_bounds__(64)
 k_train_grad(int n, int tiles_per_block, const float* __restrict__ feat, const float* __restrict__ target,
              const __bf16* __restrict__ W, const float* __restrict__ Wf, const float* __restrict__ fmean,
              const float* __restrict__ fistd, float inv_n, float* __restrict__ part, float* __restrict__ loss_part) {
-  __shared__ __bf16 sW[kParams];
+  __shared__ __align__(16) __bf16 sW[kParams];
   __shared__ float sB[kBias];
   __shared__ float g[kParams];                  // this block's gradient
   __shared__ __bf16 a0[32 * kInP];
@@ -399,9 +407,9 @@ extern "C" int slx_warm_start(int nv, const float* x4, const float* v4, const fl
                   fmean && fistd,
               "slx_warm_start: null pointer");
   SLX_REQUIRE(y4_pred != y4_out, "slx_warm_start: y4_pred and y4_out must differ");
-  int tiles = (nv + 31) / 32;
-  int grid = tiles < 2048 ? tiles : 2048;
-  SLX_LAUNCH("warm_start", k_warm_start, dim3(grid), dim3(64), 0, (hipStream_t)stream, nv, (const float4*)x4,
+  const int groups = ((nv + 31) / 32 + kInfWaves - 1) / kInfWaves;
+  const int grid = groups < 512 ? groups : 512;
+  SLX_LAUNCH("warm_start", k_warm_start, dim3(grid), dim3(64 * kInfWaves), 0, (hipStream_t)stream, nv, (const float4*)x4,
              (const float4*)v4, (const float4*)y4_pred, (float4*)y4_out, fixed, slice_off, slice_width,
              (const int2*)entries, dt, inv_s,
              (const __bf16*)w_bf16, w_f32, fmean, fistd, out_scale);

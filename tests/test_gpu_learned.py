@@ -173,7 +173,10 @@ def test_forward_matches_reference_through_warm_start(slx):
 
 def test_training_and_warm_start_gain(slx):
     """Train on plain solves (residual 1e-6) of seeded variants; on a held-out
-    seed the warm start reaches the residual 1e-5 in fewer Jacobi iterations."""
+    seed the warm start reaches the residual 1e-5 in fewer Jacobi iterations, at a
+    wall-time cost bounded by its inference launch (32 x 32 cloth: the passes it
+    saves cost less than that launch; the benchmark's time-to-residual line records
+    the 256 x 256 cloth)."""
     import torch
     learned = L()
     rng = np.random.default_rng(0)
@@ -188,21 +191,31 @@ def test_training_and_warm_start_gain(slx):
     assert torch.isfinite(feat).all() and torch.isfinite(tgt).all()
     md = learned.WarmStart(seed=0)
     md.set_normalisation(feat, tgt)
+    zero = float(((tgt / md.out_scale) ** 2).sum(dim=1).mean().item())    # the loss of predicting 0
     losses = md.train(feat, tgt, epochs=30, batch=4096, lr=2e-3)
-    print("training loss first / last epoch:", losses[0], losses[-1])
-    assert np.isfinite(losses).all() and losses[-1] < 0.3 * losses[0]
+    print("training loss first / last epoch, zero-prediction loss:", losses[0], losses[-1], zero)
+    assert np.isfinite(losses).all() and losses[-1] < 0.3 * losses[0] and losses[-1] < 0.5 * zero
+    import time
     res = {}
     for learned_on in (False, True):
         mesh, sim = cloth(slx, 32, seed=7)
         rs = learned.ResidualSolver(sim, md)
+        rs.substep(10, tol=1e-5, max_iters=2000, learned=learned_on)     # first launches of each kernel
+        torch.cuda.synchronize()
         it, falls = 0, 0
+        t0 = time.perf_counter()
         for _ in range(30):
             k, fell = rs.substep(10, tol=1e-5, max_iters=2000, learned=learned_on)
             it += k
             falls += fell
-        res[learned_on] = (it, falls)
-    print("iterations plain / learned (fallbacks):", res[False], res[True])
+        torch.cuda.synchronize()
+        res[learned_on] = (it, falls, time.perf_counter() - t0)
+    print("iterations, fallbacks, wall s: plain", res[False], "learned", res[True])
     assert res[True][0] < 0.5 * res[False][0]
+    # wall time: on this small cloth a pass costs less than the network's one
+    # launch, so the warm start saves passes but not time; the bound keeps its
+    # cost to that launch (measured: plain 2.7 ms, learned 3.3 ms per 30 substeps)
+    assert res[True][2] < 1.5 * res[False][2]
 
 
 def test_fallback_when_the_network_is_wrong(slx):
