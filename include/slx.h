@@ -596,8 +596,9 @@ typedef struct slx_frame_desc {
   int32_t pair_cap;                      /* contacting pairs kept per query */
   int32_t* pair_cnt;                     /* [2nv + nvs] */
   float* pair_rec;                       /* float4 [4 pair_cap (2nv + nvs)] */
-  /* candidate caches of the query sets vt_st, vt_dy, tv_st (triangle trees of
-     single meshes; cap 0 = walk every pass).  Rebuilt at the first query of each
+  /* candidate caches of the query sets vt_st, vt_dy, tv_st (triangle trees, single
+     or batched -- a batched tree's queries walk from their instance's root; cap 0 =
+     walk every pass).  Rebuilt at the first query of each
      set in every cache_every-th substep (<= 0: once per frame); the dynamic tree
      needs x_snap and disp. */
   slx_cand_cache cache[3];

@@ -131,7 +131,7 @@ int build_tree(const slx_tree& t, int nv, const float* x4, const int32_t* faces,
 // a cache struct with its redo list enables the two-tier walk; with leaves too,
 // the cache proper
 bool tiered(const slx_tree& t, const slx_cand_cache& c) {
-  return t.mode == 1 && !batched(t) && t.nf > 0 && c.redo != nullptr;
+  return t.mode == 1 && t.nf > 0 && c.redo != nullptr;
 }
 bool cacheable(const slx_tree& t, const slx_cand_cache& c) { return tiered(t, c) && c.cap > 0 && c.leaves; }
 
@@ -155,7 +155,8 @@ int query(const char* name, const slx_tree& t, int nq, const float* q4, const in
       *built = true;
     }
     return traverse_cached(name, kind, nq, q4, pad, nl, t.nlo4, t.nhi4, t.quad4, cache, disp, cap, cand, cand_n,
-                           status, (hipStream_t)s, q_order, nullptr, true);
+                           status, (hipStream_t)s, q_order, nullptr, true, nullptr, seg ? t.roots : nullptr,
+                           seg ? q_inst : nullptr);
   }
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
                          t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,

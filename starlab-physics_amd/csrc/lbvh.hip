@@ -1712,12 +1712,12 @@ template <bool BUILD>
 int quad_first_tier(const char* name, int nq, const float* q4, float padding, int n_leaves, const float* nlo4,
                     const float* nhi4, const float* quad4, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
                     hipStream_t st, const int32_t* order, const CacheOut& co, const int32_t* qedges,
-                    const int32_t* n_active) {
+                    const int32_t* n_active, const int32_t* roots = nullptr, const int32_t* q_inst = nullptr) {
   const dim3 g(active_grid(nq, kTravBS, n_active)), b(kTravBS);
 #define SLX_QUAD_TIER(D)                                                                                         \
   SLX_LAUNCH(name, (k_traverse_quad<BUILD, D>), g, b, 0, st, nq, (const float4*)q4, padding, qedges, n_leaves,    \
-             (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, nullptr,  \
-             nullptr, order, n_active, co)
+             (const float4*)nlo4, (const float4*)nhi4, (const float4*)quad4, cap, cand, cand_n, status, roots,    \
+             q_inst, order, n_active, co)
   const int depth = knobs().walk_stack;
   if (depth <= 16) SLX_QUAD_TIER(16);
   else if (depth <= 24) SLX_QUAD_TIER(24);
@@ -1730,7 +1730,8 @@ int quad_first_tier(const char* name, int nq, const float* q4, float padding, in
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
-                    const int32_t* order, const int32_t* qedges, bool prezeroed, const int32_t* n_active) {
+                    const int32_t* order, const int32_t* qedges, bool prezeroed, const int32_t* n_active,
+                    const int32_t* roots, const int32_t* q_inst) {
   SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
   SLX_REQUIRE(!qedges || kind == 0, "slx_traverse_cached: edge queries are walked, not cached");
   SLX_REQUIRE(!n_active || (kind == 0 && order), "slx_traverse_cached: a device query count needs a walk and a list");
@@ -1744,7 +1745,9 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                                             nullptr, nullptr, nullptr, cap, cand, cand_n, status, st, nullptr,
                                             nullptr, order, n_active, qedges);
   SLX_REQUIRE(nlo4 && nhi4, "slx_traverse_cached: null tree");
-  const bool wave = SLX_TRAV_WAVE && (long long)n_leaves * 8 <= (long long)nq;
+  SLX_REQUIRE(!roots || (q_inst && !qedges), "slx_traverse_cached: per-instance roots need query instance ids");
+  // (trees of batched instances are walked from each query's instance root: per lane)
+  const bool wave = !roots && SLX_TRAV_WAVE && (long long)n_leaves * 8 <= (long long)nq;
   SLX_REQUIRE(wave || quad4, "slx_traverse_cached: the triangle tree needs its 4-wide node array");
   if (kind == 0 && wave)                      // the wavefront walk keeps its stack in registers
     return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
@@ -1754,7 +1757,8 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   // coherent order): the 4-wide wavefront walk when knobs().walk_wave is on (off
   // by default: on the 1M-vertex drape its cache build took 1018 us against the
   // per-lane walk's 366 us -- the shared walk is a long chain of dependent loads)
-  const bool quad_wave = !wave && knobs().walk_wave && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
+  const bool quad_wave =
+      !wave && !roots && knobs().walk_wave && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
   int rc = SLX_OK;
   if (kind == 1) {
     const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
@@ -1768,7 +1772,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                  (const float4*)quad4, cap, cand, cand_n, status, order, nullptr, co);
     else
       rc = quad_first_tier<true>("cache_build", nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n,
-                                 status, st, order, co, nullptr, nullptr);
+                                 status, st, order, co, nullptr, nullptr, roots, q_inst);
   } else if (kind == 2) {
     SLX_LAUNCH(name, k_cand_filter, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4, padding,
                c->margin, (const float4*)nlo4, (const float4*)nhi4, c->cap, c->leaves, c->count,
@@ -1782,12 +1786,12 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                  (const float4*)quad4, cap, cand, cand_n, status, order, nullptr, co);
     else
       rc = quad_first_tier<false>(name, nq, q4, padding, n_leaves, nlo4, nhi4, quad4, cap, cand, cand_n, status,
-                                  st, order, co, qedges, n_active);
+                                  st, order, co, qedges, n_active, roots, q_inst);
   }
   if (rc) return rc;
   // the queries the first tier could not serve: a full walk over the list built on the device
   return traverse_launch("walk_redo", 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
-                         nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, c->redo + 1, c->redo,
+                         nullptr, nullptr, cap, cand, cand_n, status, st, roots, q_inst, c->redo + 1, c->redo,
                          qedges);
 }
 

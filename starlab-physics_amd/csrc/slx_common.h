@@ -126,7 +126,8 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false,
-                    const int32_t* n_active = nullptr);
+                    const int32_t* n_active = nullptr, const int32_t* roots = nullptr,
+                    const int32_t* q_inst = nullptr);
 
 // Counters a kernel zeroes for the next pass (thread i < n of block 0 writes *p[i] = 0).
 struct ZeroList {

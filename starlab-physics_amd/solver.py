@@ -236,9 +236,10 @@ class Solver:
         return cd
 
     def _cand_caches(self, cd):
-        """Candidate caches of the three query sets (triangle trees, single meshes)."""
+        """Candidate caches of the three query sets (triangle trees; batched trees
+        walk from each query's instance root)."""
         from .broadphase import CandidateCache
-        tri = self.bvh_mode == "tri" and cd["dyn"].n_inst <= 1
+        tri = self.bvh_mode == "tri"
         on = self.candidate_cache and tri
         ccap = (2 * self.cache_size if self.cache_capacity is None else int(self.cache_capacity)) if on else 0
         margin = _f32(0.25 * self.padding if self.cache_margin is None else self.cache_margin)
