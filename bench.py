@@ -398,7 +398,9 @@ def bench_batched(a):
                    "instances": a.instances, "vertices": V},
         "substeps_per_s": a.substeps * a.steps / el,
         "instance_substeps_per_s": a.instances * a.substeps * a.steps / el,
-        "autotune": {"best": list(best), "frame_ms": {f"{k[0]}x{k[1]}": round(v * 1e3, 3) for k, v in tune.items()}},
+        "autotune": {"best": list(best), "frame_ms": {f"{k[0]}x{k[1]}": round(v * 1e3, 3) for k, v in tune.items()},
+                     "knobs": {k: {"best": v["best"], "frame_ms": {x: round(y * 1e3, 3) for x, y in v["seconds"].items()}}
+                               for k, v in bs.tuning.items() if k != "spring_launch"}},
         "status": bs.sim.status(), "setup_s": setup,
     }
     print(json.dumps(out), flush=True)

@@ -97,27 +97,49 @@ class BatchedScene:
         self.graph.replay()
 
     # --------------------------------------------------------------- autotune
-    def autotune(self, n_substeps, n_iters, blocks=(64, 128, 256), vpts=(1, 2, 4), reps=3):
-        """Time `reps` graph replays of one frame for every launch shape of the
-        spring kernels; keep the fastest.  Each shape is captured afresh (the
-        launch shape is baked into the graph)."""
+    def _time_frame(self, state, n_substeps, n_iters, reps):
         import torch
+        self.cloth.x4.copy_(state[0])
+        self.cloth.v4.copy_(state[1])
+        self.capture(n_substeps, n_iters)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def autotune(self, n_substeps, n_iters, blocks=(64, 128, 256), vpts=(1, 2, 4), reps=3, knobs=True):
+        """Time `reps` graph replays of one frame for every launch shape of the
+        spring kernels (threads per block x vertices per thread) and keep the
+        fastest; then, one knob at a time with the others at their best, the
+        first-tier walk's stack depth, the candidate cache's margin (a multiple
+        of the padding) and the refit's subtree size.  Each setting is captured
+        afresh (launch shapes are baked into the graph).  No knob changes the
+        results.  Returns (best spring shape, {shape: seconds per frame}); the
+        other knobs' timings are in self.tuning."""
         results = {}
         state = [t.clone() for t in (self.cloth.x4, self.cloth.v4)]
         for bs in blocks:
             for vpt in vpts:
                 self.sim.spring_launch = (bs, vpt)
-                self.cloth.x4.copy_(state[0])
-                self.cloth.v4.copy_(state[1])
-                self.capture(n_substeps, n_iters)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    self.replay()
-                torch.cuda.synchronize()
-                results[(bs, vpt)] = (time.perf_counter() - t0) / reps
+                results[(bs, vpt)] = self._time_frame(state, n_substeps, n_iters, reps)
         best = min(results, key=results.get)
         self.sim.spring_launch = best
+        self.tuning = {"spring_launch": {f"{k[0]}x{k[1]}": v for k, v in results.items()}}
+        if knobs:
+            pad = self.sim.padding
+            for name, values, apply in (
+                    ("walk_stack", (16, 24, 32, 64), lambda v: setattr(self.sim, "walk_stack", v)),
+                    ("cache_margin", (0.125, 0.25, 0.5), lambda v: setattr(self.sim, "cache_margin", v * pad)),
+                    ("refit_sub", (1024, 2048), lambda v: setattr(self.sim, "refit_shape", (v, 2048)))):
+                t = {}
+                for v in values:
+                    apply(v)
+                    t[v] = self._time_frame(state, n_substeps, n_iters, reps)
+                b = min(t, key=t.get)
+                apply(b)
+                self.tuning[name] = {"best": b, "seconds": {str(k): x for k, x in t.items()}}
         self.cloth.x4.copy_(state[0])
         self.cloth.v4.copy_(state[1])
         self.capture(n_substeps, n_iters)

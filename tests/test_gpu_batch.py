@@ -123,7 +123,18 @@ def test_two_solvers_keep_their_own_launch_shapes(slx):
 
 
 def test_autotune_picks_a_valid_shape(slx):
+    """The tuner keeps the fastest spring shape and a best value of each other
+    knob; the tuned scene's frames equal an untuned scene's."""
     batch, cloths, spheres, bs = small_batch(slx, seed=9)
     best, res = bs.autotune(5, 1, reps=2)
     assert best in res and len(res) == 9
     assert bs.sim.spring_launch == best
+    assert bs.tuning["walk_stack"]["best"] in (16, 24, 32, 64) and bs.sim.walk_stack == bs.tuning["walk_stack"]["best"]
+    assert bs.tuning["cache_margin"]["best"] in (0.125, 0.25, 0.5)
+    assert bs.tuning["refit_sub"]["best"] in (1024, 2048)
+    _, _, _, ref = small_batch(slx, seed=9)
+    ref.forward(5, 1)                 # the final capture ran one eager frame from the initial state
+    for _ in range(3):
+        bs.replay()
+        ref.forward(5, 1)
+    np.testing.assert_array_equal(bs.cloth.positions(), ref.cloth.positions())
