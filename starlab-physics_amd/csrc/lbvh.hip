@@ -782,6 +782,88 @@ __device__ __forceinline__ void query_box(const float4* __restrict__ q4, const i
   }
 }
 
+// The cell tree's walk split over 16 lanes per query.  The 32^3 cell codes are all
+// distinct and differ only in their 15 high interleaved bits, so the Karras tree
+// over them is complete: its 16 nodes at depth 4 root subtrees of 2048 cells each.
+// Lane s walks the subtree reached from the root by the bits of s (1 = child_b),
+// with the sequential walk's rules (skip empty nodes, test boxes, faces of a leaf
+// in sorted order); a non-empty node's box lies inside each ancestor's box (the
+// union of its non-empty children), so a subtree the sequential walk would reach
+// is exactly one whose root overlaps and is not empty.  The sequential walk
+// visits child_b's subtree first, so the subtrees come in decreasing s: lane s's
+// candidates start after the counts of lanes s+1..15.  Each lane keeps its first
+// cap candidates in LDS until the 16 counts are known.
+constexpr int kSplit = 16, kSplitQ = 256 / kSplit, kSplitCap = 64, kSplitStack = 16;
+__global__ void __launch_bounds__(256)
+k_traverse_cell_split(int nq, const float4* __restrict__ q4, float pad, const int32_t* __restrict__ qe,
+                      const int32_t* __restrict__ num, const int32_t* __restrict__ sorted,
+                      const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                      const float4* __restrict__ fmin, const float4* __restrict__ fmax,
+                      const float* __restrict__ box, int cap, int32_t* __restrict__ cand,
+                      int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order) {
+  __shared__ int32_t s_buf[kSplitCap * 256];      // entry i of lane l at s_buf[i * 256 + l]
+  __shared__ int32_t s_stack[kSplitStack * 256];
+  const int tid = threadIdx.x, sub = tid & (kSplit - 1);
+  const int t = blockIdx.x * kSplitQ + (tid >> 4);
+  const bool active = t < nq;                     // (16 lanes of one query share it)
+  const int q = active ? (order ? order[t] : t) : 0;
+  float3 a = make_float3(0.f, 0.f, 0.f), b = a;
+  bool live = active;
+  if (active) {
+    query_box(q4, qe, q, pad, a, b);
+    const float3 c = make_float3(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+    if (flat_cell_id(c, cell_size_of(box), make_float3(box[0], box[1], box[2])) > kCells - 1) live = false;
+  }
+  int cnt = 0, flags = 0;
+  if (live) {
+    int idx = 0;
+#pragma unroll
+    for (int d = 3; d >= 0; --d) idx = ((sub >> d) & 1) ? __float_as_int(nhi[idx].w) : __float_as_int(nlo[idx].w);
+    int32_t* stk = s_stack + tid;
+    int sp = 0;
+    if (num[idx] > 0) stk[(sp++) * 256] = idx;
+    while (sp > 0) {
+      const int n = stk[(--sp) * 256];
+      const float4 lo = nlo[n], hi = nhi[n];
+      if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
+      if (n >= kCells - 1) {
+        const int l = __float_as_int(lo.w), r = __float_as_int(hi.w);
+        for (int j = l; j <= r; ++j) {
+          const int f = sorted[j];
+          if (overlap(a, b, xyz(fmin[f]), xyz(fmax[f]))) {
+            if (cnt < cap) s_buf[cnt * 256 + tid] = f;
+            ++cnt;
+          }
+        }
+      } else {
+        const int ca = __float_as_int(lo.w), cb = __float_as_int(hi.w);
+        if (num[ca] > 0) {
+          if (sp < kSplitStack) stk[(sp++) * 256] = ca; else flags |= kStatusStackOverflow;
+        }
+        if (num[cb] > 0) {
+          if (sp < kSplitStack) stk[(sp++) * 256] = cb; else flags |= kStatusStackOverflow;
+        }
+      }
+    }
+  }
+  // counts of lanes sub..15 (inclusive), within the query's 16 lanes
+  int sfx = cnt;
+#pragma unroll
+  for (int o = 1; o < kSplit; o <<= 1) {
+    const int y = __shfl_down(sfx, o, kSplit);
+    if (sub + o < kSplit) sfx += y;
+  }
+  const int total = __shfl(sfx, 0, kSplit), off = sfx - cnt;
+  if (!active) return;
+  for (int i = 0; i < cnt && i < cap; ++i) {
+    const int pos = off + i;
+    if (pos < cap) cand[(size_t)pos * nq + q] = s_buf[i * 256 + tid];
+  }
+  if (sub == 0) cand_n[q] = live ? (total < cap ? total : cap) : 0;
+  if (total > cap) flags |= kStatusCandOverflow;
+  if (flags && status) atomicOr(status, flags);
+}
+
 // traverse_cell_bvh_single, lbvh_cell.py:661-714, for a batch of query points
 // p +- pad (or edge boxes, query_box).  CELL: skip queries whose centre cell id exceeds 32767, skip empty
 // nodes, leaves list faces [l, r] of sorted; else triangle leaves (one face).
@@ -1562,7 +1644,11 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   SLX_REQUIRE(n_leaves <= 0 || (nlo4 && nhi4), "slx_traverse: null tree");
   SLX_REQUIRE(mode == 1 || (num && sorted_faces && fmin4 && fmax4 && box6), "slx_traverse: null cell tree");
   SLX_REQUIRE(mode == 0 || n_leaves < (1 << 29), "slx_traverse: triangle tree too large for 30-bit stack words");
-  if (mode == 0)
+  if (mode == 0 && !roots && cap <= kSplitCap && n_leaves == kCells)
+    SLX_LAUNCH(name, k_traverse_cell_split, dim3(blocks_for(nq, kSplitQ)), dim3(256), 0, st, nq, (const float4*)q4,
+               padding, qedges, num, sorted_faces, (const float4*)nlo4, (const float4*)nhi4, (const float4*)fmin4,
+               (const float4*)fmax4, box6, cap, cand, cand_n, status, order);
+  else if (mode == 0)
     SLX_LAUNCH(name, k_traverse<true>, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq,
                (const float4*)q4, padding, qedges, n_leaves, num, sorted_faces, (const float4*)nlo4,
                (const float4*)nhi4, (const float4*)fmin4, (const float4*)fmax4, box6, cap, cand, cand_n, status, roots,
