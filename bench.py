@@ -266,7 +266,7 @@ def main():
     if collision:
         for k, (c, n) in sim.candidates().items():
             cbar[k] = float(n.float().mean().item())
-        counts.update(Vs=terr.num_verts, Fs=terr.num_faces, cbar=cbar)
+        counts.update(Vs=terr.num_verts, Fs=terr.num_faces, cbar=cbar, cbar_stored=dict(cbar))
     if a.velocity and collision:
         counts.update(velocity_counts(sim, V))
     ee_info = {}
@@ -285,6 +285,16 @@ def main():
                 cnt = c.count[:c.nq].clamp(max=c.cap).float()
                 cs[k] = float(cnt.mean().item())
         counts["cache"] = cs
+    if collision and sim.exclude_own_faces:
+        # the cloth's stored lists leave out each vertex's own faces; the
+        # algorithmic model counts the reference's full lists: one more frame
+        # (after the timed region) with full lists measures their length
+        sim.exclude_own_faces = False
+        sim.forward(a.substeps, a.iters)
+        torch.cuda.synchronize()
+        sim.exclude_own_faces = True
+        for k, (c, n) in sim.candidates().items():
+            cbar[k] = float(n.float().mean().item())
     value = V * a.substeps * a.steps / el
     tot_ms, cnt = dom_timed
     avg_s = tot_ms / cnt / 1e3
@@ -339,6 +349,7 @@ def main():
         "step_algorithmic_bytes": rl.step_bytes(counts, a.iters, a.substeps, collision),
         "status": status,
         "mean_candidates": cbar,
+        "mean_stored_candidates": counts.get("cbar_stored", {}),
         "mean_cached_leaves": counts.get("cache", {}),
         "velocity_pairs": {k: counts[k] for k in ("pairs", "vel_records") if k in counts},
         "edge_edge": ee_info,

@@ -637,6 +637,16 @@ typedef struct slx_frame_desc {
   int32_t refit_top;                     /* 1..2048 */
   int32_t walk_wave;                     /* cloth-vertex walks of the cloth tree: 1 = one shared walk per
                                             wavefront over the 4-wide nodes, -1 = one walk per lane */
+  /* own_faces (nullable): int32 [nv][8], the faces incident to each cloth vertex
+     (-1 padded; a vertex on more than eight faces has a row of -1).  The narrow
+     phase skips a vertex's own faces (XPBD.py:258); given this table, the walks
+     of the cloth tree by the cloth's vertices count them in a list's length (so
+     a list overflows exactly when the reference's does, and keeps the entries it
+     keeps) but store only the other candidates, in the same order.  Positions and
+     velocities are the same with or without it; cand_n of that set then counts
+     the stored entries.  (No reference counterpart: the reference's lists hold
+     the incident faces too.) */
+  const int32_t* own_faces;
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

@@ -140,11 +140,12 @@ bool cacheable(const slx_tree& t, const slx_cand_cache& c) { return tiered(t, c)
 // cache (nullable): the set's cache struct; use_cache: filled when *built is
 // false, then filtered; otherwise only its redo list serves the two-tier walk.
 // disp: the tree's displacement since the cache was built (nullable = 0)
+// own (nullable): the queries' own faces, left out of the stored lists
 // (the frame keeps every redo counter at zero between passes: prezeroed walks)
 int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst,
           const int32_t* q_order, float pad, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
           slx_stream_t s, const slx_cand_cache* cache = nullptr, bool use_cache = false, bool* built = nullptr,
-          const uint32_t* disp = nullptr) {
+          const uint32_t* disp = nullptr, const int32_t* own = nullptr) {
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
   const bool seg = batched(t);
@@ -156,11 +157,12 @@ int query(const char* name, const slx_tree& t, int nq, const float* q4, const in
     }
     return traverse_cached(name, kind, nq, q4, pad, nl, t.nlo4, t.nhi4, t.quad4, cache, disp, cap, cand, cand_n,
                            status, (hipStream_t)s, q_order, nullptr, true, nullptr, seg ? t.roots : nullptr,
-                           seg ? q_inst : nullptr);
+                           seg ? q_inst : nullptr, own);
   }
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
                          t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,
-                         (hipStream_t)s, seg ? t.roots : nullptr, seg ? q_inst : nullptr, q_order);
+                         (hipStream_t)s, seg ? t.roots : nullptr, seg ? q_inst : nullptr, q_order, nullptr, nullptr,
+                         t.mode == 1 ? own : nullptr);
 }
 
 }  // namespace
@@ -475,7 +477,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       rc = join_refit();                     // the dynamic tree is needed from here on
       if (rc) return rc;
       rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
-                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp);
+                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp, d->own_faces);
       if (rc) return rc;
       if (tv_side && hipStreamWaitEvent(st, side->tvjoin, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: join failed");

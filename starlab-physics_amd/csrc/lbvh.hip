@@ -804,7 +804,7 @@ k_traverse_cell_split(int nq, const float4* __restrict__ q4, float pad, const in
   __shared__ int32_t s_buf[kSplitCap * 256];      // entry i of lane l at s_buf[i * 256 + l]
   __shared__ int32_t s_stack[kSplitStack * 256];
   const int tid = threadIdx.x, sub = tid & (kSplit - 1);
-  const int t = blockIdx.x * kSplitQ + (tid >> 4);
+  const int t = xcd_block() * kSplitQ + (tid >> 4);
   const bool active = t < nq;                     // (16 lanes of one query share it)
   const int q = active ? (order ? order[t] : t) : 0;
   float3 a = make_float3(0.f, 0.f, 0.f), b = a;
@@ -878,7 +878,7 @@ k_traverse(int nq, const float4* __restrict__ q4, float pad, const int32_t* __re
            int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* status,
            const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst, const int32_t* __restrict__ order) {
   __shared__ int32_t s_stack[kStack * kTravBS];
-  const int t = blockIdx.x * kTravBS + threadIdx.x;
+  const int t = xcd_block() * kTravBS + threadIdx.x;
   if (t >= nq) return;
   const int q = order ? order[t] : t;
   int32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
@@ -1139,7 +1139,7 @@ k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, const int32_t* 
                int32_t* status, const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst,
                const int32_t* __restrict__ order, const int32_t* __restrict__ n_active) {
   __shared__ uint32_t s_stack[kStack * kTravBS];
-  const int t = blockIdx.x * kTravBS + threadIdx.x;
+  const int t = xcd_block() * kTravBS + threadIdx.x;
   if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;     // spatially coherent queries share nodes within a wavefront
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
@@ -1210,8 +1210,35 @@ struct CacheOut {
   int cap;                 // cache capacity per query
   int32_t* leaves;         // [cap][nq] by slot
   int32_t* count;          // [nq] by slot
-  float4* q0;              // [nq] by slot
+  float4* q0;              // [nq] by slot (w: the query's own-face count, see OwnFaces)
   int32_t* redo;           // [1 + nq]
+  const int4* own;         // [nq][2] by query (nullable): faces each query leaves out of its lists
+};
+
+// Own faces of a cloth vertex queried against its own mesh's tree: up to eight
+// face ids (-1 padded), the faces incident to the vertex, which the narrow phase
+// skips (the is_in_face test, XPBD.py:258).  A walk that has them counts such a
+// face in the list's length (so a list overflows exactly when the full list
+// does, and the faces it keeps are those the full list keeps) but stores only
+// the others, in the same order: the contacts, their order and so every sum are
+// unchanged, while the cloth's lists lose most of their entries (six of the
+// eight a draped vertex has).  cand_n then counts the stored entries.  The cache
+// build leaves them out of the stored leaves too and writes their number into
+// q0[t].w, which bounds how many of them a later filter could have found.
+struct OwnFaces {
+  int4 a = make_int4(-1, -1, -1, -1), b = make_int4(-1, -1, -1, -1);
+  __device__ __forceinline__ void load(const int4* own, int q) {
+    if (own) {
+      a = own[2 * (size_t)q];
+      b = own[2 * (size_t)q + 1];
+    }
+  }
+  __device__ __forceinline__ bool has(int f) const {
+    return ((f == a.x || f == a.y) || (f == a.z || f == a.w)) || ((f == b.x || f == b.y) || (f == b.z || f == b.w));
+  }
+  __device__ __forceinline__ int count() const {
+    return ((a.x >= 0) + (a.y >= 0) + (a.z >= 0) + (a.w >= 0)) + ((b.x >= 0) + (b.y >= 0) + (b.z >= 0) + (b.w >= 0));
+  }
 };
 
 // Stored leaves go out four at a time: a lane collects them in an int4 and writes
@@ -1248,7 +1275,7 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
   // a grid-stride loop (launches sized by n_active get a capped grid); the step is
   // per workgroup, so every lane of a wavefront takes the same iterations
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int base = blockIdx.x * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
+  for (int base = xcd_block() * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
   const int lane = threadIdx.x & 63;
   const bool active = t < limit;
   const int q = active ? (order ? order[t] : t) : 0;
@@ -1375,12 +1402,14 @@ k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int
                      int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
                      const int32_t* __restrict__ n_active, CacheOut co) {
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int base = blockIdx.x * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
+  for (int base = xcd_block() * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
   const int lane = threadIdx.x & 63;
   const bool active = t < limit;
   const int q = active ? (order ? order[t] : t) : 0;
   float3 a, b, aw, bw;
+  OwnFaces own;
   if (active) {
+    own.load(co.own, q);
     query_box(q4, qe, q, pad, a, b);
     aw = a;
     bw = b;
@@ -1389,18 +1418,18 @@ k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int
       const float pw = co.pad_walk;
       aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
       bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
-      co.q0[t] = make_float4(p.x, p.y, p.z, 0.f);
+      co.q0[t] = make_float4(p.x, p.y, p.z, __int_as_float(own.count()));
     }
   } else {                                   // an empty box: overlaps nothing
     const float inf = __int_as_float(0x7f800000);
     a = aw = make_float3(inf, inf, inf);
     b = bw = make_float3(-inf, -inf, -inf);
   }
-  int cnt = 0, ccnt = 0, flags = 0;
-  auto emit = [&](int f) {
-    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
-    else flags |= kStatusCandOverflow;
-    ++cnt;
+  int all = 0, kept = 0, ccnt = 0, flags = 0;
+  auto emit = [&](int f) {                   // own faces counted, not stored (OwnFaces)
+    if (all >= cap) flags |= kStatusCandOverflow;
+    else if (!own.has(f)) cand[(size_t)(kept++) * nq + q] = f;
+    ++all;
   };
   // a leaf (node id, face id, box) whose box this lane's walk box overlaps: to the
   // cache (BUILD) and, if it overlaps the exact box, to the list.  (A line slot
@@ -1409,8 +1438,10 @@ k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int
   LeafPack pack;
   auto leaf = [&](int node, int face, const float4& lo, const float4& hi) {
     if (BUILD) {
-      pack.put(ccnt, node, co.cap, nq, t, co.leaves);
-      ++ccnt;
+      if (!own.has(face)) {
+        pack.put(ccnt, node, co.cap, nq, t, co.leaves);
+        ++ccnt;
+      }
       if (overlap(a, b, xyz(lo), xyz(hi))) emit(face);
     } else {
       emit(face);
@@ -1485,7 +1516,7 @@ k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int
     pack.finish(ccnt, co.cap, nq, t, co.leaves);
     co.count[t] = ccnt;
   }
-  cand_n[q] = cnt < cap ? cnt : cap;
+  cand_n[q] = kept;
   if (flags && status) atomicOr(status, flags);
   }(base + (int)threadIdx.x);
 }
@@ -1524,30 +1555,33 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
   __shared__ uint32_t s_stack[S * kTravBS];
   // a grid-stride loop (launches sized by n_active get a capped grid)
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int t0 = blockIdx.x * kTravBS + threadIdx.x; t0 < limit; t0 += gridDim.x * kTravBS) [&](const int t) {
+  for (int t0 = xcd_block() * kTravBS + threadIdx.x; t0 < limit; t0 += gridDim.x * kTravBS) [&](const int t) {
   const int q = order ? order[t] : t;
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   float3 a, b;
   query_box(q4, qe, q, pad, a, b);
   float3 aw = a, bw = b;                          // the walk's box
+  OwnFaces own;
+  own.load(co.own, q);
   if (BUILD) {                                    // point queries only
     const float3 p = xyz(q4[q]);
     const float pw = co.pad_walk;
     aw = make_float3(p.x - pw, p.y - pw, p.z - pw);
     bw = make_float3(p.x + pw, p.y + pw, p.z + pw);
-    co.q0[t] = make_float4(p.x, p.y, p.z, 0.f);
+    co.q0[t] = make_float4(p.x, p.y, p.z, __int_as_float(own.count()));
   }
-  int cnt = 0, ccnt = 0, flags = 0;
+  int all = 0, kept = 0, ccnt = 0, flags = 0;
   const int root = roots ? roots[q_inst[q]] : 0;
   if (nl <= 0 || root < 0) {
     cand_n[q] = 0;
     if (BUILD) co.count[t] = 0;
     return;
   }
+  // the full list's first cap entries, own faces counted but not stored
   auto emit = [&](int f) {
-    if (cnt < cap) cand[(size_t)cnt * nq + q] = f;
-    else flags |= kStatusCandOverflow;
-    ++cnt;
+    if (all >= cap) flags |= kStatusCandOverflow;
+    else if (!own.has(f)) cand[(size_t)(kept++) * nq + q] = f;
+    ++all;
   };
   LeafPack pack;
   auto keep = [&](int node) {
@@ -1559,7 +1593,7 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
     const float4 lo = nlo[root], hi = nhi[root];
     if (overlap(aw, bw, xyz(lo), xyz(hi))) {
       if (root >= nl - 1) {
-        if (BUILD) keep(root);
+        if (BUILD && !own.has(__float_as_int(lo.w))) keep(root);
         if (!BUILD || overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
       } else {
         stk[(sp++) * kTravBS] = (uint32_t)root;
@@ -1588,9 +1622,16 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
       if (!overlap(aw, bw, xyz(lo[k]), xyz(hi[k]))) continue;
       const int face = __float_as_int(hi[k].w), node = __float_as_int(lo[k].w);
       if (BUILD && face >= 0) {                   // node word under the face word: popped face first
+        const bool exact = overlap(a, b, xyz(lo[k]), xyz(hi[k]));
+        if (own.has(face)) {                      // not stored: only counted, when in the exact list
+          if (!exact) continue;
+          if (sp < S) stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | kExactBit;
+          else flags |= kStatusStackOverflow;
+          continue;
+        }
         if (sp + 1 < S) {
           stk[(sp++) * kTravBS] = (uint32_t)node | kCacheNode;
-          stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | (overlap(a, b, xyz(lo[k]), xyz(hi[k])) ? kExactBit : 0u);
+          stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | (exact ? kExactBit : 0u);
         } else {
           flags |= kStatusStackOverflow;
         }
@@ -1610,7 +1651,7 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
     co.redo[1 + atomicAdd(&co.redo[0], 1)] = q;
     return;
   }
-  cand_n[q] = cnt < cap ? cnt : cap;
+  cand_n[q] = kept;
   if (flags && status) atomicOr(status, flags);
   }(t0);
 }
@@ -1633,8 +1674,9 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots,
                     const int32_t* q_inst, const int32_t* order, const int32_t* n_active,
-                    const int32_t* qedges) {
+                    const int32_t* qedges, const int32_t* own) {
   SLX_REQUIRE(!roots || q_inst, "slx_traverse: per-instance roots need query instance ids");
+  SLX_REQUIRE(!own || !qedges, "slx_traverse: own faces belong to vertex queries");
   SLX_REQUIRE(!qedges || !roots, "slx_traverse: edge queries of batched trees are not supported");
   SLX_REQUIRE(!n_active || (order && mode == 1), "slx_traverse: a device query count needs a query list");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
@@ -1658,10 +1700,13 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   // leaves and the shared walk is short).  Batched trees: lanes have different
   // roots.  The candidate lists are the same either way.
   else if (roots || !SLX_TRAV_WAVE || (long long)n_leaves * 8 > (long long)nq) {
-    if (quad4 && SLX_TRAV_QUAD)
+    if (quad4 && SLX_TRAV_QUAD) {
+      CacheOut co{};
+      co.own = (const int4*)own;
       SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(active_grid(nq, kTravBS, n_active)), dim3(kTravBS), 0, st, nq,
                  (const float4*)q4, padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
-                 (const float4*)quad4, cap, cand, cand_n, status, roots, q_inst, order, n_active, CacheOut{});
+                 (const float4*)quad4, cap, cand, cand_n, status, roots, q_inst, order, n_active, co);
+    }
     else
       SLX_LAUNCH(name, k_traverse_tri, dim3(blocks_for(nq, kTravBS)), dim3(kTravBS), 0, st, nq, (const float4*)q4,
                  padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand, cand_n, status, roots,
@@ -1691,24 +1736,26 @@ namespace slx {
 // vertices, so it moves by at most D; the query box by d).  Filtering the stored
 // leaves by the exact box then gives exactly the walk's list and order.  The test
 // keeps an allowance of 2^-20 of the coordinates' magnitude for rounding.  A
-// query whose condition fails, or whose stored list was incomplete (more than
-// the cache's capacity, or a stack overflow), is appended to a list and walked
-// in full.
+// query whose condition fails, whose stored list was incomplete (more than the
+// cache's capacity, or a stack overflow), or whose list may overflow, is
+// appended to a list and walked in full.
 
 __global__ void __launch_bounds__(256)
 k_cand_filter(int nq, const float4* __restrict__ q4, float pad, float margin, const float4* __restrict__ nlo,
               const float4* __restrict__ nhi, int ccap, const int32_t* __restrict__ cache,
               const int32_t* __restrict__ cache_n, const float4* __restrict__ q0, const uint32_t* __restrict__ disp,
               int cap, int32_t* __restrict__ cand, int32_t* __restrict__ cand_n, int32_t* __restrict__ redo,
-              int32_t* status, const int32_t* __restrict__ order) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+              const int32_t* __restrict__ order) {
+  const int t = xcd_block() * 256 + threadIdx.x;
   const bool active = t < nq;
   const int q = active ? (order ? order[t] : t) : 0;
   bool ok = false;
-  int n = 0;
+  int n = 0, nown = 0;
   float3 p = make_float3(0.f, 0.f, 0.f);
   if (active) {
-    const float3 p0 = xyz(q0[t]);
+    const float4 c0 = q0[t];
+    const float3 p0 = xyz(c0);
+    nown = __float_as_int(c0.w);
     p = xyz(q4[q]);
     const float dq = fmaxf(fmaxf(fabsf(p.x - p0.x), fabsf(p.y - p0.y)), fabsf(p.z - p0.z));
     const float D = disp ? __uint_as_float(*disp) : 0.0f;
@@ -1716,8 +1763,30 @@ k_cand_filter(int nq, const float4* __restrict__ q4, float pad, float margin, co
     n = cache_n[t];
     ok = n <= ccap && (dq + D) + ldexpf(mag, -20) <= margin;   // false for NaN
   }
-  // queries to walk again, appended with one atomic per wavefront
-  const bool need = active && !ok;
+  // the stored leaves overlapping the exact box, in order
+  int cnt = 0;
+  if (ok) {
+    const float pp = pad * 1.0f;
+    const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
+    const int4* rows = reinterpret_cast<const int4*>(cache);
+    for (int j0 = 0; j0 < n; j0 += 4) {
+      const int4 r = rows[(size_t)(j0 >> 2) * nq + t];
+      const int Ls[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (j0 + k >= n) break;
+        const float4 lo = nlo[Ls[k]], hi = nhi[Ls[k]];
+        if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
+        if (cnt < cap) cand[(size_t)cnt * nq + q] = __float_as_int(lo.w);
+        ++cnt;
+      }
+    }
+  }
+  // queries to walk again, appended with one atomic per wavefront: a stale cache,
+  // or a list that may overflow (with the nown own faces the build left out of
+  // the cache, the full list has at most cnt + nown entries; the walk then
+  // decides, and sets the overflow bit if it does)
+  const bool need = active && (!ok || cnt + nown > cap);
   const unsigned long long m = __ballot(need);
   if (m) {
     const int lane = threadIdx.x & 63;
@@ -1727,25 +1796,7 @@ k_cand_filter(int nq, const float4* __restrict__ q4, float pad, float margin, co
     base = __shfl(base, leader, 64);
     if (need) redo[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = q;
   }
-  if (!active || !ok) return;
-  const float pp = pad * 1.0f;
-  const float3 a = make_float3(p.x - pp, p.y - pp, p.z - pp), b = make_float3(p.x + pp, p.y + pp, p.z + pp);
-  int cnt = 0;
-  const int4* rows = reinterpret_cast<const int4*>(cache);
-  for (int j0 = 0; j0 < n; j0 += 4) {
-    const int4 r = rows[(size_t)(j0 >> 2) * nq + t];
-    const int Ls[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (j0 + k >= n) break;
-      const float4 lo = nlo[Ls[k]], hi = nhi[Ls[k]];
-      if (!overlap(a, b, xyz(lo), xyz(hi))) continue;
-      if (cnt < cap) cand[(size_t)cnt * nq + q] = __float_as_int(lo.w);
-      ++cnt;
-    }
-  }
-  cand_n[q] = cnt < cap ? cnt : cap;
-  if (cnt > cap && status) atomicOr(status, kStatusCandOverflow);
+  if (active && !need) cand_n[q] = cnt;
 }
 
 __global__ void k_zero_one(int32_t* __restrict__ a) {
@@ -1817,8 +1868,9 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order, const int32_t* qedges, bool prezeroed, const int32_t* n_active,
-                    const int32_t* roots, const int32_t* q_inst) {
+                    const int32_t* roots, const int32_t* q_inst, const int32_t* own) {
   SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
+  SLX_REQUIRE(!own || !qedges, "slx_traverse_cached: own faces belong to vertex queries");
   SLX_REQUIRE(!qedges || kind == 0, "slx_traverse_cached: edge queries are walked, not cached");
   SLX_REQUIRE(!n_active || (kind == 0 && order), "slx_traverse_cached: a device query count needs a walk and a list");
   SLX_REQUIRE(c && c->redo && (kind == 0 || (c->cap >= 1 && c->leaves && c->count && c->q0)),
@@ -1847,7 +1899,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
       !wave && !roots && knobs().walk_wave && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
   int rc = SLX_OK;
   if (kind == 1) {
-    const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo};
+    const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo, (const int4*)own};
     if (wave)
       SLX_LAUNCH("cache_build_wave", k_traverse_tri_wave<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
                  (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4, cap, cand,
@@ -1862,10 +1914,11 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   } else if (kind == 2) {
     SLX_LAUNCH(name, k_cand_filter, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq, (const float4*)q4, padding,
                c->margin, (const float4*)nlo4, (const float4*)nhi4, c->cap, c->leaves, c->count,
-               (const float4*)c->q0, disp, cap, cand, cand_n, c->redo, status, order);
+               (const float4*)c->q0, disp, cap, cand, cand_n, c->redo, order);
   } else {
     CacheOut co{};
     co.redo = c->redo;
+    co.own = (const int4*)own;
     if (quad_wave)
       SLX_LAUNCH(name, k_traverse_quad_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
                  (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
@@ -1878,7 +1931,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   // the queries the first tier could not serve: a full walk over the list built on the device
   return traverse_launch("walk_redo", 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
                          nullptr, nullptr, cap, cand, cand_n, status, st, roots, q_inst, c->redo + 1, c->redo,
-                         qedges);
+                         qedges, own);
 }
 
 }  // namespace slx

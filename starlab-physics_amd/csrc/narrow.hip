@@ -211,6 +211,8 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
            int pcap, int32_t* __restrict__ pair_cnt, float4* __restrict__ pair_rec, int32_t* status, int q_begin,
            int q_end) {
   const int nqt = 2 * Q.nv + Q.nvs;
+  // (in dispatch order, not xcd_block(): grouping its blocks by XCD measured
+  // slower on the 1M-vertex drape, 119 -> 126 us for the first set)
   const int qg = q_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);   // queries [q_begin, q_end)
   if (qg >= q_end) return;
   int set, qv, ncand, nq;

@@ -24,6 +24,28 @@ constexpr int kStatusCellRange = SLX_STATUS_CELL_RANGE;
 constexpr int kStatusRecOverflow = SLX_STATUS_REC_OVERFLOW;
 constexpr int kStatusPairOverflow = SLX_STATUS_PAIR_OVERFLOW;
 
+// The dispatcher hands workgroups to the 8 XCDs round-robin by workgroup id,
+// and each XCD has its own L2.  xcd_block() renumbers a launch's workgroups so
+// that each XCD runs one contiguous range of logical blocks: blocks adjacent in
+// the processing order (queries close in space, sharing tree lines, faces and
+// vertices; neighbouring cloth rows) then share one L2 instead of eight.  It is
+// a bijection of [0, gridDim.x) (XCD x takes gridDim/8 blocks, one more for the
+// first gridDim % 8 XCDs).  Kernels that use it compute nothing from the block
+// order, so their results do not depend on it.
+#ifndef SLX_XCD_MAP
+#define SLX_XCD_MAP 1
+#endif
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+#if SLX_XCD_MAP
+  const int g = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % kXcds, tall = g / kXcds, extra = g % kXcds;
+  return x * tall + min(x, extra) + b / kXcds;
+#else
+  return (int)blockIdx.x;
+#endif
+}
+
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -117,7 +139,12 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
                     const float* quad4, const float* fmin4, const float* fmax4, const float* box6, int cap,
                     int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st, const int32_t* roots = nullptr,
                     const int32_t* q_inst = nullptr, const int32_t* order = nullptr,
-                    const int32_t* n_active = nullptr, const int32_t* qedges = nullptr);
+                    const int32_t* n_active = nullptr, const int32_t* qedges = nullptr,
+                    const int32_t* own = nullptr);
+
+// own (nullable, [nq][8] face ids, -1 padded): each vertex query's own faces,
+// counted in its list's length but not stored (the 4-wide walks and the cache
+// filter; the other walks store them: either list gives the same contacts)
 
 // two-tier traversal of a triangle tree with a candidate cache (lbvh.hip): kind 0
 // = walk, 1 = walk with the enlarged box filling the cache, 2 = filter the cache;
@@ -127,7 +154,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false,
                     const int32_t* n_active = nullptr, const int32_t* roots = nullptr,
-                    const int32_t* q_inst = nullptr);
+                    const int32_t* q_inst = nullptr, const int32_t* own = nullptr);
 
 // Counters a kernel zeroes for the next pass (thread i < n of block 0 writes *p[i] = 0).
 struct ZeroList {

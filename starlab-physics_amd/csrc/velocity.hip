@@ -218,7 +218,7 @@ k_vel_contacts(int nv, int nvs, const int32_t* __restrict__ fdy, const int32_t* 
                int32_t* __restrict__ bcnt, float4* __restrict__ brec, float4* __restrict__ self_acc,
                int32_t* status) {
   const int nqt = 2 * nv + nvs;
-  const int qg = blockIdx.x * blockDim.x + threadIdx.x;
+  const int qg = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (qg >= nqt) return;
   const int set = qg < nv ? 0 : (qg < 2 * nv ? 1 : 2);
   const int qv = set == 0 ? qg : (set == 1 ? qg - nv : qg - 2 * nv);

@@ -178,7 +178,7 @@ k_spring(int nv, const float4* __restrict__ x4, const float4* __restrict__ v4,
          const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_width,
          const int2* __restrict__ ent, float k_s, float k_b, float dt, float3 gdt2,
          float damping) {
-  const int base = blockIdx.x * blockDim.x * VPT + threadIdx.x;
+  const int base = xcd_block() * (int)blockDim.x * VPT + (int)threadIdx.x;
 #pragma unroll
   for (int j = 0; j < VPT; ++j) {
     const int v = base + j * (int)blockDim.x;

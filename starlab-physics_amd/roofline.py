@@ -8,7 +8,7 @@ whatever the layout: each array counted once per pass, on-chip temporaries
 (dx, nc) not counted.  These are the figures roofline fractions are quoted on.
 V/E/B/F are the dynamic mesh's vertex, edge, bending and face counts, Vs/Fs the
 static mesh's, c[set] the measured mean number of broad-phase candidates per
-query of a query set:
+query of a query set (the reference's full lists, key "cbar"):
 
   predict (compute_y)          40 V           read x, v, f; write y
   one Jacobi iteration         12 (E+B) + 40 V  ids + rest length; read y, x, f*w; write y
@@ -37,7 +37,9 @@ The candidate cache is an implementation device: a walk that builds it and a
 filter that reads it are both, algorithmically, one traversal of the query set.
 
 IMPLEMENTATION bytes (`kernel_impl_bytes`, `step_impl_bytes`) -- what this
-package's layout moves at best: float4 vertex records (16 B per position), the
+package's layout moves at best, with c the mean length of the lists as stored
+(key "cbar_stored": the cloth vertices' lists of the cloth tree leave out their
+own faces): float4 vertex records (16 B per position), the
 4-wide node lines a walk loads (128 B per line, 128 (Ft - 1) + 32 for the
 array), the even-depth 4-wide lines the refit writes besides the binary node
 arrays, the cache's stored leaves and snapshots:
@@ -149,6 +151,8 @@ def impl_edge_traversal(ne, Ft, ce, quad=True):
 def _table(n: dict, impl: bool):
     V, E, B, F = n["V"], n["E"], n["B"], n["F"]
     c = n.get("cbar", {})
+    if impl:                           # the lists as stored (without the cloth vertices' own faces)
+        c = n.get("cbar_stored", c)
     cs = n.get("cache", {})            # mean stored leaves per query of the cached sets
     sp = spring_pass(V, E, B)
     t = {

@@ -44,6 +44,13 @@ Extensions (all default to the reference's behaviour):
   static_rebuild_every_frame (True)  rebuild the static tree at every forward as
                                    the reference does (False: build it once and
                                    again only when the padding changes)
+  exclude_own_faces (True)         triangle trees: the cloth vertices' lists of
+                                   the cloth tree leave out each vertex's own
+                                   faces (which the narrow phase skips,
+                                   XPBD.py:258) but count them toward the cap;
+                                   positions and velocities are unchanged, the
+                                   lists shorter (their counts then count the
+                                   stored candidates)
   check_status (False)             read the status word after every forward
                                    and raise SlxError naming the flags set
   spring_launch (None)             (threads per block, vertices per thread) of
@@ -60,6 +67,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _abi
+from . import topology as _topo
 from .broadphase import DeviceTree
 
 
@@ -128,6 +136,7 @@ class Solver:
         self.active_grid = None
         self.refit_shape = None
         self.walk_wave = None
+        self.exclude_own_faces = True
         self._ctx = None           # side streams + events of this solver
 
     # ------------------------------------------------------------ reference API
@@ -363,6 +372,10 @@ class Solver:
             d.dyn = cd["dyn"].struct()
             d.cap, d.rec_cap = self.cache_size, cd["rec_cap"]
             d.cand, d.cand_n, d.pair_num = P(cd["cand"]), P(cd["cand_n"]), P(cd["pair_num"])
+            if self.exclude_own_faces and self.bvh_mode == "tri":
+                if cd.get("own") is None:
+                    cd["own"] = torch.from_numpy(_topo.own_faces(m.num_verts, m.fid_np)).to(self.device)
+                d.own_faces = P(cd["own"])
             if cd.get("caches"):
                 for k, c in enumerate(cd["caches"]):
                     if c is not None:

@@ -65,6 +65,29 @@ def edge_order(edges: np.ndarray, vert_order: np.ndarray) -> np.ndarray:
     return np.argsort(rank[e[:, 0]], kind="stable").astype(np.int32)
 
 
+def own_faces(nv: int, faces: np.ndarray, width: int = 8) -> np.ndarray:
+    """int32 [nv, width]: the faces incident to each vertex, in increasing face
+    id, -1 padded.  A vertex on more than `width` faces gets a row of -1 (its
+    candidate lists then keep its own faces; the narrow phase skips them)."""
+    fid = np.asarray(faces, np.int64).reshape(-1, 3)
+    out = np.full((nv, width), -1, np.int32)
+    if len(fid) == 0 or nv == 0:
+        return out
+    v = fid.reshape(-1)
+    f = np.repeat(np.arange(len(fid), dtype=np.int64), 3)
+    o = np.lexsort((f, v))
+    v, f = v[o], f[o]
+    first = np.ones(len(v), bool)              # a face listing a vertex twice counts once
+    first[1:] = (v[1:] != v[:-1]) | (f[1:] != f[:-1])
+    v, f = v[first], f[first]
+    cnt = np.bincount(v, minlength=nv)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    rank = np.arange(len(v)) - start[v]
+    ok = cnt[v] <= width
+    out[v[ok], rank[ok]] = f[ok]
+    return out
+
+
 def bending_pairs(faces: np.ndarray) -> np.ndarray:
     """Opposite-vertex pairs across interior edges.
 

@@ -15,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def drape(n, walk_wave=None):
+def drape(n, walk_wave=None, own=True, cap=None):
     import bench
     slx = importlib.import_module("starlab-physics_amd")
     from oracle import oracle as orc
@@ -23,6 +23,9 @@ def drape(n, walk_wave=None):
     sim.enable_velocity_update = True          # what the benchmark runs (friction keeps the drape at rest)
     sim.candidate_cache = True
     sim.walk_wave = walk_wave
+    sim.exclude_own_faces = own
+    if cap is not None:
+        sim.cache_size = cap
     st = dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
     ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
@@ -31,11 +34,12 @@ def drape(n, walk_wave=None):
     return cloth, sim, ref
 
 
-def check_frames(n, frames, substeps=20, iters=2, walk_wave=None):
-    cloth, sim, ref = drape(n, walk_wave)
+def check_frames(n, frames, substeps=20, iters=2, walk_wave=None, own=True):
+    cloth, sim, ref = drape(n, walk_wave, own)
     pairs = 0
     for f in range(frames):
-        print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters} walk_wave={walk_wave}", flush=True)
+        print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters} walk_wave={walk_wave} own={own}",
+              flush=True)
         sim.clear_status()
         sim.forward(substeps, iters)
         ref.forward(substeps, iters)
@@ -49,11 +53,37 @@ def check_frames(n, frames, substeps=20, iters=2, walk_wave=None):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("walk_wave", [False, True])
-def test_drape_256_ten_frames_bit_exact(walk_wave):
+@pytest.mark.parametrize("walk_wave,own", [(False, True), (True, True), (False, False)])
+def test_drape_256_ten_frames_bit_exact(walk_wave, own):
     """Both walks of the cloth tree: one per lane, and one shared walk per
-    wavefront (k_traverse_quad_wave)."""
-    check_frames(256, 10, walk_wave=walk_wave)
+    wavefront (k_traverse_quad_wave); the cloth's lists with and without its
+    vertices' own faces."""
+    check_frames(256, 10, walk_wave=walk_wave, own=own)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("own", [True, False])
+def test_drape_256_overflowing_lists_bit_exact(own):
+    """A cap of 7 candidates (six of them a draped vertex's own faces): lists
+    overflow wherever the cloth folds onto itself.  The overflow flag is raised in
+    exactly the frames the oracle's lists overflow, and positions and velocities
+    stay bit-exact: lists that leave out the own faces still count them toward the
+    cap and so keep the entries the truncated full lists keep."""
+    cloth, sim, ref = drape(256, own=own, cap=7)
+    flagged = 0
+    for f in range(6):
+        print(f"[drape-overflow] n=256 cap=7 frame={f} own={own}", flush=True)
+        before = ref.stats["overflow"]
+        sim.clear_status()
+        sim.forward(20, 2)
+        ref.forward(20, 2)
+        st = sim.status()
+        assert st & ~1 == 0, (f, st)
+        assert (st & 1 != 0) == (ref.stats["overflow"] > before), f
+        flagged += st & 1
+        np.testing.assert_array_equal(cloth.positions(), ref.x, err_msg=f"x, frame {f}")
+        np.testing.assert_array_equal(cloth.v4[:, :3].cpu().numpy(), ref.v, err_msg=f"v, frame {f}")
+    assert flagged > 0, "the cap must be exceeded"
 
 
 @pytest.mark.timeout(900)

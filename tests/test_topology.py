@@ -216,3 +216,23 @@ def test_face_edges_and_edge_order(slx, rng):
     assert np.all(np.diff(order)[same] > 0)          # stable within a first vertex
     with pytest.raises(ValueError):
         slx.topology.face_edges(f, e[::-1])          # edges must be sorted
+
+
+def test_own_faces_are_the_incident_faces(slx):
+    """Row v of own_faces is the set of faces the narrow phase skips for vertex v
+    (the is_in_face test, XPBD.py:258: v is a corner of the face), in increasing
+    face id; a vertex on more than eight faces keeps none (a row of -1)."""
+    topo = __import__("importlib").import_module("starlab-physics_amd.topology")
+    g = slx.scene.grid_cloth(7, size=1.0, height=0.2, jitter=1e-3, seed=0)
+    fan = np.array([[0, 1 + k, 1 + (k + 1) % 10] for k in range(10)])     # vertex 0 on ten faces
+    odd = np.array([[11, 12, 12], [11, 12, 13]])                           # a face naming 12 twice
+    for nv, faces in ((len(g.x), g.faces), (14, np.concatenate([fan, odd]))):
+        own = topo.own_faces(nv, faces)
+        assert own.shape == (nv, 8) and own.dtype == np.int32
+        for v in range(nv):
+            inc = [f for f in range(len(faces)) if v in faces[f]]
+            want = inc + [-1] * (8 - len(inc)) if len(inc) <= 8 else [-1] * 8
+            assert own[v].tolist() == want, v
+    own = topo.own_faces(len(g.x), g.faces)
+    assert (own >= 0).sum(axis=1).max() == 6          # a regular grid: six faces per interior vertex
+    assert topo.own_faces(3, np.zeros((0, 3), np.int32)).tolist() == [[-1] * 8] * 3
