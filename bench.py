@@ -43,6 +43,8 @@ def parse():
                    help="candidate cache margin as a multiple of the padding (default: the solver's)")
     p.add_argument("--cache-every", type=int, default=None, help="substeps between candidate cache rebuilds")
     p.add_argument("--walk-stack", type=int, default=None, help="first-tier triangle walk stack depth (16/24/32/64)")
+    p.add_argument("--refit-shape", default=None,
+                   help="SUB,TOP: leaves per refit subtree workgroup and top nodes kept in LDS")
     p.add_argument("--walk-wave", type=int, default=None,
                    help="1: cloth-vertex walks of the cloth tree as one shared walk per wavefront; 0: one per lane")
     p.add_argument("--no-friction", action="store_true",
@@ -228,6 +230,8 @@ def main():
         sim.walk_stack = a.walk_stack
     if a.walk_wave is not None:
         sim.walk_wave = bool(a.walk_wave)
+    if a.refit_shape is not None:
+        sim.refit_shape = tuple(int(v) for v in a.refit_shape.split(","))
     if a.cache_margin is not None:
         sim.cache_margin = a.cache_margin * sim.padding
     if a.cache_every is not None:
@@ -306,7 +310,7 @@ def main():
     # calibrated on a copy kernel), valid for the default workload
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
     default_run = (collision and a.velocity and not a.no_cache and a.cache_margin is None and a.cache_every is None
-                   and a.walk_stack is None and a.walk_wave is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
+                   and a.walk_stack is None and a.walk_wave is None and a.refit_shape is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
                    and not a.edge_edge)
     if default_run and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)

@@ -279,7 +279,7 @@ int slx_traverse_cached(int kind, int nq, const float* q4, float padding, int n_
                         const float* nlo4, const float* nhi4, const float* quad4,
                         const slx_cand_cache* cache, const uint32_t* disp, int cap, int32_t* cand,
                         int32_t* cand_n, int32_t* status, const int32_t* order, slx_stream_t stream);
-/* Stack depth of the first-tier walk: 16, 24, 32 (default) or 64 entries. */
+/* Stack depth of the first-tier walk: 16, 24 (default), 32 or 64 entries. */
 int slx_set_walk_stack(int depth);
 /* Workgroups (1..65535, default 2048) of a walk whose query count is on the
  * device (the second tier, the edge gate's live lists): such a walk loops over

@@ -141,11 +141,12 @@ bool cacheable(const slx_tree& t, const slx_cand_cache& c) { return tiered(t, c)
 // false, then filtered; otherwise only its redo list serves the two-tier walk.
 // disp: the tree's displacement since the cache was built (nullable = 0)
 // own (nullable): the queries' own faces, left out of the stored lists
+// redo_name: timer label of the second-tier walk
 // (the frame keeps every redo counter at zero between passes: prezeroed walks)
 int query(const char* name, const slx_tree& t, int nq, const float* q4, const int32_t* q_inst,
           const int32_t* q_order, float pad, int cap, int32_t* cand, int32_t* cand_n, int32_t* status,
           slx_stream_t s, const slx_cand_cache* cache = nullptr, bool use_cache = false, bool* built = nullptr,
-          const uint32_t* disp = nullptr, const int32_t* own = nullptr) {
+          const uint32_t* disp = nullptr, const int32_t* own = nullptr, const char* redo_name = nullptr) {
   int nl = t.mode == 0 ? kCells : t.nf;
   if (t.nf <= 0) nl = 0;
   const bool seg = batched(t);
@@ -157,7 +158,7 @@ int query(const char* name, const slx_tree& t, int nq, const float* q4, const in
     }
     return traverse_cached(name, kind, nq, q4, pad, nl, t.nlo4, t.nhi4, t.quad4, cache, disp, cap, cand, cand_n,
                            status, (hipStream_t)s, q_order, nullptr, true, nullptr, seg ? t.roots : nullptr,
-                           seg ? q_inst : nullptr, own);
+                           seg ? q_inst : nullptr, own, redo_name);
   }
   return traverse_launch(name, t.mode, nq, q4, pad, nl, t.num, t.sorted_faces, t.nlo4, t.nhi4,
                          t.mode == 1 ? t.quad4 : nullptr, t.fmin4, t.fmax4, t.box6, cap, cand, cand_n, status,
@@ -451,13 +452,15 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
         if (refit_pending && hipStreamWaitEvent(side->q, side->sub, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
-                   n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr);
+                   n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr, nullptr,
+                   "walk_redo_vt_st");
         if (rc) return rc;
         if (!tv_fresh) {                     // obstacle vertices vs the dynamic tree, beside its cache build
           if (refit_pending && hipStreamWaitEvent(side->q, side->join, 0) != hipSuccess)
             return fail(SLX_EHIP, "slx_frame: fork failed");
           rc = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
-                     d->status, (slx_stream_t)side->q, &d->cache[2], reuse_t, &built[2], d->dyn.disp);
+                     d->status, (slx_stream_t)side->q, &d->cache[2], reuse_t, &built[2], d->dyn.disp, nullptr,
+                     "walk_redo_tv_st");
           if (rc) return rc;
           if (hipEventRecord(side->tvjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
           tv_fresh = tv_side = true;
@@ -477,7 +480,8 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       rc = join_refit();                     // the dynamic tree is needed from here on
       if (rc) return rc;
       rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
-                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp, d->own_faces);
+                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp, d->own_faces,
+                 "walk_redo_vt_dy");
       if (rc) return rc;
       if (tv_side && hipStreamWaitEvent(st, side->tvjoin, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: join failed");

@@ -42,17 +42,7 @@ constexpr int kTravBS = 128;     // traversal block size
 
 // ----------------------------------------------------------------- AABBs
 
-// computeAABB_faces, meshTaichiWrapper.py:350-361: componentwise min/max of the
-// three corners, minus/plus the padding
-__device__ __forceinline__ void corner_box(float3 a, float3 b, float3 c, float pad, float3& lo, float3& hi) {
-  const float p = pad * 1.0f;
-  lo = make_float3(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p, fminf(fminf(a.z, b.z), c.z) - p);
-  hi = make_float3(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p, fmaxf(fmaxf(a.z, b.z), c.z) + p);
-}
-__device__ __forceinline__ void face_box(const int32_t* __restrict__ faces, const float4* __restrict__ x4, int f,
-                                         float pad, float3& lo, float3& hi) {
-  corner_box(xyz(x4[faces[3 * f]]), xyz(x4[faces[3 * f + 1]]), xyz(x4[faces[3 * f + 2]]), pad, lo, hi);
-}
+// (corner_box / face_box, the padded box of a face: slx_common.h)
 
 __global__ void k_face_aabbs(int nf, const int32_t* __restrict__ faces, const float4* __restrict__ x4,
                              float pad, float4* __restrict__ fmin, float4* __restrict__ fmax) {
@@ -1139,7 +1129,7 @@ k_traverse_tri(int nq, const float4* __restrict__ q4, float pad, const int32_t* 
                int32_t* status, const int32_t* __restrict__ roots, const int32_t* __restrict__ q_inst,
                const int32_t* __restrict__ order, const int32_t* __restrict__ n_active) {
   __shared__ uint32_t s_stack[kStack * kTravBS];
-  const int t = xcd_block() * kTravBS + threadIdx.x;
+  const int t = (n_active ? (int)blockIdx.x : xcd_block()) * kTravBS + threadIdx.x;
   if (t >= nq || (n_active && t >= *n_active)) return;
   const int q = order ? order[t] : t;     // spatially coherent queries share nodes within a wavefront
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
@@ -1275,7 +1265,8 @@ k_traverse_tri_wave(int nq, const float4* __restrict__ q4, float pad, const int3
   // a grid-stride loop (launches sized by n_active get a capped grid); the step is
   // per workgroup, so every lane of a wavefront takes the same iterations
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int base = xcd_block() * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
+  for (int base = (n_active ? (int)blockIdx.x : xcd_block()) * 256; base < limit; base += gridDim.x * 256)
+  [&](const int t) {
   const int lane = threadIdx.x & 63;
   const bool active = t < limit;
   const int q = active ? (order ? order[t] : t) : 0;
@@ -1402,7 +1393,8 @@ k_traverse_quad_wave(int nq, const float4* __restrict__ q4, float pad, const int
                      int32_t* __restrict__ cand_n, int32_t* status, const int32_t* __restrict__ order,
                      const int32_t* __restrict__ n_active, CacheOut co) {
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int base = xcd_block() * 256; base < limit; base += gridDim.x * 256) [&](const int t) {
+  for (int base = (n_active ? (int)blockIdx.x : xcd_block()) * 256; base < limit; base += gridDim.x * 256)
+  [&](const int t) {
   const int lane = threadIdx.x & 63;
   const bool active = t < limit;
   const int q = active ? (order ? order[t] : t) : 0;
@@ -1555,7 +1547,8 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
   __shared__ uint32_t s_stack[S * kTravBS];
   // a grid-stride loop (launches sized by n_active get a capped grid)
   const int limit = n_active ? min(nq, *n_active) : nq;
-  for (int t0 = xcd_block() * kTravBS + threadIdx.x; t0 < limit; t0 += gridDim.x * kTravBS) [&](const int t) {
+  for (int t0 = (n_active ? (int)blockIdx.x : xcd_block()) * kTravBS + threadIdx.x; t0 < limit;
+       t0 += gridDim.x * kTravBS) [&](const int t) {
   const int q = order ? order[t] : t;
   uint32_t* stk = s_stack + threadIdx.x;          // entry d at stk[d * kTravBS]
   float3 a, b;
@@ -1868,7 +1861,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* c,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order, const int32_t* qedges, bool prezeroed, const int32_t* n_active,
-                    const int32_t* roots, const int32_t* q_inst, const int32_t* own) {
+                    const int32_t* roots, const int32_t* q_inst, const int32_t* own, const char* redo_name) {
   SLX_REQUIRE(kind >= 0 && kind <= 2, "slx_traverse_cached: kind must be 0 (walk), 1 (build) or 2 (filter)");
   SLX_REQUIRE(!own || !qedges, "slx_traverse_cached: own faces belong to vertex queries");
   SLX_REQUIRE(!qedges || kind == 0, "slx_traverse_cached: edge queries are walked, not cached");
@@ -1929,7 +1922,8 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   }
   if (rc) return rc;
   // the queries the first tier could not serve: a full walk over the list built on the device
-  return traverse_launch("walk_redo", 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
+  return traverse_launch(redo_name ? redo_name : "walk_redo", 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4,
+                         nhi4, quad4, nullptr,
                          nullptr, nullptr, cap, cand, cand_n, status, st, roots, q_inst, c->redo + 1, c->redo,
                          qedges, own);
 }

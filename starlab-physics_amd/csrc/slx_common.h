@@ -31,7 +31,9 @@ constexpr int kStatusPairOverflow = SLX_STATUS_PAIR_OVERFLOW;
 // vertices; neighbouring cloth rows) then share one L2 instead of eight.  It is
 // a bijection of [0, gridDim.x) (XCD x takes gridDim/8 blocks, one more for the
 // first gridDim % 8 XCDs).  Kernels that use it compute nothing from the block
-// order, so their results do not depend on it.
+// order, so their results do not depend on it.  (Not for a launch whose active
+// work is a prefix of its grid, e.g. a count known only on the device: grouped
+// by XCD, that prefix would run on the first XCDs alone.)
 #ifndef SLX_XCD_MAP
 #define SLX_XCD_MAP 1
 #endif
@@ -71,6 +73,18 @@ __device__ __forceinline__ bool overlap(float3 amin, float3 amax, float3 bmin, f
 }
 
 inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// computeAABB_faces, meshTaichiWrapper.py:350-361: componentwise min/max of the
+// three corners, minus/plus the padding
+__device__ __forceinline__ void corner_box(float3 a, float3 b, float3 c, float pad, float3& lo, float3& hi) {
+  const float p = pad * 1.0f;
+  lo = make_float3(fminf(fminf(a.x, b.x), c.x) - p, fminf(fminf(a.y, b.y), c.y) - p, fminf(fminf(a.z, b.z), c.z) - p);
+  hi = make_float3(fmaxf(fmaxf(a.x, b.x), c.x) + p, fmaxf(fmaxf(a.y, b.y), c.y) + p, fmaxf(fmaxf(a.z, b.z), c.z) + p);
+}
+__device__ __forceinline__ void face_box(const int32_t* __restrict__ faces, const float4* __restrict__ x4, int f,
+                                         float pad, float3& lo, float3& hi) {
+  corner_box(xyz(x4[faces[3 * f]]), xyz(x4[faces[3 * f + 1]]), xyz(x4[faces[3 * f + 2]]), pad, lo, hi);
+}
 
 // Per-vertex contact-delta buckets (position and velocity passes share them):
 // bcnt[v] claimed slots, brec[v * bucket + i] = (delta, order key bits),
@@ -149,12 +163,14 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
 // two-tier traversal of a triangle tree with a candidate cache (lbvh.hip): kind 0
 // = walk, 1 = walk with the enlarged box filling the cache, 2 = filter the cache;
 // queries the first tier cannot serve are walked again with the deep stack
+// (timer label redo_name, default "walk_redo")
 int traverse_cached(const char* name, int kind, int nq, const float* q4, float padding, int n_leaves,
                     const float* nlo4, const float* nhi4, const float* quad4, const slx_cand_cache* cache,
                     const uint32_t* disp, int cap, int32_t* cand, int32_t* cand_n, int32_t* status, hipStream_t st,
                     const int32_t* order, const int32_t* qedges = nullptr, bool prezeroed = false,
                     const int32_t* n_active = nullptr, const int32_t* roots = nullptr,
-                    const int32_t* q_inst = nullptr, const int32_t* own = nullptr);
+                    const int32_t* q_inst = nullptr, const int32_t* own = nullptr,
+                    const char* redo_name = nullptr);
 
 // Counters a kernel zeroes for the next pass (thread i < n of block 0 writes *p[i] = 0).
 struct ZeroList {

@@ -163,4 +163,4 @@ def test_two_tier_walk_equals_traversal(slx, rng, depth):
             cache = bp.CandidateCache(nv, 256, 0.005, torch.device("cuda"))
             same(cached(A, bp, tree, q, pad, cap, cache, True, None), ref)
     finally:
-        A.call("slx_set_walk_stack", 32)
+        A.call("slx_set_walk_stack", 24)   # the library default

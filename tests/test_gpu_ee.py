@@ -118,7 +118,7 @@ def test_edge_traversal_two_tier_bit_exact(slx, orc, depth):
         cand, cnt, st = traverse_edges(A, tree, cloth.edge_indices, cloth.x4, ne, pad, cap,
                                        order=cloth.edge_order(), redo=redo)
     finally:
-        A.call("slx_set_walk_stack", 32)
+        A.call("slx_set_walk_stack", 24)   # the library default
     ot = orc.TriBVH(cloth.fid_np, cloth.positions(), pad)
     oc, on, fl = ot.query(*edge_boxes(cloth.positions(), cloth.eid_np, pad), cap)
     cnt, cand = cnt.cpu().numpy(), cand.cpu().numpy()
@@ -337,7 +337,7 @@ def test_frames_with_edge_edge_shallow_walk(slx):
             ref.forward(4)
             np.testing.assert_array_equal(cloth.positions(), ref.x)
     finally:
-        A.call("slx_set_walk_stack", 32)
+        A.call("slx_set_walk_stack", 24)   # the library default
 
 
 @pytest.mark.parametrize("blocks", [1, 3])
