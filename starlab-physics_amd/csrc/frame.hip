@@ -333,6 +333,10 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
                                    (reuse_t && cacheable(d->dyn, d->cache[2])));
   SLX_REQUIRE(!dyn_cached || (d->dyn.x_snap && d->dyn.disp), "slx_frame: cached queries of the dynamic tree need "
               "its x_snap and disp");
+  // caches rebuilt at every substep: x does not change between a rebuild and the
+  // filters that use it (the same substep), so the tree's displacement since is
+  // zero -- zeroed once below, with no snapshot and no measurement per substep
+  const bool every_substep = d->cache_every == 1;
   bool built[3] = {false, false, false};
   if (nvs > 0 && d->rebuild_static) {
     rc = build_tree(d->st, nvs, d->xs4, d->faces_st, pad, d->status, d->ws, d->ws_bytes, stream);
@@ -366,6 +370,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       add(d->ee.redo);
       add(d->ee.redo + 1 + (size_t)d->ee.ne);
     }
+    if (dyn_cached && every_substep) add((int32_t*)d->dyn.disp);
     rc = frame_zero(nv, d->rec_cap, d->ws, after, st);
     if (rc) return rc;
   }
@@ -380,7 +385,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
                           d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub,
                           lines_only);
       if (rc) return rc;
-      disp_pending = dyn_cached;             // measured on the main stream after the first spring pass
+      disp_pending = dyn_cached && !every_substep;   // measured on the main stream after the first spring pass
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
       refit_pending = true;
       tv_fresh = false;
@@ -389,7 +394,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       // caches rebuilt in this substep, against x as it is now (the refit above
       // builds the dynamic tree's boxes from this x)
       built[0] = built[1] = built[2] = false;
-      if (dyn_cached) {
+      if (dyn_cached && !every_substep) {
         rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);   // (zeroes disp)
         if (rc) return rc;
         snapped = true;
