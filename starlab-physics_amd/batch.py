@@ -78,17 +78,7 @@ class BatchedScene:
     def capture(self, n_substeps, n_iters):
         """Capture one frame into a HIP graph (buffers are allocated and the static
         tree built by an eager warm-up frame first)."""
-        import torch
-        self.sim.forward(n_substeps, n_iters)          # warm-up: allocations + static tree
-        torch.cuda.synchronize()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                self.sim.forward(n_substeps, n_iters)
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
+        g = self.sim.capture(n_substeps, n_iters)      # (an eager frame first: allocations, static tree)
         self.graph = g
         self.graph_shape = (n_substeps, n_iters)
         return g

@@ -41,6 +41,7 @@ Extensions (all default to the reference's behaviour):
                                    triangle trees); False walks and tests every
                                    edge
   status()                         device status word (overflow flags)
+  capture(n_substeps, n_iters)     one frame captured into a HIP graph for replay
   static_rebuild_every_frame (True)  rebuild the static tree at every forward as
                                    the reference does (False: build it once and
                                    again only when the padding changes)
@@ -171,6 +172,31 @@ class Solver:
             st = self.status()
             if st:
                 raise _abi.SlxError(f"slx_frame status {st:#x}: {_abi.status_flags(st)}")
+
+    def capture(self, n_substeps, n_iters=None):
+        """One frame, forward(n_substeps, n_iters), captured into a HIP graph
+        (torch.cuda.CUDAGraph) and returned: each graph.replay() runs that frame
+        on the solver's current state with its launch shapes and buffers baked
+        in, and no host work per launch.  An eager frame runs first (allocations,
+        the static tree), so the state advances by one frame; the captured frame
+        itself does not run.  Settings changed afterwards need a new capture;
+        check_status is not consulted inside the graph."""
+        import torch
+        self.forward(n_substeps, n_iters)
+        torch.cuda.synchronize()
+        check, self.check_status = self.check_status, False
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        try:
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self.forward(n_substeps, n_iters)
+        finally:
+            self.check_status = check
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        return g
 
     def __del__(self):
         ctx, self._ctx = getattr(self, "_ctx", None), None
