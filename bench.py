@@ -42,6 +42,10 @@ def parse():
     p.add_argument("--cache-margin", type=float, default=None,
                    help="candidate cache margin as a multiple of the padding (default: the solver's)")
     p.add_argument("--cache-every", type=int, default=None, help="substeps between candidate cache rebuilds")
+    p.add_argument("--static-cache-every", type=int, default=None,
+                   help="substeps between rebuilds of the cloth vertices' cache of the static tree")
+    p.add_argument("--static-cache-margin", type=float, default=None,
+                   help="that cache's margin as a multiple of the padding")
     p.add_argument("--walk-stack", type=int, default=None, help="first-tier triangle walk stack depth (16/24/32/64)")
     p.add_argument("--refit-shape", default=None,
                    help="SUB,TOP: leaves per refit subtree workgroup and top nodes kept in LDS")
@@ -236,6 +240,10 @@ def main():
         sim.cache_margin = a.cache_margin * sim.padding
     if a.cache_every is not None:
         sim.cache_every = a.cache_every
+    if a.static_cache_every is not None:
+        sim.static_cache_every = a.static_cache_every
+    if a.static_cache_margin is not None:
+        sim.static_cache_margin = a.static_cache_margin * sim.padding
     V, E, B, F = cloth.num_verts, cloth.num_edges, cloth.bending_constraint_count, cloth.num_faces
     for _ in range(a.warmup):
         sim.forward(a.substeps, a.iters)
@@ -311,6 +319,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "cloth1024_draped_traffic.json")
     default_run = (collision and a.velocity and not a.no_cache and a.cache_margin is None and a.cache_every is None
                    and a.walk_stack is None and a.walk_wave is None and a.refit_shape is None and a.n == 1024 and a.substeps == 20 and a.iters == 2
+                   and a.static_cache_every is None and a.static_cache_margin is None
                    and not a.edge_edge)
     if default_run and os.path.exists(tpath):
         tk = json.load(open(tpath))["kernels"].get(dom)
@@ -342,6 +351,9 @@ def main():
                    "candidate_cache": sim.candidate_cache,
                    "cache_margin_m": 0.25 * sim.padding if sim.cache_margin is None else sim.cache_margin,
                    "cache_every": sim.cache_every,
+                   "static_cache_every": sim.static_cache_every,
+                   "static_cache_margin_m": (2.0 * sim.padding if sim.static_cache_margin is None
+                                             else sim.static_cache_margin),
                    "thickness_m": SCENE["thickness"], "padding_m": SCENE["padding"], "cache_size": sim.cache_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": (achieved / 8000.0) if achieved else None, "traffic": traffic, "kernel": dom,

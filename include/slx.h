@@ -647,6 +647,10 @@ typedef struct slx_frame_desc {
      the stored entries.  (No reference counterpart: the reference's lists hold
      the incident faces too.) */
   const int32_t* own_faces;
+  /* cache_every_static: substeps between rebuilds of the first query set's cache
+     (cloth vertices against the static tree, which never moves: an entry stays
+     valid while its query stays within the margin); 0 = cache_every */
+  int32_t cache_every_static;
 } slx_frame_desc;
 
 size_t slx_frame_ws_bytes(const slx_frame_desc* d);

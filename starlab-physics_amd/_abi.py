@@ -141,7 +141,8 @@ class FrameDesc(C.Structure):
                 ("n_sewn", I32), ("sew_verts", PTR), ("sew_off", PTR), ("sew_list", PTR), ("sew_pairs", PTR),
                 ("k_sew", F32), ("sew_ws", PTR), ("ee", EdgeSets), ("ctx", PTR), ("spring_block", I32),
                 ("spring_vpt", I32), ("walk_stack", I32), ("active_grid", I32), ("refit_sub", I32),
-                ("refit_top", I32), ("walk_wave", I32), ("own_faces", PTR)]
+                ("refit_top", I32), ("walk_wave", I32), ("own_faces", PTR),
+                ("cache_every_static", I32)]
 
 
 class SlxError(RuntimeError):

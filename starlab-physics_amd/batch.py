@@ -67,6 +67,9 @@ class BatchedScene:
             sel[int(vb[i])] = 1
             sel[int(vb[i]) + n - 1] = 1
         sim.set_fixed_vertices(sel)
+        # the cloths fall onto their spheres: the obstacle set's cache is rebuilt
+        # every substep unless the autotuner finds otherwise
+        sim.static_cache_every = 1
         self.sim = sim
         self.graph = None
         self.shape = (256, 1)
@@ -104,7 +107,8 @@ class BatchedScene:
         spring kernels (threads per block x vertices per thread) and keep the
         fastest; then, one knob at a time with the others at their best, the
         first-tier walk's stack depth, the candidate cache's margin (a multiple
-        of the padding) and the refit's subtree size.  Each setting is captured
+        of the padding), the refit's subtree size and the interval of the
+        obstacle set's cache.  Each setting is captured
         afresh (launch shapes are baked into the graph).  No knob changes the
         results.  Returns (best spring shape, {shape: seconds per frame}); the
         other knobs' timings are in self.tuning."""
@@ -122,7 +126,8 @@ class BatchedScene:
             for name, values, apply in (
                     ("walk_stack", (16, 24, 32, 64), lambda v: setattr(self.sim, "walk_stack", v)),
                     ("cache_margin", (0.125, 0.25, 0.5), lambda v: setattr(self.sim, "cache_margin", v * pad)),
-                    ("refit_sub", (1024, 2048), lambda v: setattr(self.sim, "refit_shape", (v, 2048)))):
+                    ("refit_sub", (1024, 2048), lambda v: setattr(self.sim, "refit_shape", (v, 2048))),
+                    ("static_cache_every", (1, 4), lambda v: setattr(self.sim, "static_cache_every", v))):
                 t = {}
                 for v in values:
                     apply(v)

@@ -329,6 +329,11 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
   // the cloth-vertex sets are queried n_iters times per substep, the
   // obstacle-vertex set once
   const bool reuse_v = d->cache_every != 1 || d->n_iters > 1, reuse_t = d->cache_every != 1;
+  // the first set's cache (cloth vertices against the static tree) has an interval
+  // of its own: that tree never moves (D = 0), so a query's entry stays valid
+  // while the query itself stays within the margin
+  const int every_st = d->cache_every_static != 0 ? d->cache_every_static : d->cache_every;
+  const bool reuse_s = d->n_iters > 1 || (every_st != 1 && d->n_substeps > 1);
   const bool dyn_cached = coll && ((reuse_v && cacheable(d->dyn, d->cache[1])) ||
                                    (reuse_t && cacheable(d->dyn, d->cache[2])));
   SLX_REQUIRE(!dyn_cached || (d->dyn.x_snap && d->dyn.disp), "slx_frame: cached queries of the dynamic tree need "
@@ -390,10 +395,11 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       refit_pending = true;
       tv_fresh = false;
     }
+    if (coll && (s == 0 || (every_st > 0 && s % every_st == 0))) built[0] = false;
     if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
       // caches rebuilt in this substep, against x as it is now (the refit above
       // builds the dynamic tree's boxes from this x)
-      built[0] = built[1] = built[2] = false;
+      built[1] = built[2] = false;
       if (dyn_cached && !every_substep) {
         rc = slx_tree_snapshot(nv, d->x4, d->dyn.x_snap, d->dyn.disp, stream);   // (zeroes disp)
         if (rc) return rc;
@@ -457,7 +463,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
         if (refit_pending && hipStreamWaitEvent(side->q, side->sub, 0) != hipSuccess)
           return fail(SLX_EHIP, "slx_frame: fork failed");
         rc = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
-                   n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_v, &built[0], nullptr, nullptr,
+                   n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_s, &built[0], nullptr, nullptr,
                    "walk_redo_vt_st");
         if (rc) return rc;
         if (!tv_fresh) {                     // obstacle vertices vs the dynamic tree, beside its cache build

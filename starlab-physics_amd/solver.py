@@ -21,6 +21,11 @@ Extensions (all default to the reference's behaviour):
   cache_capacity (None)            leaves stored per query (None = 2 cache_size)
   cache_every (1)                  substeps between cache rebuilds (<= 0: once
                                    per frame)
+  static_cache_every (4)           the same for the cloth vertices' cache of the
+  static_cache_margin (None)       static tree (None: cache_every), and its margin
+                                   (None = 2 x padding): that tree never moves, so
+                                   an entry stays valid while its query stays
+                                   within the margin
   load_sewing_pairs(path | pairs)  sewing pairs (XPBD.py:405-417); with
   enable_sewing (False)            they are applied after the collision pass of
                                    every iteration (XPBD.py:418-441, whose call
@@ -112,6 +117,8 @@ class Solver:
         self.cache_margin = None
         self.cache_capacity = None
         self.cache_every = 1
+        self.static_cache_every = 4      # the static set's cache interval (None: cache_every)
+        self.static_cache_margin = None  # and its margin (None: 2 x padding)
         self.enable_sewing = False
         self.sewing_stiffness = None
         self._sew = None
@@ -278,7 +285,8 @@ class Solver:
         on = self.candidate_cache and tri
         ccap = (2 * self.cache_size if self.cache_capacity is None else int(self.cache_capacity)) if on else 0
         margin = _f32(0.25 * self.padding if self.cache_margin is None else self.cache_margin)
-        key = (tri, ccap, margin)
+        smargin = _f32(2.0 * self.padding if self.static_cache_margin is None else self.static_cache_margin)
+        key = (tri, ccap, margin, smargin)
         if cd.get("cache_key") == key:
             return
         cd["cache_key"] = key
@@ -289,7 +297,7 @@ class Solver:
         nv, nvs = m.num_verts, (s.num_verts if s is not None else 0)
         if on:
             cd["dyn"].enable_snapshot()
-        cd["caches"] = [CandidateCache(nv, ccap, margin, self.device) if s is not None else None,
+        cd["caches"] = [CandidateCache(nv, ccap, smargin, self.device) if s is not None else None,
                         CandidateCache(nv, ccap, margin, self.device),
                         CandidateCache(nvs, ccap, margin, self.device) if s is not None else None]
 
@@ -407,6 +415,7 @@ class Solver:
                     if c is not None:
                         d.cache[k] = c.struct()
                 d.cache_every = int(self.cache_every)
+                d.cache_every_static = int(self.static_cache_every or 0)
             if self.enable_velocity_update:
                 d.pair_cap, d.pair_cnt, d.pair_rec = cd["pair_cap"], P(cd["pair_cnt"]), P(cd["pair_rec"])
                 if s is not None:

@@ -132,6 +132,8 @@ def test_autotune_picks_a_valid_shape(slx):
     assert bs.tuning["walk_stack"]["best"] in (16, 24, 32, 64) and bs.sim.walk_stack == bs.tuning["walk_stack"]["best"]
     assert bs.tuning["cache_margin"]["best"] in (0.125, 0.25, 0.5)
     assert bs.tuning["refit_sub"]["best"] in (1024, 2048)
+    assert bs.tuning["static_cache_every"]["best"] in (1, 4)
+    assert bs.sim.static_cache_every == bs.tuning["static_cache_every"]["best"]
     _, _, _, ref = small_batch(slx, seed=9)
     ref.forward(5, 1)                 # the final capture ran one eager frame from the initial state
     for _ in range(3):

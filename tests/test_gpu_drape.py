@@ -15,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def drape(n, walk_wave=None, own=True, cap=None):
+def drape(n, walk_wave=None, own=True, cap=None, static_every=None):
     import bench
     slx = importlib.import_module("starlab-physics_amd")
     from oracle import oracle as orc
@@ -26,6 +26,8 @@ def drape(n, walk_wave=None, own=True, cap=None):
     sim.exclude_own_faces = own
     if cap is not None:
         sim.cache_size = cap
+    if static_every is not None:
+        sim.static_cache_every = static_every
     st = dict(x=terr.positions(), faces=terr.fid_np, m_inv=terr.m_inv_np)
     ref = orc.OracleCloth(cloth.host_dict(), st, dhat=sim.dHat, stiffness_stretch=sim.stiffness_stretch,
                           stiffness_bending=sim.stiffness_bending, g=sim.g, dt=sim.dt, padding=sim.padding,
@@ -34,8 +36,8 @@ def drape(n, walk_wave=None, own=True, cap=None):
     return cloth, sim, ref
 
 
-def check_frames(n, frames, substeps=20, iters=2, walk_wave=None, own=True):
-    cloth, sim, ref = drape(n, walk_wave, own)
+def check_frames(n, frames, substeps=20, iters=2, walk_wave=None, own=True, static_every=None):
+    cloth, sim, ref = drape(n, walk_wave, own, static_every=static_every)
     pairs = 0
     for f in range(frames):
         print(f"[drape] n={n} frame={f} substeps={substeps} iters={iters} walk_wave={walk_wave} own={own}",
@@ -53,12 +55,14 @@ def check_frames(n, frames, substeps=20, iters=2, walk_wave=None, own=True):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("walk_wave,own", [(False, True), (True, True), (False, False)])
-def test_drape_256_ten_frames_bit_exact(walk_wave, own):
+@pytest.mark.parametrize("walk_wave,own,static_every", [(False, True, None), (True, True, None), (False, False, None),
+                                                        (False, True, 1), (False, True, -1)])
+def test_drape_256_ten_frames_bit_exact(walk_wave, own, static_every):
     """Both walks of the cloth tree: one per lane, and one shared walk per
     wavefront (k_traverse_quad_wave); the cloth's lists with and without its
-    vertices' own faces."""
-    check_frames(256, 10, walk_wave=walk_wave, own=own)
+    vertices' own faces; the terrain set's cache rebuilt every 4 substeps (the
+    default), every substep, and once per frame."""
+    check_frames(256, 10, walk_wave=walk_wave, own=own, static_every=static_every)
 
 
 @pytest.mark.timeout(900)
