@@ -161,10 +161,12 @@ int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int
  * write it).  The reference refits with one thread per leaf and global arrival
  * counters (lbvh_cell.py:490-522); the schedule instead cuts the tree into subtrees
  * of at most sub_leaves leaves, each refit by one workgroup in LDS, and the nodes
- * above them, refit by one workgroup, so no device-wide fence is needed.  The
- * boxes are the same.  slx_set_refit_shape sets sub_leaves (1..2048, default 2048)
- * and how many top nodes keep their state in LDS (0..2048, default 2048) for
- * schedules made afterwards. */
+ * above them: their maximal subtrees of at most 64 x sub_leaves leaves one
+ * workgroup each, then the rest in one workgroup, so no device-wide fence is
+ * needed.  The boxes are the same.  slx_set_refit_shape sets sub_leaves (1..2048,
+ * default 2048) and how many top nodes of one workgroup keep their state in LDS
+ * (0..2048, default 2048; at most 512 are kept, the others count arrivals in
+ * global memory) for schedules made afterwards. */
 size_t slx_refit_sched_ints(int n_leaves);
 int slx_refit_schedule(int n_leaves, const int32_t* child_b, const int32_t* parent,
                        const int32_t* range_l, const int32_t* range_r, int32_t* sched,

@@ -221,19 +221,25 @@ __global__ void k_fill_i32(int n, int32_t* __restrict__ p, int32_t v) {
 // topology, cuts the tree into maximal subtrees of at most T leaves (a leaf range
 // of an LBVH node is contiguous, and every internal node's index lies inside its
 // own range), each refit by one workgroup in LDS with workgroup-scope arrival
-// counters, and the top part (nodes over more than T leaves), refit by a single
-// workgroup whose per-node state also lives in LDS.  The stream orders the two
+// counters, and the top part (nodes over more than T leaves) in LDS too: its
+// maximal subtrees of at most T2 leaves (groups) one workgroup each, then the
+// nodes above them (the final top) in a single workgroup.  The stream orders the
 // kernels; no device-scope fence is executed.  Every node is still the union of its
 // two children in the same operand order as the serial refit, so the boxes are
 // bit-identical.
 //
 // Schedule (int32), for a tree over n leaves:
-//   [0] n_sub  [1] n_top  [2] T  [3] top nodes with LDS state  [4..7] unused
+//   [0] n_sub  [1] n_top  [2] T  [3] top nodes with LDS state per workgroup
+//   [4] even lines  [5] n_groups  [6] n_final  [7] T2
 //   sub[n]      int4 {subtree root, first leaf, last leaf, up}
 //   top_slot[n] internal node -> top slot (-1 below the top part)
 //   top_node[n] top slot -> node
 //   top_up[n]   top slot -> up link
 //   gflag[n]    arrival counters of top slots without LDS state (left at zero)
+//   parity[n]   depth parity of each subtree root (even lines)
+//   grp[n]      internal node -> its group (set-up only)
+//   goff[n]     group g's top slots are goff[g] .. goff[g+1]-1; the final top's 0 .. n_final-1
+//   groot[n]    group g's root node
 // An up link is (top slot of the parent) << 1 | (1 if the node is the parent's
 // second child), or -1 for the root.
 constexpr int kSubMax = 2048;   // leaves of one subtree (LDS of k_refit_sub)
@@ -243,8 +249,19 @@ constexpr int kSubGrid = 2048;
 // two fit on a CU and one's loads overlap the other's walk)
 constexpr int kSubHalf = 1024;
 constexpr int kSubHalfBS = 512;
-constexpr int kTopMax = 2048;   // top nodes with LDS state (LDS of k_refit_top)
-constexpr int kTopBS = 1024;
+constexpr int kTopMax = 2048;   // largest refit_top knob (top nodes with LDS state per workgroup)
+// The top part's workgroups: 256 threads and LDS state for 512 top nodes (38 KB),
+// so they start beside other streams' walks instead of waiting for a whole CU; a
+// group holds about kTopGroupLeaves top nodes, and slots past the LDS state count
+// arrivals in global memory.
+constexpr int kTopBS = 256;
+constexpr int kTopCap = 512;
+constexpr int kTopOneBS = 1024;   // the whole top part in one workgroup (state for kTopMax nodes)
+// a group of the top part spans at most kTopGroupLeaves subtrees' worth of
+// leaves (T2 = kTopGroupLeaves * T): about that many top nodes, well within one
+// workgroup's LDS state
+constexpr int kTopGroupLeaves = 64;
+constexpr int kTopGroupGrid = 256;   // workgroups of k_refit_groups (grid-stride over the groups)
 
 struct Sched {
   const int32_t* p;
@@ -261,15 +278,22 @@ struct Sched {
   // pop only grandchildren, so odd-depth lines are never read)
   __device__ bool even_lines() const { return p[4] != 0; }
   __device__ int sub_parity(int i) const { return p[8 + 8 * (size_t)n + i]; }   // depth of subtree root i, mod 2
+  __device__ int n_groups() const { return p[5]; }
+  __device__ int n_final() const { return p[6]; }
+  __device__ int goff(int g) const { return p[8 + 10 * (size_t)n + g]; }
+  __device__ int groot(int g) const { return p[8 + 11 * (size_t)n + g]; }
 };
 
-__global__ void k_sched_header(int32_t* sched, int T, int top_lds, int even_lines) {
+__global__ void k_sched_header(int32_t* sched, int T, int top_lds, int even_lines, int T2) {
   if (threadIdx.x == 0) {
     sched[0] = 0;
     sched[1] = 0;
     sched[2] = T;
     sched[3] = top_lds;
     sched[4] = even_lines;
+    sched[5] = 0;
+    sched[6] = 0;
+    sched[7] = T2;
   }
 }
 
@@ -297,14 +321,26 @@ __global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const in
                              const int32_t* __restrict__ rr, int32_t* __restrict__ sched) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= 2 * n - 1) return;
-  const int T = sched[2];
+  const int T = sched[2], T2 = sched[7];
   const size_t N = (size_t)n;
   const int sz = node_leaves(j, n, rl, rr);
+  if (j < n) sched[8 + 10 * N + j] = 0;          // group sizes, counted next
   if (j < n - 1) {
     int slot = -1;
     if (sz > T) {
-      slot = atomicAdd(&sched[1], 1);
-      sched[8 + 5 * N + slot] = j;
+      atomicAdd(&sched[1], 1);
+      // its group: the highest ancestor (or itself) over at most T2 leaves; none
+      // (the final top) when it has more
+      int r = -1;
+      if (sz <= T2) {
+        r = j;
+        for (int p = parent[r]; p >= 0 && node_leaves(p, n, rl, rr) <= T2; p = parent[r]) r = p;
+      }
+      sched[8 + 9 * N + j] = r;
+      if (r == j) {                                // a group root: its group id, kept in top_slot until k_sched_slots
+        slot = atomicAdd(&sched[5], 1);
+        sched[8 + 11 * N + slot] = j;
+      }
     }
     sched[8 + 4 * N + j] = slot;
     sched[8 + 7 * N + j] = 0;
@@ -315,6 +351,59 @@ __global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const in
     const int first = j < n - 1 ? rl[j] : j - (n - 1);
     reinterpret_cast<int4*>(sched + 8)[e] = make_int4(j, first, first + sz - 1, p);
   }
+}
+
+// top nodes: group ids (the group root's, -1 for the final top) and group sizes
+__global__ void k_sched_gcount(int n, const int32_t* __restrict__ rl, const int32_t* __restrict__ rr,
+                               int32_t* __restrict__ sched) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = (size_t)n;
+  if (j >= n - 1 || node_leaves(j, n, rl, rr) <= sched[2]) return;
+  const int r = sched[8 + 9 * N + j];
+  const int g = r < 0 ? -1 : sched[8 + 4 * N + r];
+  sched[8 + 9 * N + j] = g;
+  atomicAdd(g < 0 ? &sched[6] : &sched[8 + 10 * N + g], 1);
+}
+
+// group offsets: the final top's slots first, then each group's (exclusive scan
+// of the sizes in one workgroup); the slot fill counters (in top_up, which
+// k_sched_link overwrites) zeroed
+__global__ void __launch_bounds__(1024) k_sched_gscan(int n, int32_t* __restrict__ sched) {
+  __shared__ int s_sum[1024];
+  const size_t N = (size_t)n;
+  const int ng = sched[5];
+  int32_t* goff = sched + 8 + 10 * N;
+  int carry = sched[6];
+  for (int b = 0; b < ng; b += 1024) {
+    const int i = b + (int)threadIdx.x;
+    const int v = i < ng ? goff[i] : 0;
+    s_sum[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {        // inclusive Hillis-Steele scan
+      const int t = (int)threadIdx.x >= o ? s_sum[threadIdx.x - o] : 0;
+      __syncthreads();
+      s_sum[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < ng) goff[i] = carry + s_sum[threadIdx.x] - v;
+    carry += s_sum[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) goff[ng] = carry;
+  for (int g = threadIdx.x; g <= ng; g += 1024) sched[8 + 6 * N + g] = 0;
+}
+
+// top slots: the final top's and each group's contiguous (order within them free)
+__global__ void k_sched_slots(int n, const int32_t* __restrict__ rl, const int32_t* __restrict__ rr,
+                              int32_t* __restrict__ sched) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = (size_t)n;
+  if (j >= n - 1 || node_leaves(j, n, rl, rr) <= sched[2]) return;
+  const int g = sched[8 + 9 * N + j], ng = sched[5];
+  const int slot = g < 0 ? atomicAdd(&sched[8 + 6 * N + ng], 1)
+                         : sched[8 + 10 * N + g] + atomicAdd(&sched[8 + 6 * N + g], 1);
+  sched[8 + 4 * N + j] = slot;
+  sched[8 + 5 * N + slot] = j;
 }
 
 __device__ __forceinline__ int up_link(int j, int p, const int32_t* cb, const int32_t* sched, size_t N) {
@@ -583,46 +672,86 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
   }
 }
 
-// The top part in one workgroup.  Each top slot with LDS state has two child
-// cells; a finished node puts its box into its parent's cell, then arrives at the
-// parent.  Slots beyond the LDS capacity read their children from the node arrays
-// and count arrivals in global memory; one workgroup runs, so workgroup scope
-// orders both.
-template <bool COUNTED>
-__global__ void __launch_bounds__(kTopBS)
-k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
-            const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
-  __shared__ NodeBox s_cell[kTopMax][2];
-  __shared__ int s_flag[kTopMax];
-  __shared__ int s_up[kTopMax];
-  __shared__ int s_node[kTopMax], s_l[kTopMax], s_r[kTopMax];
-  const Sched S{sched, n};
-  const int n_sub = S.n_sub(), n_top = S.n_top();
-  const int nl = n_top < S.top_lds() ? n_top : S.top_lds();
-  for (int i = threadIdx.x; i < nl; i += kTopBS) {
+// The top part: one workgroup refits the top slots [s0, s1) (a group, or the
+// final top), the first top_lds of them with per-node state in LDS.  Each slot
+// with LDS state has two child cells; a finished node puts its box into its
+// parent's cell, then arrives at the parent.  The range's children outside it are
+// subtree roots and (final top) group roots, whose boxes the earlier kernels
+// stored.  Slots beyond the LDS capacity read their children from the node arrays
+// and count arrivals in global memory; only this workgroup touches the range's
+// slots, so workgroup scope orders both.  A node whose parent is outside the
+// range (a group's root) stores its box and ends the walk.
+// The 4-wide line of top node p (float4 j of 8: slot j >> 1, low or high half j & 1)
+// from the node arrays, once its grandchildren's boxes are final.
+__device__ __forceinline__ float4 top_line_entry(int n, int p, int j, const int32_t* __restrict__ ca,
+                                                 const int32_t* __restrict__ cb, const float4* nlo,
+                                                 const float4* nhi) {
+  const int slot = j >> 1, half = j & 1;
+  const int c = slot < 2 ? ca[p] : cb[p];
+  if (c >= n - 1 && (slot & 1)) {                     // a leaf child's companion slot: empty
+    const float inf = __int_as_float(0x7f800000);
+    return half ? make_float4(-inf, -inf, -inf, __int_as_float(-1)) : make_float4(inf, inf, inf, __int_as_float(-1));
+  }
+  const int g = c >= n - 1 ? c : __float_as_int((slot & 1) ? nhi[c].w : nlo[c].w);
+  const float4 lo = nlo[g];
+  if (half) {
+    const float4 hi = nhi[g];
+    return make_float4(hi.x, hi.y, hi.z, __int_as_float(g >= n - 1 ? __float_as_int(lo.w) : -1));
+  }
+  return make_float4(lo.x, lo.y, lo.z, __int_as_float(g));
+}
+
+template <bool COUNTED, int CAP, int BS>
+__device__ void refit_top_range(const Sched& S, int s0, int s1, bool final_top, const int32_t* __restrict__ ca,
+                                const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi,
+                                float4* quad) {
+  __shared__ NodeBox s_cell[CAP][2];
+  __shared__ int s_flag[CAP];
+  __shared__ int s_up[CAP];
+  __shared__ int s_node[CAP], s_l[CAP], s_r[CAP];
+  const int n_sub = S.n_sub(), n_groups = S.n_groups();
+  const int cnt = s1 - s0;
+  if (cnt <= 0) return;                                 // (the same for the whole workgroup)
+  const int cap = S.top_lds() < CAP ? S.top_lds() : CAP;
+  const int nl = cnt < cap ? cnt : cap;
+  auto inside = [&](int up) { return up >= 0 && (up >> 1) >= s0 && (up >> 1) < s1; };
+  for (int i = threadIdx.x; i < nl; i += BS) {
     s_flag[i] = 0;
-    s_up[i] = S.top_up(i);
-    const int p = S.top_node(i);
+    s_up[i] = S.top_up(s0 + i);
+    const int p = S.top_node(s0 + i);
     s_node[i] = p;
     s_l[i] = ca[p];
     s_r[i] = cb[p];
   }
-  for (int i = threadIdx.x; i < n_sub; i += kTopBS) {
-    const int4 e = S.sub(i);
-    if (e.w >= 0 && (e.w >> 1) < nl) s_cell[e.w >> 1][e.w & 1] = load_node<COUNTED>(e.x, num, nlo, nhi);
+  // the range's children below it: subtree roots, and group roots for the final top
+  const int n_kids = n_sub + (final_top ? n_groups : 0);
+  auto kid = [&](int i, int& node) -> int {
+    if (i < n_sub) {
+      const int4 e = S.sub(i);
+      node = e.x;
+      return e.w;
+    }
+    node = S.groot(i - n_sub);
+    return S.top_up(S.top_slot(node));
+  };
+  for (int i = threadIdx.x; i < n_kids; i += BS) {
+    int node;
+    const int up = kid(i, node);
+    if (inside(up) && (up >> 1) - s0 < nl) s_cell[(up >> 1) - s0][up & 1] = load_node<COUNTED>(node, num, nlo, nhi);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_sub; i += kTopBS) {
-    int up = S.sub(i).w;
-    while (up >= 0) {
-      const int slot = up >> 1;
+  for (int i = threadIdx.x; i < n_kids; i += BS) {
+    int node;
+    int up = kid(i, node);
+    while (inside(up)) {
+      const int slot = (up >> 1) - s0;
       const bool lds = slot < nl;
       int prev;
       if (lds) {
         __threadfence_block();
         prev = atomicAdd(&s_flag[slot], 1);
       } else {
-        prev = __hip_atomic_fetch_add(S.gflag(slot), 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        prev = __hip_atomic_fetch_add(S.gflag(s0 + slot), 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       if (prev == 0) break;
       int p, l, r;
@@ -635,69 +764,109 @@ k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
         A = s_cell[slot][0];
         B = s_cell[slot][1];
       } else {
-        __hip_atomic_store(S.gflag(slot), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        p = S.top_node(slot);
+        __hip_atomic_store(S.gflag(s0 + slot), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p = S.top_node(s0 + slot);
         l = ca[p];
         r = cb[p];
         A = load_node<COUNTED>(l, num, nlo, nhi);
         B = load_node<COUNTED>(r, num, nlo, nhi);
       }
       const NodeBox P = unite<COUNTED>(A, B);
-      up = lds ? s_up[slot] : S.top_up(slot);
-      const bool up_lds = up >= 0 && (up >> 1) < nl;
-      // a node whose parent reads the node arrays stores now; the others are
-      // stored after the walk (a store here would make every level's fence wait
-      // for it)
+      up = lds ? s_up[slot] : S.top_up(s0 + slot);
+      const bool up_lds = inside(up) && (up >> 1) - s0 < nl;
+      // a node whose parent reads the node arrays (or is outside the range)
+      // stores now; the others are stored after the walk (a store here would
+      // make every level's fence wait for it)
       if (!lds || !up_lds) store_node<COUNTED>(p, P, l, r, num, nlo, nhi);
-      if (up_lds) s_cell[up >> 1][up & 1] = P;
+      if (up_lds) s_cell[(up >> 1) - s0][up & 1] = P;
     }
   }
   __syncthreads();
   // the deferred nodes: both child cells are final, and uniting them again gives
   // the same box
-  for (int i = threadIdx.x; i < nl; i += kTopBS) {
+  for (int i = threadIdx.x; i < nl; i += BS) {
     const int up = s_up[i];
-    if (up >= 0 && (up >> 1) < nl)
+    if (inside(up) && (up >> 1) - s0 < nl)
       store_node<COUNTED>(s_node[i], unite<COUNTED>(s_cell[i][0], s_cell[i][1]), s_l[i], s_r[i], num, nlo, nhi);
+  }
+  // the range's 4-wide lines (triangle trees): every grandchild's box is final
+  // now -- in this range (stored above by this workgroup), a subtree root or its
+  // child, or a group root (earlier kernels)
+  if (!COUNTED && quad) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 8 * cnt; t += BS) {
+      const int p = S.top_node(s0 + (t >> 3));
+      quad[8 * (size_t)p + (t & 7)] = top_line_entry(S.n, p, t & 7, ca, cb, nlo, nhi);
+    }
   }
 }
 
-// The top nodes' 4-wide lines, once every box is final: 8 threads per line (one
-// float4 each), grid-stride over the top nodes (their count is on the device).
+// the groups of the top part, one workgroup each (grid-stride over the groups,
+// whose count is on the device)
+template <bool COUNTED>
+__global__ void __launch_bounds__(kTopBS)
+k_refit_groups(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
+               const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
+  const Sched S{sched, n};
+  for (int g = blockIdx.x; g < S.n_groups(); g += gridDim.x) {
+    refit_top_range<COUNTED, kTopCap, kTopBS>(S, S.goff(g), S.goff(g + 1), false, ca, cb, num, nlo, nhi, quad);
+    __syncthreads();
+  }
+}
+
+// the final top (the nodes over more than T2 leaves), in one workgroup
+template <bool COUNTED>
+__global__ void __launch_bounds__(kTopBS)
+k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
+            const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
+  const Sched S{sched, n};
+  refit_top_range<COUNTED, kTopCap, kTopBS>(S, 0, S.n_final(), true, ca, cb, num, nlo, nhi, quad);
+}
+
+// the whole top part in one workgroup with LDS state for up to kTopMax nodes: the
+// groups one after the other, then the final top (trees of at most kTopMax
+// subtrees, whose schedule makes the whole top part one group); its lines are
+// written by k_top_lines
+template <bool COUNTED>
+__global__ void __launch_bounds__(kTopOneBS)
+k_refit_top_one(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
+                const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
+  const Sched S{sched, n};
+  for (int g = 0; g < S.n_groups(); ++g) {
+    refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, S.goff(g), S.goff(g + 1), false, ca, cb, num, nlo, nhi, quad);
+    __syncthreads();
+  }
+  refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, 0, S.n_final(), true, ca, cb, num, nlo, nhi, quad);
+}
+
+// The top nodes' 4-wide lines after k_refit_top_one: 8 threads per line (one
+// float4 each), grid-stride over the top nodes (their count is on the device);
+// one workgroup writing them in turn would wait on their gathers one after another.
 __global__ void __launch_bounds__(256)
 k_top_lines(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
             const float4* __restrict__ nlo, const float4* __restrict__ nhi, float4* __restrict__ quad) {
   const Sched S{sched, n};
   const int n_top = S.n_top();
   for (int t = blockIdx.x * 256 + threadIdx.x; t < 8 * n_top; t += gridDim.x * 256) {
-    const int i = t >> 3, j = t & 7, slot = j >> 1, half = j & 1;
-    const int p = S.top_node(i);
-    const int c = slot < 2 ? ca[p] : cb[p];
-    float4 v;
-    if (c >= n - 1 && (slot & 1)) {                   // a leaf child's companion slot: empty
-      const float inf = __int_as_float(0x7f800000);
-      v = half ? make_float4(-inf, -inf, -inf, __int_as_float(-1)) : make_float4(inf, inf, inf, __int_as_float(-1));
-    } else {
-      const int g = c >= n - 1 ? c : __float_as_int((slot & 1) ? nhi[c].w : nlo[c].w);
-      const float4 lo = nlo[g];
-      if (half) {
-        const float4 hi = nhi[g];
-        v = make_float4(hi.x, hi.y, hi.z, __int_as_float(g >= n - 1 ? __float_as_int(lo.w) : -1));
-      } else {
-        v = make_float4(lo.x, lo.y, lo.z, __int_as_float(g));
-      }
-    }
-    quad[8 * (size_t)p + j] = v;
+    const int p = S.top_node(t >> 3);
+    quad[8 * (size_t)p + (t & 7)] = top_line_entry(n, p, t & 7, ca, cb, nlo, nhi);
   }
 }
 
-size_t refit_sched_ints(int n) { return 8 + 9 * (size_t)(n > 0 ? n : 1); }
+// one workgroup for the whole top part while the tree has at most kTopMax
+// subtrees' worth of leaves
+static inline bool top_in_one(int n) { return (long long)n <= (long long)kTopMax * knobs().refit_sub; }
+
+size_t refit_sched_ints(int n) { return 8 + 12 * (size_t)(n > 0 ? n : 1) + 8; }
 
 int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_t* rl, const int32_t* rr,
                    int32_t* sched, hipStream_t st, bool even_lines) {
   SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, knobs().refit_sub, knobs().refit_top,
-             even_lines ? 1 : 0);
+             even_lines ? 1 : 0, top_in_one(n) ? n : kTopGroupLeaves * knobs().refit_sub);
   SLX_LAUNCH("sched", k_sched_mark, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, st, n, parent, rl, rr, sched);
+  SLX_LAUNCH("sched", k_sched_gcount, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, rl, rr, sched);
+  SLX_LAUNCH("sched", k_sched_gscan, dim3(1), dim3(1024), 0, st, n, sched);
+  SLX_LAUNCH("sched", k_sched_slots, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, rl, rr, sched);
   SLX_LAUNCH("sched", k_sched_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cb, parent, sched);
   if (even_lines) SLX_LAUNCH("sched", k_sched_parity, dim3(1), dim3(1024), 0, st, n, parent, sched);
   return check_launch("refit_schedule");
@@ -720,10 +889,17 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
                parent, num, nlo, nhi);
   }
   if (after_sub && hipEventRecord(after_sub, st) != hipSuccess) return fail(SLX_EHIP, "refit: event record failed");
-  SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
-             src.quad);
-  if (!COUNTED && src.quad)
-    SLX_LAUNCH("refit_top", k_top_lines, dim3(64), dim3(256), 0, st, n, sched, ca, cb, nlo, nhi, src.quad);
+  if (top_in_one(n)) {
+    SLX_LAUNCH("refit_top", k_refit_top_one<COUNTED>, dim3(1), dim3(kTopOneBS), 0, st, n, sched, ca, cb, num, nlo,
+               nhi, nullptr);
+    if (!COUNTED && src.quad)
+      SLX_LAUNCH("refit_top", k_top_lines, dim3(64), dim3(256), 0, st, n, sched, ca, cb, nlo, nhi, src.quad);
+  } else {
+    SLX_LAUNCH("refit_groups", k_refit_groups<COUNTED>, dim3(kTopGroupGrid), dim3(kTopBS), 0, st, n, sched, ca, cb,
+               num, nlo, nhi, src.quad);
+    SLX_LAUNCH("refit_top", k_refit_top<COUNTED>, dim3(1), dim3(kTopBS), 0, st, n, sched, ca, cb, num, nlo, nhi,
+               src.quad);
+  }
   return check_launch(name);
 }
 
