@@ -235,7 +235,8 @@ __global__ void k_fill_i32(int n, int32_t* __restrict__ p, int32_t v) {
 //   top_slot[n] internal node -> top slot (-1 below the top part)
 //   top_node[n] top slot -> node
 //   top_up[n]   top slot -> up link
-//   gflag[n]    arrival counters of top slots without LDS state (left at zero)
+//   gflag[n]    arrival counters of top slots without LDS state (left at zero;
+//               while the schedule is made, the list of top nodes)
 //   parity[n]   depth parity of each subtree root (even lines)
 //   grp[n]      internal node -> its group (set-up only)
 //   goff[n]     group g's top slots are goff[g] .. goff[g+1]-1; the final top's 0 .. n_final-1
@@ -328,7 +329,7 @@ __global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const in
   if (j < n - 1) {
     int slot = -1;
     if (sz > T) {
-      atomicAdd(&sched[1], 1);
+      sched[8 + 7 * N + atomicAdd(&sched[1], 1)] = j;   // listed for k_sched_gcount / k_sched_slots
       // its group: the highest ancestor (or itself) over at most T2 leaves; none
       // (the final top) when it has more
       int r = -1;
@@ -343,7 +344,6 @@ __global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const in
       }
     }
     sched[8 + 4 * N + j] = slot;
-    sched[8 + 7 * N + j] = 0;
   }
   const int p = parent[j];
   if (sz <= T && (p < 0 || node_leaves(p, n, rl, rr) > T)) {
@@ -353,16 +353,17 @@ __global__ void k_sched_mark(int n, const int32_t* __restrict__ parent, const in
   }
 }
 
-// top nodes: group ids (the group root's, -1 for the final top) and group sizes
-__global__ void k_sched_gcount(int n, const int32_t* __restrict__ rl, const int32_t* __restrict__ rr,
-                               int32_t* __restrict__ sched) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// top nodes (from the list k_sched_mark made): group ids (the group root's, -1
+// for the final top) and group sizes
+__global__ void k_sched_gcount(int n, int32_t* __restrict__ sched) {
   const size_t N = (size_t)n;
-  if (j >= n - 1 || node_leaves(j, n, rl, rr) <= sched[2]) return;
-  const int r = sched[8 + 9 * N + j];
-  const int g = r < 0 ? -1 : sched[8 + 4 * N + r];
-  sched[8 + 9 * N + j] = g;
-  atomicAdd(g < 0 ? &sched[6] : &sched[8 + 10 * N + g], 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < sched[1]; i += gridDim.x * blockDim.x) {
+    const int j = sched[8 + 7 * N + i];
+    const int r = sched[8 + 9 * N + j];
+    const int g = r < 0 ? -1 : sched[8 + 4 * N + r];
+    sched[8 + 9 * N + j] = g;
+    atomicAdd(g < 0 ? &sched[6] : &sched[8 + 10 * N + g], 1);
+  }
 }
 
 // group offsets: the final top's slots first, then each group's (exclusive scan
@@ -393,17 +394,20 @@ __global__ void __launch_bounds__(1024) k_sched_gscan(int n, int32_t* __restrict
   for (int g = threadIdx.x; g <= ng; g += 1024) sched[8 + 6 * N + g] = 0;
 }
 
-// top slots: the final top's and each group's contiguous (order within them free)
-__global__ void k_sched_slots(int n, const int32_t* __restrict__ rl, const int32_t* __restrict__ rr,
-                              int32_t* __restrict__ sched) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// top slots: the final top's and each group's contiguous (order within them
+// free); the list's entries become the slots' arrival counters, zeroed (each
+// thread reads and clears only its own entry)
+__global__ void k_sched_slots(int n, int32_t* __restrict__ sched) {
   const size_t N = (size_t)n;
-  if (j >= n - 1 || node_leaves(j, n, rl, rr) <= sched[2]) return;
-  const int g = sched[8 + 9 * N + j], ng = sched[5];
-  const int slot = g < 0 ? atomicAdd(&sched[8 + 6 * N + ng], 1)
-                         : sched[8 + 10 * N + g] + atomicAdd(&sched[8 + 6 * N + g], 1);
-  sched[8 + 4 * N + j] = slot;
-  sched[8 + 5 * N + slot] = j;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < sched[1]; i += gridDim.x * blockDim.x) {
+    const int j = sched[8 + 7 * N + i];
+    sched[8 + 7 * N + i] = 0;
+    const int g = sched[8 + 9 * N + j], ng = sched[5];
+    const int slot = g < 0 ? atomicAdd(&sched[8 + 6 * N + ng], 1)
+                           : sched[8 + 10 * N + g] + atomicAdd(&sched[8 + 6 * N + g], 1);
+    sched[8 + 4 * N + j] = slot;
+    sched[8 + 5 * N + slot] = j;
+  }
 }
 
 __device__ __forceinline__ int up_link(int j, int p, const int32_t* cb, const int32_t* sched, size_t N) {
@@ -864,9 +868,10 @@ int refit_schedule(int n, const int32_t* cb, const int32_t* parent, const int32_
   SLX_LAUNCH("sched", k_sched_header, dim3(1), dim3(64), 0, st, sched, knobs().refit_sub, knobs().refit_top,
              even_lines ? 1 : 0, top_in_one(n) ? n : kTopGroupLeaves * knobs().refit_sub);
   SLX_LAUNCH("sched", k_sched_mark, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, st, n, parent, rl, rr, sched);
-  SLX_LAUNCH("sched", k_sched_gcount, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, rl, rr, sched);
+  const int lg = blocks_for(n, 256) < 256 ? blocks_for(n, 256) : 256;   // grid-stride over the top nodes
+  SLX_LAUNCH("sched", k_sched_gcount, dim3(lg), dim3(256), 0, st, n, sched);
   SLX_LAUNCH("sched", k_sched_gscan, dim3(1), dim3(1024), 0, st, n, sched);
-  SLX_LAUNCH("sched", k_sched_slots, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, rl, rr, sched);
+  SLX_LAUNCH("sched", k_sched_slots, dim3(lg), dim3(256), 0, st, n, sched);
   SLX_LAUNCH("sched", k_sched_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cb, parent, sched);
   if (even_lines) SLX_LAUNCH("sched", k_sched_parity, dim3(1), dim3(1024), 0, st, n, parent, sched);
   return check_launch("refit_schedule");
