@@ -87,6 +87,14 @@ static inline int32_t morton_in_box(const float* pos, const float* lo, const flo
   return orc_morton3d(t[0], t[1], t[2]);
 }
 
+/* Triangle trees: the box's largest extent on every axis (lo + e, the same float
+   operations as the library), so thin sheets keep their codes' bits for their width. */
+static inline int32_t morton_in_cube(const float* pos, const float* lo, const float* hi) {
+  const float e = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  const float hc[3] = {lo[0] + e, lo[1] + e, lo[2] + e};
+  return morton_in_box(pos, lo, hc);
+}
+
 static void cell_grid(const float* lo, const float* hi, float* cs, float* centers) {
   for (int c = 0; c < 3; ++c) cs[c] = (hi[c] - lo[c]) / (float)GRID;
   if (!centers) return;
@@ -324,7 +332,7 @@ void orc_tri_build(int nf, const float* fmin, const float* fmax, const float* lo
   for (int f = 0; f < nf; ++f) {
     float pos[3];
     for (int c = 0; c < 3; ++c) pos[c] = 0.5f * (fmin[3 * f + c] + fmax[3 * f + c]);
-    codes[f] = morton_in_box(pos, lo, hi);
+    codes[f] = morton_in_cube(pos, lo, hi);
     keys[f] = codes[f];
     sorted_faces[f] = f;
   }

@@ -168,7 +168,8 @@ int orc_cell_build(int nf, const float* fmin, const float* fmax, const float* lo
                    int32_t* prefix, int32_t* sorted_faces, int32_t* num_faces,
                    int32_t* leaf_l, int32_t* leaf_r, float* nmin, float* nmax);
 
-/* Per-triangle LBVH: Morton codes of face-box centres normalised by [lo,hi],
+/* Per-triangle LBVH: Morton codes of face-box centres normalised by the cube
+   [lo, lo + e] (e = [lo,hi]'s largest extent, the same scale on every axis),
    stable sort, Karras with index tie-break, leaves = sorted faces, refit. */
 void orc_tri_build(int nf, const float* fmin, const float* fmax, const float* lo,
                    const float* hi, int32_t* codes, int32_t* sorted_faces, int32_t* child_a,

@@ -332,8 +332,8 @@ class CellBVH:
 
 
 class TriBVH:
-    """Per-triangle LBVH: Morton sort of face-box centres, Karras with index
-    tie-break, refit."""
+    """Per-triangle LBVH: Morton sort of face-box centres (normalised by the cube
+    of the mesh box's largest extent), Karras with index tie-break, refit."""
 
     def __init__(self, faces, x, padding):
         self.kind = 1

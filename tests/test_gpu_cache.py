@@ -132,7 +132,8 @@ def test_cache_wavefront_walk_of_a_coarse_tree(slx, rng):
 def test_two_tier_walk_equals_traversal(slx, rng, depth):
     """The first-tier walk with a shallow stack, plus the full walk of the queries
     it could not serve, gives the traversal's lists, whatever the depth (large
-    boxes overflow the shallow stacks)."""
+    boxes overflow the shallow stacks: a 16-entry stack overflows for a few
+    hundred queries at padding 0.1 on this cloth)."""
     import torch
     A = importlib.import_module("starlab-physics_amd._abi")
     bp = importlib.import_module("starlab-physics_amd.broadphase")
@@ -140,7 +141,7 @@ def test_two_tier_walk_equals_traversal(slx, rng, depth):
     cloth = crumpled(slx, 48, seed=3)
     nv = cloth.num_verts
     tree = bp.DeviceTree(cloth, "tri")
-    pad = np.float32(0.04)
+    pad = np.float32(0.1)
     tree.build(float(pad))
     q = torch.zeros((nv, 4), dtype=torch.float32, device="cuda")
     q[:, :3] = T(cloth.positions() + rng.uniform(-0.02, 0.02, (nv, 3)).astype(np.float32))

@@ -100,14 +100,14 @@ def test_edge_traversal_bit_exact(slx, orc, mode, pad, cap):
 @pytest.mark.parametrize("depth", [16, 32])
 def test_edge_traversal_two_tier_bit_exact(slx, orc, depth):
     """The two-tier walk over edge boxes, in the Morton-rank edge order: a
-    shallow first-tier stack (16 entries overflow here), the overflowing edges
-    walked again in full; the oracle's lists."""
+    shallow first-tier stack (16 entries overflow here, for thousands of edges at
+    padding 0.5), the overflowing edges walked again in full; the oracle's lists."""
     import importlib
     import torch
     bp = importlib.import_module("starlab-physics_amd.broadphase")
     A = abi()
     cloth, _ = ee_scene(slx, n=40)
-    pad, cap = 0.3, 64
+    pad, cap = 0.5, 64
     tree = bp.DeviceTree(cloth, "tri")
     tree.build(pad)
     ne = cloth.num_edges

@@ -47,6 +47,10 @@ def test_frames_with_sewing_bit_exact(slx, rng, collision):
     pairs = np.stack([left[1:-1], left[1:-1] + 19], 1).astype(np.int32)
     sim.load_sewing_pairs(pairs=pairs)
     sim.enable_sewing = True
+    # the sewn tube folds onto itself: room for every contact delta a vertex gets
+    # (past the bucket capacity the library drops deltas and flags it; the oracle
+    # has no such limit)
+    sim.records_per_vertex = 512
     ref = tc.orc_cloth(cloth, sim.mesh_st, sim)
     ref.collision = collision
     ref.sewing = pairs
@@ -54,6 +58,7 @@ def test_frames_with_sewing_bit_exact(slx, rng, collision):
     for _ in range(6):
         sim.forward(10, 1)
         ref.forward(10, 1)
+    assert not sim.status() & 8, "a vertex's contact deltas exceeded the bucket capacity"
     np.testing.assert_array_equal(cloth.positions(), ref.x)
     gap = np.linalg.norm(ref.x[pairs[:, 0]] - ref.x[pairs[:, 1]], axis=1).mean()
     assert gap < 0.5 * gap0, "sewing must pull the edges together"

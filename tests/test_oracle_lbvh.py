@@ -225,3 +225,31 @@ def test_query_cell_filter_skips_out_of_grid(slx, orc):
     far = np.array([[100.0, 0.5, 0.0]], np.float32)
     cand, cnt, _ = t.query(far - 0.01, far + 0.01, 10)
     assert cnt[0] == 0
+
+
+def test_tri_codes_normalise_by_the_largest_extent(slx, orc):
+    """Triangle-tree codes: face-box centres normalised by one scale, the mesh box's
+    largest extent e, on every axis (a flat sheet keeps its bits for its width).
+    Known answers: the face at the far x end of a sheet 4x wider than tall codes
+    x = 1 and y within the lower quarter; and the codes equal an independent
+    numpy restatement for every face."""
+    x, faces = make_faces(slx, n=9)
+    x = x.astype(np.float64)
+    x[:, 1] = 0.25 * x[:, 0] * x[:, 0]          # a gentle slope: y spans less than x and z
+    x = x.astype(np.float32)
+    pad = np.float32(0.01)
+    t = orc.TriBVH(faces, x, pad)
+    fmin, fmax = t.fmin.astype(np.float32), t.fmax.astype(np.float32)
+    lo, hi = t.lo.astype(np.float32), t.hi.astype(np.float32)
+    e = np.float32(max(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]))
+    top = (lo + e).astype(np.float32)
+    c = (np.float32(0.5) * (fmin + fmax)).astype(np.float32)
+    u = np.clip((c - lo) / (top - lo), 0, 1).astype(np.float32)
+    want = np.array([orc.morton3d(*map(float, r)) for r in u], np.int64)
+    np.testing.assert_array_equal(t.codes.astype(np.int64), want)
+    # per-axis normalisation would have coded y over its own extent: different codes
+    u_axis = np.clip((c - lo) / (hi - lo), 0, 1).astype(np.float32)
+    per_axis = np.array([orc.morton3d(*map(float, r)) for r in u_axis], np.int64)
+    assert (per_axis != want).any()
+    # y bits stay below a quarter of the cube (the sheet is at most ~1/4 as tall as wide)
+    assert float(u[:, 1].max()) <= 0.3
