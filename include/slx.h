@@ -186,8 +186,10 @@ int slx_cell_build(int nf, const float* fmin4, const float* fmax4, const float* 
                    void* ws, size_t ws_bytes, slx_stream_t stream);
 
 /* Per-triangle LBVH (the reference's unused per-face path, lbvh_cell.py:153-194,
- * 237-247, 383-405, completed): 30-bit Morton codes of face-box centres in box6,
- * stable radix sort of (code, face id), Karras tree with an index tie-break for
+ * 237-247, 383-405, completed): 30-bit Morton codes of face-box centres in the
+ * cube [lo, lo + e] of box6 (e = its largest extent, one scale on every axis, so
+ * a thin sheet keeps its bits for its width), stable radix sort of (code, face
+ * id), Karras tree with an index tie-break for
  * equal codes, refit.  codes/sorted_faces [nf], child_a/child_b [nf-1], parent
  * [2nf-1], nlo4/nhi4 float4 [2nf-1]. */
 size_t slx_tri_build_ws_bytes(int nf);
