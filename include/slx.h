@@ -164,7 +164,7 @@ int slx_cell_topology(int32_t* codes, int32_t* sorted_ids, int32_t* child_a, int
  * above them: their maximal subtrees of at most 64 x sub_leaves leaves one
  * workgroup each, then the rest in one workgroup, so no device-wide fence is
  * needed.  The boxes are the same.  slx_set_refit_shape sets sub_leaves (1..2048,
- * default 2048) and how many top nodes of one workgroup keep their state in LDS
+ * default 1024) and how many top nodes of one workgroup keep their state in LDS
  * (0..2048, default 2048; at most 512 are kept, the others count arrivals in
  * global memory) for schedules made afterwards. */
 size_t slx_refit_sched_ints(int n_leaves);

@@ -31,6 +31,9 @@ int exclusive_scan(int n_max, const int* n_dev, const int* in, int* out, int* to
 #ifndef SLX_TRAV_QUAD
 #define SLX_TRAV_QUAD 1   // 4-wide node walk when the tree has its quad array (0: binary walk)
 #endif
+#ifndef SLX_TRAV_SCALAR
+#define SLX_TRAV_SCALAR 1 // a line every lane pops is read with scalar loads (0: vector loads only)
+#endif
 #ifndef SLX_TRAV_STACK
 #define SLX_TRAV_STACK 64
 #endif
@@ -263,6 +266,9 @@ constexpr int kSubGrid = 2048;
 // two fit on a CU and one's loads overlap the other's walk)
 constexpr int kSubHalf = 1024;
 constexpr int kSubHalfBS = 512;
+// and subtrees of at most 512 leaves in a quarter-size one (39 KB: four per CU)
+constexpr int kSubQuarter = 512;
+constexpr int kSubQuarterBS = 256;
 constexpr int kTopMax = 2048;   // largest refit_top knob (top nodes with LDS state per workgroup)
 // The top part's workgroups: 256 threads and LDS state for 512 top nodes (38 KB),
 // so they start beside other streams' walks instead of waiting for a whole CU; a
@@ -570,15 +576,19 @@ __device__ __forceinline__ NodeBox leaf_box(int k, int n, const LeafSrc& s, int3
   return b;
 }
 
-// One workgroup per subtree.  Phase 1 loads, coalesced, the leaves' boxes and
-// parents and the subtree's internal nodes' children and parents (all contiguous
-// index ranges) into LDS; phase 2 walks up from every leaf, the second child to
-// arrive at a node refitting it, touching global memory only to store boxes.
+// One workgroup per subtree, two leaves per thread (leaves k and k + BS).  Phase 1
+// loads, coalesced, the leaves' boxes and parents and the subtree's internal
+// nodes' children and parents (all contiguous index ranges) into LDS; phase 2 walks
+// up from every leaf, the second child to arrive at a node refitting it, touching
+// global memory only to store boxes; phase 3 writes the lines.  Each phase keeps a
+// thread's independent chains in flight together: the subtree's phases run one
+// after another on its CU, so their latency, not bandwidth, sets the time.
 template <int LEAF, int SUBMAX, int BS>
 __global__ void __launch_bounds__(BS)
 k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t* __restrict__ ca,
             const int32_t* __restrict__ cb, const int32_t* __restrict__ parent, int32_t* num, float4* nlo,
             float4* nhi) {
+  static_assert(SUBMAX == 2 * BS, "two leaves per thread");
   constexpr bool COUNTED = LEAF == kLeafCells;
   __shared__ NodeBox s_leaf[SUBMAX];
   __shared__ NodeBox s_int[SUBMAX];
@@ -586,16 +596,30 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
   __shared__ int s_ca[SUBMAX], s_cb[SUBMAX], s_par[SUBMAX], s_lface[SUBMAX];
   const Sched S{sched, n};
   const int n_sub = S.n_sub();
+  const int tid = (int)threadIdx.x;
   for (int si = blockIdx.x; si < n_sub; si += gridDim.x) {
     const int4 e = S.sub(si);
     const int root = e.x, first = e.y, cnt = e.z - e.y + 1;
+    int lpar[2] = {-1, -1};                 // the parents of this thread's leaves
+    auto load_links = [&](int k) {          // internal node first+k (those outside the subtree are never read)
+      if (k < cnt && first + k < n - 1) {
+        s_ca[k] = ca[first + k];
+        s_cb[k] = cb[first + k];
+        s_par[k] = parent[first + k];
+      }
+    };
     if constexpr (LEAF == kLeafVertices) {
-      // two leaves per thread, every load issued before any is used (each leaf is
-      // three dependent gathers: face id, corner ids, corner positions)
-      for (int k = threadIdx.x; k < cnt; k += 2 * BS) {
-        const int k2 = k + BS;
+      // every load issued before any is used (a leaf is three dependent gathers:
+      // face id, corner ids, corner positions; its parent and the internal nodes'
+      // links are independent of them)
+      if (tid < cnt) {
+        const int k2 = tid + BS;
         const bool two = k2 < cnt;
-        const int f1 = src.sorted[first + k], f2 = two ? src.sorted[first + k2] : f1;
+        const int f1 = src.sorted[first + tid], f2 = two ? src.sorted[first + k2] : f1;
+        lpar[0] = parent[n - 1 + first + tid];
+        if (two) lpar[1] = parent[n - 1 + first + k2];
+        load_links(tid);
+        load_links(k2);
         const int a1 = src.faces[3 * f1], b1 = src.faces[3 * f1 + 1], c1 = src.faces[3 * f1 + 2];
         const int a2 = src.faces[3 * f2], b2 = src.faces[3 * f2 + 1], c2 = src.faces[3 * f2 + 2];
         const float4 pa1 = src.x4[a1], pb1 = src.x4[b1], pc1 = src.x4[c1];
@@ -604,84 +628,132 @@ k_refit_sub(int n, const int32_t* __restrict__ sched, LeafSrc src, const int32_t
         corner_box(xyz(pa1), xyz(pb1), xyz(pc1), src.pad, B1.lo, B1.hi);
         corner_box(xyz(pa2), xyz(pb2), xyz(pc2), src.pad, B2.lo, B2.hi);
         B1.num = B2.num = 1;
-        s_leaf[k] = B1;
-        s_lface[k] = f1;
-        store_node<false>(n - 1 + first + k, B1, f1, f1, num, nlo, nhi);
+        s_leaf[tid] = B1;
+        s_lface[tid] = f1;
+        store_node<false>(n - 1 + first + tid, B1, f1, f1, num, nlo, nhi);
         if (two) {
           s_leaf[k2] = B2;
           s_lface[k2] = f2;
           store_node<false>(n - 1 + first + k2, B2, f2, f2, num, nlo, nhi);
         }
       }
-    }
-    for (int k = threadIdx.x; k < cnt; k += BS) {
-      if constexpr (LEAF != kLeafVertices) {
+      s_flag[tid] = 0;
+      s_flag[tid + BS] = 0;
+    } else {
+      for (int i = 0; i < 2; ++i) {
+        const int k = tid + i * BS;
+        s_flag[k] = 0;
+        if (k >= cnt) continue;
+        lpar[i] = parent[n - 1 + first + k];
         s_leaf[k] = leaf_box<LEAF>(first + k, n, src, num, nlo, nhi);
         if constexpr (!COUNTED) s_lface[k] = src.sorted[first + k];
-      }
-      s_flag[k] = 0;
-      if (first + k < n - 1) {          // internal node first+k (those outside the subtree are never read)
-        s_ca[k] = ca[first + k];
-        s_cb[k] = cb[first + k];
-        s_par[k] = parent[first + k];
+        load_links(k);
       }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < cnt; k += BS) {
-      int node = n - 1 + first + k;
-      if (node == root) continue;
-      int p = parent[node];
-      while (node != root) {               // p is inside the subtree: first <= p <= last
+    // phase 2: both leaves' walks advance together
+    {
+      int node[2], p[2];
+      bool act[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        node[i] = n - 1 + first + tid + i * BS;
+        act[i] = tid + i * BS < cnt && node[i] != root;
+        p[i] = lpar[i];                       // inside the subtree while node != root
+      }
+      while (act[0] || act[1]) {
+        int old[2] = {1, 1};
         __threadfence_block();
-        if (atomicAdd(&s_flag[p - first], 1) == 0) break;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          if (act[i]) old[i] = atomicAdd(&s_flag[p[i] - first], 1);
         __threadfence_block();
-        const int l = s_ca[p - first], r = s_cb[p - first];
-        const NodeBox A = l >= n - 1 ? s_leaf[l - (n - 1) - first] : s_int[l - first];
-        const NodeBox B = r >= n - 1 ? s_leaf[r - (n - 1) - first] : s_int[r - first];
-        s_int[p - first] = unite<COUNTED>(A, B);
-        node = p;
-        p = s_par[p - first];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (!act[i]) continue;
+          if (old[i] == 0) {                  // the sibling's walk refits p
+            act[i] = false;
+            continue;
+          }
+          const int q = p[i] - first;
+          const int l = s_ca[q], r = s_cb[q];
+          const NodeBox A = l >= n - 1 ? s_leaf[l - (n - 1) - first] : s_int[l - first];
+          const NodeBox B = r >= n - 1 ? s_leaf[r - (n - 1) - first] : s_int[r - first];
+          s_int[q] = unite<COUNTED>(A, B);
+          node[i] = p[i];
+          p[i] = s_par[q];
+          act[i] = node[i] != root;
+        }
       }
     }
     __syncthreads();
     // the subtree's internal nodes (both children arrived: flag 2) to global memory,
-    // coalesced: node boxes one node per thread, quad lines 8 threads per line
-    for (int k = threadIdx.x; k < cnt; k += BS)
+    // coalesced: node boxes one node per thread, quad lines four threads per line
+    for (int k = tid; k < cnt; k += BS)
       if (s_flag[k] == 2 && (!src.lines_only || first + k == root || s_par[k] == root))
         store_node<COUNTED>(first + k, s_int[k], s_ca[k], s_cb[k], num, nlo, nhi);
     if constexpr (!COUNTED) {
       if (src.quad && S.even_lines()) {      // odd-depth nodes' lines are never read: flag 2 -> 1
         const int rp = S.sub_parity(si);
-        for (int k = threadIdx.x; k < cnt; k += BS) {
-          if (s_flag[k] != 2) continue;
-          int x = first + k, d = rp;
-          while (x != root) {
-            x = s_par[x - first];
-            ++d;
-          }
-          if (d & 1) s_flag[k] = 1;
+        int x[2], d[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int k = tid + i * BS;
+          x[i] = (k < cnt && s_flag[k] == 2) ? first + k : root;
+          d[i] = rp;
+        }
+        while (x[0] != root || x[1] != root) {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+            if (x[i] != root) {
+              x[i] = s_par[x[i] - first];
+              ++d[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int k = tid + i * BS;
+          if (k < cnt && (d[i] & 1) && s_flag[k] == 2) s_flag[k] = 1;
         }
         __syncthreads();
       }
       if (src.quad) {
-        auto box_of = [&](int g) -> const NodeBox& {
-          return g >= n - 1 ? s_leaf[g - (n - 1) - first] : s_int[g - first];
-        };
-        for (int t = threadIdx.x; t < 8 * cnt; t += BS) {
-          const int k = t >> 3, j = t & 7;
-          if (s_flag[k] != 2) continue;
-          const int slot = j >> 1, half = j & 1;
-          const int c = slot < 2 ? s_ca[k] : s_cb[k];
-          float4 v;
-          if (c >= n - 1 && (slot & 1)) {            // a leaf child's companion slot: empty
-            const float inf = __int_as_float(0x7f800000);
-            v = half ? make_float4(-inf, -inf, -inf, __int_as_float(-1)) : make_float4(inf, inf, inf, __int_as_float(-1));
-          } else {
-            const int g = c >= n - 1 ? c : ((slot & 1) ? s_cb[c - first] : s_ca[c - first]);
-            const NodeBox& B = box_of(g);
-            v = half ? slot_hi(B, g >= n - 1 ? s_lface[g - (n - 1) - first] : -1) : slot_lo(B, g);
+        // slot (k, j) of item t = 4k + j: lo and hi of grandchild j of node first+k;
+        // a thread's eight items are read together, then stored
+        constexpr int IT = 4 * SUBMAX / BS;
+        const float inf = __int_as_float(0x7f800000);
+        int g[IT];
+        bool ok[IT], empty[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          const int t = tid + i * BS, k = t >> 2, j = t & 3;
+          const int kc = t < 4 * cnt ? k : 0;
+          const int c = j < 2 ? s_ca[kc] : s_cb[kc];
+          ok[i] = t < 4 * cnt && s_flag[kc] == 2;
+          empty[i] = c >= n - 1 && (j & 1);   // a leaf child's companion slot
+          g[i] = c;
+          if (ok[i] && c < n - 1) {
+            const int cc = c - first;
+            g[i] = (j & 1) ? s_cb[cc] : s_ca[cc];
           }
-          src.quad[8 * (size_t)(first + k) + j] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          if (!ok[i]) continue;
+          const int t = tid + i * BS, k = t >> 2, j = t & 3;
+          float4 lo, hi;
+          if (empty[i]) {
+            lo = make_float4(inf, inf, inf, __int_as_float(-1));
+            hi = make_float4(-inf, -inf, -inf, __int_as_float(-1));
+          } else {
+            const bool leaf = g[i] >= n - 1;
+            const NodeBox& B = leaf ? s_leaf[g[i] - (n - 1) - first] : s_int[g[i] - first];
+            lo = slot_lo(B, g[i]);
+            hi = slot_hi(B, leaf ? s_lface[g[i] - (n - 1) - first] : -1);
+          }
+          float4* dst = src.quad + 8 * (size_t)(first + k) + 2 * j;
+          dst[0] = lo;
+          dst[1] = hi;
         }
       }
     }
@@ -719,14 +791,13 @@ __device__ __forceinline__ float4 top_line_entry(int n, int p, int j, const int3
 }
 
 template <bool COUNTED, int CAP, int BS>
-__device__ void refit_top_range(const Sched& S, int s0, int s1, bool final_top, const int32_t* __restrict__ ca,
+__device__ void refit_top_range(const Sched& S, int s0, int s1, const int32_t* __restrict__ ca,
                                 const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi,
                                 float4* quad) {
   __shared__ NodeBox s_cell[CAP][2];
   __shared__ int s_flag[CAP];
   __shared__ int s_up[CAP];
   __shared__ int s_node[CAP], s_l[CAP], s_r[CAP];
-  const int n_sub = S.n_sub(), n_groups = S.n_groups();
   const int cnt = s1 - s0;
   if (cnt <= 0) return;                                 // (the same for the whole workgroup)
   const int cap = S.top_lds() < CAP ? S.top_lds() : CAP;
@@ -740,26 +811,37 @@ __device__ void refit_top_range(const Sched& S, int s0, int s1, bool final_top, 
     s_l[i] = ca[p];
     s_r[i] = cb[p];
   }
-  // the range's children below it: subtree roots, and group roots for the final top
-  const int n_kids = n_sub + (final_top ? n_groups : 0);
-  auto kid = [&](int i, int& node) -> int {
-    if (i < n_sub) {
-      const int4 e = S.sub(i);
-      node = e.x;
-      return e.w;
-    }
-    node = S.groot(i - n_sub);
-    return S.top_up(S.top_slot(node));
-  };
-  for (int i = threadIdx.x; i < n_kids; i += BS) {
-    int node;
-    const int up = kid(i, node);
-    if (inside(up) && (up >> 1) - s0 < nl) s_cell[(up >> 1) - s0][up & 1] = load_node<COUNTED>(node, num, nlo, nhi);
+  // the range's children below it (subtree roots, and group roots for the final
+  // top): the children of its nodes that are not its nodes; child b of slot i has
+  // up link (s0 + i) << 1 | b.  A slot with LDS state has bit 8 + b of its arrival
+  // counter set when child b is in the range (arrivals count in the low byte).
+  __syncthreads();
+  for (int i = threadIdx.x; i < cnt; i += BS) {
+    const int up = i < nl ? s_up[i] : S.top_up(s0 + i);
+    if (inside(up) && (up >> 1) - s0 < nl) atomicOr(&s_flag[(up >> 1) - s0], 256 << (up & 1));
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_kids; i += BS) {
+  auto kid = [&](int i, int b, int& node) -> bool {
+    if (i < nl) {
+      node = b ? s_r[i] : s_l[i];
+      // (bits 8-9 stay fixed while walks add arrivals to the low byte)
+      return !(__hip_atomic_load(&s_flag[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & (256 << b));
+    }
+    const int c = b ? cb[S.top_node(s0 + i)] : ca[S.top_node(s0 + i)];
+    node = c;
+    if (c >= S.n - 1) return true;                      // a leaf
+    const int ts = S.top_slot(c);
+    return ts < s0 || ts >= s1;
+  };
+  for (int t = threadIdx.x; t < 2 * nl; t += BS) {
     int node;
-    int up = kid(i, node);
+    if (kid(t >> 1, t & 1, node)) s_cell[t >> 1][t & 1] = load_node<COUNTED>(node, num, nlo, nhi);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 2 * cnt; t += BS) {
+    int node;
+    if (!kid(t >> 1, t & 1, node)) continue;
+    int up = ((s0 + (t >> 1)) << 1) | (t & 1);
     while (inside(up)) {
       const int slot = (up >> 1) - s0;
       const bool lds = slot < nl;
@@ -770,7 +852,7 @@ __device__ void refit_top_range(const Sched& S, int s0, int s1, bool final_top, 
       } else {
         prev = __hip_atomic_fetch_add(S.gflag(s0 + slot), 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      if (prev == 0) break;
+      if ((lds ? (prev & 255) : prev) == 0) break;
       int p, l, r;
       NodeBox A, B;
       if (lds) {
@@ -826,7 +908,7 @@ k_refit_groups(int n, const int32_t* __restrict__ sched, const int32_t* __restri
                const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
   const Sched S{sched, n};
   for (int g = blockIdx.x; g < S.n_groups(); g += gridDim.x) {
-    refit_top_range<COUNTED, kTopCap, kTopBS>(S, S.goff(g), S.goff(g + 1), false, ca, cb, num, nlo, nhi, quad);
+    refit_top_range<COUNTED, kTopCap, kTopBS>(S, S.goff(g), S.goff(g + 1), ca, cb, num, nlo, nhi, quad);
     __syncthreads();
   }
 }
@@ -837,7 +919,7 @@ __global__ void __launch_bounds__(kTopBS)
 k_refit_top(int n, const int32_t* __restrict__ sched, const int32_t* __restrict__ ca,
             const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
   const Sched S{sched, n};
-  refit_top_range<COUNTED, kTopCap, kTopBS>(S, 0, S.n_final(), true, ca, cb, num, nlo, nhi, quad);
+  refit_top_range<COUNTED, kTopCap, kTopBS>(S, 0, S.n_final(), ca, cb, num, nlo, nhi, quad);
 }
 
 // the whole top part in one workgroup with LDS state for up to kTopMax nodes: the
@@ -850,10 +932,10 @@ k_refit_top_one(int n, const int32_t* __restrict__ sched, const int32_t* __restr
                 const int32_t* __restrict__ cb, int32_t* num, float4* nlo, float4* nhi, float4* quad) {
   const Sched S{sched, n};
   for (int g = 0; g < S.n_groups(); ++g) {
-    refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, S.goff(g), S.goff(g + 1), false, ca, cb, num, nlo, nhi, quad);
+    refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, S.goff(g), S.goff(g + 1), ca, cb, num, nlo, nhi, quad);
     __syncthreads();
   }
-  refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, 0, S.n_final(), true, ca, cb, num, nlo, nhi, quad);
+  refit_top_range<COUNTED, kTopMax, kTopOneBS>(S, 0, S.n_final(), ca, cb, num, nlo, nhi, quad);
 }
 
 // The top nodes' 4-wide lines after k_refit_top_one: 8 threads per line (one
@@ -870,9 +952,12 @@ k_top_lines(int n, const int32_t* __restrict__ sched, const int32_t* __restrict_
   }
 }
 
-// one workgroup for the whole top part while the tree has at most kTopMax
-// subtrees' worth of leaves
-static inline bool top_in_one(int n) { return (long long)n <= (long long)kTopMax * knobs().refit_sub; }
+// One workgroup takes the whole top part when it is likely to have LDS state for
+// all its nodes: subtrees of Morton-ordered meshes hold more than half their limit
+// on average, so a tree of n leaves has fewer than 2n / refit_sub of them (a
+// larger top part is still refit correctly, with arrivals past the LDS state
+// counted in global memory, only more slowly).
+static inline bool top_in_one(int n) { return 2LL * n <= (long long)kTopMax * knobs().refit_sub; }
 
 size_t refit_sched_ints(int n) { return 8 + 12 * (size_t)(n > 0 ? n : 1) + 8; }
 
@@ -897,7 +982,11 @@ int refit(const char* name, int n, const int32_t* sched, const LeafSrc& src, con
           const int32_t* parent, int32_t* num, float4* nlo, float4* nhi, hipStream_t st,
           hipEvent_t after_sub = nullptr) {
   constexpr bool COUNTED = LEAF == kLeafCells;
-  if (knobs().refit_sub <= kSubHalf) {
+  if (knobs().refit_sub <= kSubQuarter) {
+    const int grid = n < 4 * kSubGrid ? n : 4 * kSubGrid;
+    SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubQuarter, kSubQuarterBS>), dim3(grid), dim3(kSubQuarterBS), 0, st, n,
+               sched, src, ca, cb, parent, num, nlo, nhi);
+  } else if (knobs().refit_sub <= kSubHalf) {
     const int grid = n < 2 * kSubGrid ? n : 2 * kSubGrid;
     SLX_LAUNCH(name, (k_refit_sub<LEAF, kSubHalf, kSubHalfBS>), dim3(grid), dim3(kSubHalfBS), 0, st, n, sched, src,
                ca, cb, parent, num, nlo, nhi);
@@ -1797,37 +1886,60 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
       keep((int)(e & kLow30));
       continue;
     }
-    const float4* qn = quad + 8 * (size_t)e;
-    float4 lo[4], hi[4];
+    // The four slots of a popped line: overlapping ones pushed in slot order.
+    auto visit = [&](const float4 (&lo)[4], const float4 (&hi)[4]) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      lo[k] = qn[2 * k];
-      hi[k] = qn[2 * k + 1];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (!overlap(aw, bw, xyz(lo[k]), xyz(hi[k]))) continue;
-      const int face = __float_as_int(hi[k].w), node = __float_as_int(lo[k].w);
-      if (BUILD && face >= 0) {                   // node word under the face word: popped face first
-        const bool exact = overlap(a, b, xyz(lo[k]), xyz(hi[k]));
-        if (own.has(face)) {                      // not stored: only counted, when in the exact list
-          if (!exact) continue;
-          if (sp < S) stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | kExactBit;
-          else flags |= kStatusStackOverflow;
+      for (int k = 0; k < 4; ++k) {
+        if (!overlap(aw, bw, xyz(lo[k]), xyz(hi[k]))) continue;
+        const int face = __float_as_int(hi[k].w), node = __float_as_int(lo[k].w);
+        if (BUILD && face >= 0) {                 // node word under the face word: popped face first
+          const bool exact = overlap(a, b, xyz(lo[k]), xyz(hi[k]));
+          if (own.has(face)) {                    // not stored: only counted, when in the exact list
+            if (!exact) continue;
+            if (sp < S) stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | kExactBit;
+            else flags |= kStatusStackOverflow;
+            continue;
+          }
+          if (sp + 1 < S) {
+            stk[(sp++) * kTravBS] = (uint32_t)node | kCacheNode;
+            stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | (exact ? kExactBit : 0u);
+          } else {
+            flags |= kStatusStackOverflow;
+          }
           continue;
         }
-        if (sp + 1 < S) {
-          stk[(sp++) * kTravBS] = (uint32_t)node | kCacheNode;
-          stk[(sp++) * kTravBS] = (uint32_t)face | kEmit | (exact ? kExactBit : 0u);
-        } else {
-          flags |= kStatusStackOverflow;
-        }
-        continue;
+        const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)node;
+        if (sp < S) stk[(sp++) * kTravBS] = w;
+        else flags |= kStatusStackOverflow;
       }
-      const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)node;
-      if (sp < S) stk[(sp++) * kTravBS] = w;
-      else flags |= kStatusStackOverflow;
+    };
+    float4 lo[4], hi[4];
+    // Lanes of a wavefront hold nearby queries and pop the same line about a
+    // third of the time (the top of the tree): the line then comes through the
+    // scalar cache, sparing the vector memory pipeline its eight loads per lane
+    // (it is then moved to vector registers: testing the slots from scalar
+    // registers, with the slot code duplicated for this branch, was slower).
+    const uint32_t e0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e);
+    if (SLX_TRAV_SCALAR && __ballot(e != e0) == 0ull) {
+      // the constant address space makes a wave-uniform load a scalar one
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const __attribute__((address_space(4))) f4* qs =
+          (const __attribute__((address_space(4))) f4*)(quad + 8 * (size_t)e0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const f4 l = qs[2 * k], h = qs[2 * k + 1];
+        lo[k] = make_float4(l.x, l.y, l.z, l.w);
+        hi[k] = make_float4(h.x, h.y, h.z, h.w);
+      }
+    } else {
+      const float4* qn = quad + 8 * (size_t)e;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        lo[k] = qn[2 * k];
+        hi[k] = qn[2 * k + 1];
+      }
     }
+    visit(lo, hi);
     if (co.redo && (flags & kStatusStackOverflow)) break;   // walked again by the second tier
   }
   if (BUILD) {

@@ -116,10 +116,12 @@ def test_tree_build_bit_exact(slx, orc, mode):
         np.testing.assert_array_equal(gt.parent[:nn].cpu().numpy(), ot.parent)
 
 
-# (leaves per workgroup subtree, top nodes with LDS state): the default, several
-# subtrees with part of the top part in global memory, all of it in global memory,
-# and one leaf per subtree (every internal node in the top part)
-REFIT_SHAPES = [(2048, 2048), (64, 16), (7, 0), (1, 3)]
+# (leaves per workgroup subtree, top nodes with LDS state): full-size subtree
+# workgroups, the default (half-size ones), several subtrees with part of the top
+# part in global memory, all of it in global memory, one leaf per subtree (every
+# internal node in the top part), and a top part in groups with LDS state for all
+# of it (4 leaves per subtree: 2n > 2048 * 4 on this mesh)
+REFIT_SHAPES = [(2048, 2048), (1024, 2048), (64, 16), (7, 0), (1, 3), (4, 512)]
 
 
 @pytest.mark.parametrize("shape", REFIT_SHAPES)
@@ -143,7 +145,7 @@ def test_tri_refit_bit_exact(slx, orc, shape):
             np.testing.assert_array_equal(w[:mesh.num_faces - 1], ot.child_a)
             np.testing.assert_array_equal(w[mesh.num_faces - 1:], ot.sorted_faces)
     finally:
-        A.call("slx_set_refit_shape", 2048, 2048)
+        A.call("slx_set_refit_shape", 1024, 2048)   # the library default
 
 
 @pytest.mark.parametrize("shape", [(64, 16), (5, 0)])
@@ -160,7 +162,7 @@ def test_cell_build_with_refit_shapes(slx, orc, shape):
         A.call("slx_refit_schedule", NC, A.ptr(topo["child_b"]), A.ptr(topo["parent"]), A.ptr(topo["range_l"]),
                A.ptr(topo["range_r"]), A.ptr(sched), A.stream_handle())
     finally:
-        A.call("slx_set_refit_shape", 2048, 2048)
+        A.call("slx_set_refit_shape", 1024, 2048)   # the library default
     mesh = crumpled(slx, 48)
     gt = bp.DeviceTree(mesh, "cell")
     gt.sched = sched
@@ -171,27 +173,39 @@ def test_cell_build_with_refit_shapes(slx, orc, shape):
     np.testing.assert_array_equal(gt.nhi4[:, :3].cpu().numpy(), ot.nmax)
 
 
-def test_tri_refit_full_size(slx, orc):
-    """The 1023x1023-quad cloth (2093058 faces): the default schedule's top part
-    is about 1600 nodes, all with LDS state."""
+@pytest.mark.parametrize("sub", [1024, 2048])
+def test_tri_refit_full_size(slx, orc, sub):
+    """The 1023x1023-quad cloth (2093058 faces): with the default 1024-leaf
+    subtrees the top part (about 3200 nodes) is refit in groups, one workgroup
+    each, then the final top; with 2048-leaf subtrees it is about 1600 nodes, all
+    in one workgroup with LDS state."""
     import importlib
     import torch
     bp = importlib.import_module("starlab-physics_amd.broadphase")
-    g = slx.scene.grid_cloth(1024, size=1.0, height=0.12, jitter=1e-3, seed=0)
-    mesh = slx.Mesh(x=g.x, faces=g.faces)
-    gt = bp.DeviceTree(mesh, "tri")
-    gt.build(2.5e-4)
-    ot = orc.TriBVH(mesh.fid_np, mesh.positions(), 2.5e-4)
-    nn = 2 * mesh.num_faces - 1
-    np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin)
-    np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax)
-    mesh.x4[:, :3] += torch.randn_like(mesh.x4[:, :3]) * 1e-4
-    gt.refit(2.5e-4)
-    ot.refit(mesh.positions())
-    np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin)
-    np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax)
-    hdr = gt.sched[:4].cpu().numpy()
-    assert hdr[2] == 2048 and 0 < hdr[1] <= 2048, hdr
+    A = abi()
+    A.call("slx_set_refit_shape", sub, 2048)
+    try:
+        g = slx.scene.grid_cloth(1024, size=1.0, height=0.12, jitter=1e-3, seed=0)
+        mesh = slx.Mesh(x=g.x, faces=g.faces)
+        gt = bp.DeviceTree(mesh, "tri")
+        gt.build(2.5e-4)
+        ot = orc.TriBVH(mesh.fid_np, mesh.positions(), 2.5e-4)
+        nn = 2 * mesh.num_faces - 1
+        np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin)
+        np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax)
+        mesh.x4[:, :3] += torch.randn_like(mesh.x4[:, :3]) * 1e-4
+        gt.refit(2.5e-4)
+        ot.refit(mesh.positions())
+        np.testing.assert_array_equal(gt.nlo4[:nn, :3].cpu().numpy(), ot.nmin)
+        np.testing.assert_array_equal(gt.nhi4[:nn, :3].cpu().numpy(), ot.nmax)
+        hdr = gt.sched[:8].cpu().numpy()
+        assert hdr[2] == sub, hdr
+        if sub == 2048:
+            assert 0 < hdr[1] <= 2048 and hdr[5] == 1, hdr      # one group: the whole top part
+        else:
+            assert hdr[1] > 2048 and hdr[5] > 1, hdr            # groups, then the final top
+    finally:
+        A.call("slx_set_refit_shape", 1024, 2048)   # the library default
 
 
 @pytest.mark.parametrize("mode,pad,cap", [("cell", 0.01, 40), ("tri", 0.01, 40), ("tri", 0.03, 8)])
