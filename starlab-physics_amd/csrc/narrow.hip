@@ -178,6 +178,13 @@ __device__ __forceinline__ void contact(int set, int qv, int3 fv, const float4* 
   }
 }
 
+// Candidates are loaded kCandGroup at a time.  Most vertices of the draped 1M
+// cloth have one to three stored candidates, and the kernel's occupancy is set by
+// its registers: one at a time, with the register budget of five waves per SIMD
+// (a few bytes of spills), gave 19.30-19.32 ms per frame against 19.46-19.52 ms
+// for groups of four at four waves (six waves spilled too much: 20.9 ms).
+constexpr int kCandGroup = 1;
+
 struct QuerySets {
   int nv, nvs, cap;
   const int32_t *c0, *n0, *c1, *n1, *c2, *n2;  // column-major candidate lists [cap][nq]
@@ -203,7 +210,7 @@ __device__ __forceinline__ bool query_of(const QuerySets& Q, int qg, int& set, i
 // REC: the query also stores its contacting pair records for the velocity pass
 // (see slx_collision_pass for the layout).
 template <bool REC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restrict__ fst,
            const float4* __restrict__ y4, const float4* __restrict__ xs4, const float* __restrict__ fixed,
            const float* __restrict__ m_inv, float dhat, float k, int bucket, int32_t* __restrict__ bcnt,
@@ -222,19 +229,19 @@ k_contacts(QuerySets Q, const int32_t* __restrict__ fdy, const int32_t* __restri
   float3 dx = make_float3(0.f, 0.f, 0.f);
   if (query_of(Q, qg, set, qv, cand, ncand, nq)) {
     const int32_t* F = set == 0 ? fst : fdy;
-    // candidates in groups of four: all ids and corner ids are loaded before the
-    // group is processed, in candidate order
-    for (int j0 = 0; j0 < ncand; j0 += 4) {
-      int3 fv[4];
-      int fid[4];
+    // candidates in groups of kCandGroup: all ids and corner ids of a group are
+    // loaded before it is processed, in candidate order
+    for (int j0 = 0; j0 < ncand; j0 += kCandGroup) {
+      int3 fv[kCandGroup];
+      int fid[kCandGroup];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
+      for (int g = 0; g < kCandGroup; ++g) {
         const int f = j0 + g < ncand ? cand[(size_t)(j0 + g) * nq + qv] : -1;
         fid[g] = f;
         fv[g] = f >= 0 ? make_int3(F[3 * f], F[3 * f + 1], F[3 * f + 2]) : make_int3(-1, -1, -1);
       }
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
+      for (int g = 0; g < kCandGroup; ++g) {
         if (j0 + g >= ncand) break;
         if (set == 1 && (fv[g].x == qv || fv[g].y == qv || fv[g].z == qv)) continue;
         Contact c;
