@@ -33,7 +33,7 @@ int check_launch(const char* what) {
 // 21.74 with 32 entries, whose deeper LDS stack costs occupancy; refit subtrees of
 // 1024 leaves: two workgroups per CU overlap one's loads with the other's walk,
 // 19.49 against 20.0 ms per frame with 2048)
-static Knobs g_knobs = {256, 1, 24, 2048, 1024, 2048, 0};
+static Knobs g_knobs = {256, 1, 24, 2048, 1024, 2048, 0, nullptr};
 static thread_local const Knobs* t_knobs = nullptr;
 
 Knobs& knob_defaults() { return g_knobs; }

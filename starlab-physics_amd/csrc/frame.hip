@@ -379,16 +379,33 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
     rc = frame_zero(nv, d->rec_cap, d->ws, after, st);
     if (rc) return rc;
   }
+  // Substeps that only filter the dynamic caches need the tree's leaf boxes from x,
+  // not its internal boxes: the filters test cached leaves, and the few queries
+  // they send to a full walk walk the internal boxes of the last full refit (at the
+  // caches' rebuild, from x_snap) with their box grown by the displacement bound D
+  // since, testing leaves by their current boxes -- the same leaves in the same
+  // order.  Those substeps refit the leaves only.
+  const bool stale_ok = refits && dyn_cached && !every_substep && !ee && !batched(d->dyn) && lines_only &&
+                        (long long)d->dyn.nf * 8 > (long long)nv;
+  bool tree_stale = false;
   for (int s = 0; s < d->n_substeps; ++s) {
     bool refit_pending = false, snapped = false, disp_pending = false, v_done = false;
+    const bool rebuild_now = coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0));
     if (refits && s > 0) {
       // fork: the refit reads x as the previous substep left it (ahead of the
       // snapshot below, which it does not need)
       if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess)
         return fail(SLX_EHIP, "slx_frame: fork failed");
-      rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
-                          d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub,
-                          lines_only);
+      tree_stale = stale_ok && !rebuild_now;
+      if (tree_stale) {
+        rc = tri_leaf_boxes_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.nlo4, d->dyn.nhi4,
+                                 side->s);
+        if (!rc && hipEventRecord(side->sub, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: event failed");
+      } else {
+        rc = tri_refit_impl(d->dyn.nf, d->faces, d->x4, pad, d->dyn.sorted_faces, d->dyn.child_a, d->dyn.child_b,
+                            d->dyn.parent, d->dyn.sched, d->dyn.nlo4, d->dyn.nhi4, d->dyn.quad4, side->s, side->sub,
+                            lines_only);
+      }
       if (rc) return rc;
       disp_pending = dyn_cached && !every_substep;   // measured on the main stream after the first spring pass
       if (hipEventRecord(side->join, side->s) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
@@ -396,7 +413,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       tv_fresh = false;
     }
     if (coll && (s == 0 || (every_st > 0 && s % every_st == 0))) built[0] = false;
-    if (coll && (s == 0 || (d->cache_every > 0 && s % d->cache_every == 0))) {
+    if (rebuild_now) {
       // caches rebuilt in this substep, against x as it is now (the refit above
       // builds the dynamic tree's boxes from this x)
       built[1] = built[2] = false;
@@ -472,6 +489,9 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
           if (!tv_fresh) {                     // obstacle vertices vs the dynamic tree, beside its cache build
             if (refit_pending && hipStreamWaitEvent(side->q, side->join, 0) != hipSuccess)
               return fail(SLX_EHIP, "slx_frame: fork failed");
+            Knobs kst = knobs();
+            kst.stale_disp = tree_stale ? d->dyn.disp : nullptr;
+            KnobScope ks_stale(kst);
             rc2 = query("traverse_tv_st", d->dyn, nvs, d->xs4, d->st.vert_inst, d->st.vert_order, pad, cap, c2, n2,
                        d->status, (slx_stream_t)side->q, &d->cache[2], reuse_t, &built[2], d->dyn.disp, nullptr,
                        "walk_redo_tv_st");
@@ -503,9 +523,14 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       }
       rc = join_refit();                     // the dynamic tree is needed from here on
       if (rc) return rc;
-      rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c1,
-                 n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp, d->own_faces,
-                 "walk_redo_vt_dy");
+      {
+        Knobs kst = knobs();
+        kst.stale_disp = tree_stale ? d->dyn.disp : nullptr;
+        KnobScope ks_stale(kst);
+        rc = query("traverse_vt_dy", d->dyn, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap,
+                   c1, n1, d->status, stream, &d->cache[1], reuse_v, &built[1], d->dyn.disp, d->own_faces,
+                   "walk_redo_vt_dy");
+      }
       if (rc) return rc;
       if (static_after) {
         rc = static_block();

@@ -1359,6 +1359,30 @@ int slx::tri_refit_impl(int nf, const int32_t* faces, const float* x4, float pad
                               (float4*)nhi4, st, after_sub);
 }
 
+// The leaves' boxes alone (leaf k: face sorted[k], its corners' box grown by pad),
+// computed and stored exactly as the subtree refit's first phase does.
+__global__ void __launch_bounds__(256)
+k_leaf_boxes(int nf, const int32_t* __restrict__ faces, const float4* __restrict__ x4, float pad,
+             const int32_t* __restrict__ sorted, float4* __restrict__ nlo, float4* __restrict__ nhi) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const int f = sorted[k];
+  NodeBox B;
+  corner_box(xyz(x4[faces[3 * f]]), xyz(x4[faces[3 * f + 1]]), xyz(x4[faces[3 * f + 2]]), pad, B.lo, B.hi);
+  B.num = 1;
+  store_node<false>(nf - 1 + k, B, f, f, nullptr, nlo, nhi);
+}
+
+int slx::tri_leaf_boxes_impl(int nf, const int32_t* faces, const float* x4, float padding,
+                             const int32_t* sorted_faces, float* nlo4, float* nhi4, hipStream_t st) {
+  SLX_REQUIRE(nf >= 0, "tri_leaf_boxes: nf < 0");
+  if (nf == 0) return SLX_OK;
+  SLX_REQUIRE(faces && x4 && sorted_faces && nlo4 && nhi4, "tri_leaf_boxes: null pointer");
+  SLX_LAUNCH("tri_leaf_boxes", k_leaf_boxes, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, faces,
+             (const float4*)x4, padding, sorted_faces, (float4*)nlo4, (float4*)nhi4);
+  return check_launch("tri_leaf_boxes");
+}
+
 extern "C" int slx_tri_build(int nf, const float* fmin4, const float* fmax4, const float* box6, int32_t* codes,
                              int32_t* sorted_faces, int32_t* child_a, int32_t* child_b, int32_t* parent,
                              int32_t* range_l, int32_t* range_r, int32_t* sched, float* nlo4, float* nhi4,
@@ -1486,6 +1510,7 @@ struct CacheOut {
   float4* q0;              // [nq] by slot (w: the query's own-face count, see OwnFaces)
   int32_t* redo;           // [1 + nq]
   const int4* own;         // [nq][2] by query (nullable): faces each query leaves out of its lists
+  const uint32_t* stale;   // (per-lane walks, nullable) displacement bound of internal boxes older than the leaves'
 };
 
 // Own faces of a cloth vertex queried against its own mesh's tree: up to eight
@@ -1837,6 +1862,15 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
   float3 a, b;
   query_box(q4, qe, q, pad, a, b);
   float3 aw = a, bw = b;                          // the walk's box
+  if (!BUILD && co.stale) {
+    // internal boxes from positions up to D older: the walk's box grows by D (and
+    // a margin for the boxes' rounding), and leaves are tested by their current boxes
+    const float mag = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))),
+                            fmaxf(fabsf(b.y), fabsf(b.z)));
+    const float e = __uint_as_float(*co.stale) + ldexpf(mag, -20);
+    aw = make_float3(a.x - e, a.y - e, a.z - e);
+    bw = make_float3(b.x + e, b.y + e, b.z + e);
+  }
   OwnFaces own;
   own.load(co.own, q);
   if (BUILD) {                                    // point queries only
@@ -1870,7 +1904,7 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
     if (overlap(aw, bw, xyz(lo), xyz(hi))) {
       if (root >= nl - 1) {
         if (BUILD && !own.has(__float_as_int(lo.w))) keep(root);
-        if (!BUILD || overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
+        if ((!BUILD && !co.stale) || overlap(a, b, xyz(lo), xyz(hi))) emit(__float_as_int(lo.w));
       } else {
         stk[(sp++) * kTravBS] = (uint32_t)root;
       }
@@ -1908,6 +1942,7 @@ k_traverse_quad(int nq, const float4* __restrict__ q4, float pad, const int32_t*
           }
           continue;
         }
+        if (!BUILD && co.stale && face >= 0 && !overlap(a, b, xyz(nlo[node]), xyz(nhi[node]))) continue;
         const uint32_t w = face >= 0 ? ((uint32_t)face | kEmit) : (uint32_t)node;
         if (sp < S) stk[(sp++) * kTravBS] = w;
         else flags |= kStatusStackOverflow;
@@ -1979,6 +2014,9 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
   SLX_REQUIRE(!qedges || !roots, "slx_traverse: edge queries of batched trees are not supported");
   SLX_REQUIRE(!n_active || (order && mode == 1), "slx_traverse: a device query count needs a query list");
   SLX_REQUIRE(mode == 0 || mode == 1, "slx_traverse: mode must be 0 (cell) or 1 (triangle)");
+  SLX_REQUIRE(!knobs().stale_disp || (mode == 1 && quad4 && SLX_TRAV_QUAD && !roots &&
+                                      (long long)n_leaves * 8 > (long long)nq),
+              "slx_traverse: boxes older than the leaves need the per-lane 4-wide walk");
   SLX_REQUIRE(nq >= 0 && cap >= 0, "slx_traverse: bad sizes");
   if (nq == 0) return SLX_OK;
   SLX_REQUIRE(q4 && cand_n && (cap == 0 || cand), "slx_traverse: null pointer");
@@ -2002,6 +2040,7 @@ int traverse_launch(const char* name, int mode, int nq, const float* q4, float p
     if (quad4 && SLX_TRAV_QUAD) {
       CacheOut co{};
       co.own = (const int4*)own;
+      co.stale = knobs().stale_disp;
       SLX_LAUNCH(name, (k_traverse_quad<false, kStack>), dim3(active_grid(nq, kTravBS, n_active)), dim3(kTravBS), 0, st, nq,
                  (const float4*)q4, padding, qedges, n_leaves, (const float4*)nlo4, (const float4*)nhi4,
                  (const float4*)quad4, cap, cand, cand_n, status, roots, q_inst, order, n_active, co);
@@ -2186,6 +2225,8 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   // (trees of batched instances are walked from each query's instance root: per lane)
   const bool wave = !roots && SLX_TRAV_WAVE && (long long)n_leaves * 8 <= (long long)nq;
   SLX_REQUIRE(wave || quad4, "slx_traverse_cached: the triangle tree needs its 4-wide node array");
+  SLX_REQUIRE(!knobs().stale_disp || (kind != 1 && !wave && !roots),
+              "slx_traverse_cached: boxes older than the leaves need the per-lane walk, and no cache build");
   if (kind == 0 && wave)                      // the wavefront walk keeps its stack in registers
     return traverse_launch(name, 1, nq, q4, padding, n_leaves, nullptr, nullptr, nlo4, nhi4, quad4, nullptr,
                            nullptr, nullptr, cap, cand, cand_n, status, st, nullptr, nullptr, order, n_active, qedges);
@@ -2195,7 +2236,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
   // by default: on the 1M-vertex drape its cache build took 1018 us against the
   // per-lane walk's 366 us -- the shared walk is a long chain of dependent loads)
   const bool quad_wave =
-      !wave && !roots && knobs().walk_wave && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
+      !wave && !roots && knobs().walk_wave && !knobs().stale_disp && !qedges && !n_active && (long long)nq * 4 >= n_leaves;
   int rc = SLX_OK;
   if (kind == 1) {
     const CacheOut co{padding + c->margin, c->cap, c->leaves, c->count, (float4*)c->q0, c->redo, (const int4*)own};
@@ -2218,6 +2259,7 @@ int traverse_cached(const char* name, int kind, int nq, const float* q4, float p
     CacheOut co{};
     co.redo = c->redo;
     co.own = (const int4*)own;
+    co.stale = knobs().stale_disp;
     if (quad_wave)
       SLX_LAUNCH(name, k_traverse_quad_wave<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, nq,
                  (const float4*)q4, padding, nullptr, n_leaves, (const float4*)nlo4, (const float4*)nhi4,

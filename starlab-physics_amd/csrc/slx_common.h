@@ -213,6 +213,10 @@ int tri_refit_impl(int nf, const int32_t* faces, const float* x4, float padding,
                    const int32_t* child_a, const int32_t* child_b, const int32_t* parent, const int32_t* sched,
                    float* nlo4, float* nhi4, float* quad4, hipStream_t st, hipEvent_t after_sub,
                    bool lines_only = false);
+// The leaves' boxes of a triangle tree from x (the refit's first phase alone): the
+// internal boxes keep the positions of the last refit.
+int tri_leaf_boxes_impl(int nf, const int32_t* faces, const float* x4, float padding, const int32_t* sorted_faces,
+                        float* nlo4, float* nhi4, hipStream_t st);
 // slx_tree_displacement; zero = false keeps disp as it is before the max (a
 // snapshot taken in the same stream order has zeroed it)
 int tree_displacement(int nv, const float* x4, const float* x_snap, uint32_t* disp, bool zero, hipStream_t st);
@@ -238,6 +242,10 @@ int ee_gate(const slx_edge_sets* ee, int nv, const float4* x4, const float4* y4,
 // descriptor for that call on the calling thread (KnobScope).
 struct Knobs {
   int spring_block, spring_vpt, walk_stack, active_grid, refit_sub, refit_top, walk_wave;
+  // (frame-internal) the dynamic tree's displacement bound when its internal boxes
+  // are older than its leaf boxes: per-lane walks then enlarge their box by it and
+  // test leaves by their current boxes (null: boxes current)
+  const uint32_t* stale_disp;
 };
 Knobs& knob_defaults();
 const Knobs& knobs();
