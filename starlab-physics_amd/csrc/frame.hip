@@ -472,7 +472,8 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       }
       bool static_pending = false, tv_side = false;
       // the static tree's queries and the first set's contacts on the query stream
-      auto static_block = [&]() -> int {
+      // part: 0 everything, 1 the obstacle vertices' query alone, 2 everything else
+      auto static_block = [&](int part) -> int {
         int rc2 = SLX_OK;
         if (nvs > 0) {
           if (hipEventRecord(side->qfork, st) != hipSuccess || hipStreamWaitEvent(side->q, side->qfork, 0) != hipSuccess)
@@ -482,11 +483,13 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
           // us, frame -4.7%); its top part is one workgroup
           if (refit_pending && hipStreamWaitEvent(side->q, side->sub, 0) != hipSuccess)
             return fail(SLX_EHIP, "slx_frame: fork failed");
-          rc2 = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap, c0,
-                     n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_s, &built[0], nullptr, nullptr,
-                     "walk_redo_vt_st");
-          if (rc2) return rc2;
-          if (!tv_fresh) {                     // obstacle vertices vs the dynamic tree, beside its cache build
+          if (part != 1) {
+            rc2 = query("traverse_vt_st", d->st, nv, (const float*)ycur, d->dyn.vert_inst, d->dyn.vert_order, pad, cap,
+                        c0, n0, d->status, (slx_stream_t)side->q, &d->cache[0], reuse_s, &built[0], nullptr, nullptr,
+                        "walk_redo_vt_st");
+            if (rc2) return rc2;
+          }
+          if (part != 2 && !tv_fresh) {        // obstacle vertices vs the dynamic tree, beside its cache build
             if (refit_pending && hipStreamWaitEvent(side->q, side->join, 0) != hipSuccess)
               return fail(SLX_EHIP, "slx_frame: fork failed");
             Knobs kst = knobs();
@@ -499,6 +502,7 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
             if (hipEventRecord(side->tvjoin, side->q) != hipSuccess) return fail(SLX_EHIP, "slx_frame: join failed");
             tv_fresh = tv_side = true;
           }
+          if (part == 1) return rc2;
           rc2 = collision_pass_impl(nv, nvs, d->faces, d->faces_st, (float*)ycur, d->xs4, d->fixed, d->m_inv, d->dhat,
                                    d->k_col, cap, c0, n0, c1, n1, c2, n2, pn, rec ? d->pair_cap : 0,
                                    rec ? d->pair_cnt : nullptr, rec ? d->pair_rec : nullptr, nullptr, d->rec_cap,
@@ -516,9 +520,13 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
       // Later passes of a substep fork it after the cloth's own cache filter: both
       // filters at once each took twice as long, on the critical path (1M drape
       // 19.27-19.33 -> 19.09 ms per frame).
+      // A pass that only filters the cloth's caches (a leaf-only refit) splits it:
+      // the obstacle vertices' query first (the main stream's contacts wait for it),
+      // the rest after the cloth's filter.
       const bool static_after = it > 0;
+      const bool static_split = !static_after && tree_stale;
       if (!static_after) {
-        rc = static_block();
+        rc = static_block(static_split ? 1 : 0);
         if (rc) return rc;
       }
       rc = join_refit();                     // the dynamic tree is needed from here on
@@ -532,8 +540,8 @@ static int frame_impl(const slx_frame_desc* d, slx_stream_t stream) {
                    "walk_redo_vt_dy");
       }
       if (rc) return rc;
-      if (static_after) {
-        rc = static_block();
+      if (static_after || static_split) {
+        rc = static_block(static_split ? 2 : 0);
         if (rc) return rc;
       }
       if (tv_side && hipStreamWaitEvent(st, side->tvjoin, 0) != hipSuccess)
